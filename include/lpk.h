@@ -1,0 +1,228 @@
+/*
+ * lpk.h -- C ABI of the MI355X (gfx950) IQ-generation backend.
+ *
+ * Drop-in boundary for lipika's per-chunk IQ path. The reference has no FFI;
+ * its named slot for a backend is `enum Backend { Cpu }`
+ * (src/backend/mod.rs:14-18, selected by `--backend`, src/bin/lipika.rs:25-27),
+ * and the chunk interface it would bind is internal Rust:
+ *   - scene setup        Transmitter::from_config      src/pipeline.rs:312-341
+ *                        Receiver::from_config         src/pipeline.rs:352-364
+ *   - one chunk body     generate_band, loop body      src/pipeline.rs:915-983
+ *                        (clock, fan-out via gated_contribution :1033-1079,
+ *                         ordered sum :965-976, AWGN :979, post-sum :983)
+ *   - per-transmitter    accumulate_transmitter_contribution
+ *                                                      src/channel/mod.rs:138-156
+ *
+ * Conventions:
+ *   - every function returns 0 (LPK_OK) or a negative lpk_status; nothing
+ *     aborts and no C++ exception crosses the boundary;
+ *   - all pointers in descriptors are host pointers unless named `*_dev`;
+ *     table contents are copied by lpk_scene_upload, so the caller may free
+ *     them on return;
+ *   - a context owns one HIP stream and is not thread-safe: one host thread per
+ *     context, any number of contexts per process;
+ *   - lpk_render_chunk is asynchronous on the context's stream; lpk_sync waits.
+ * Host language note: the reference is Rust and no Rust toolchain exists in
+ * this build, so the host side above this ABI is C++ / Python; a Rust
+ * `extern "C"` block over this header is the intended binding.
+ */
+#ifndef LPK_H
+#define LPK_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LPK_ABI_VERSION 1
+
+typedef enum {
+    LPK_OK = 0,
+    LPK_E_INVALID = -1,    /* bad argument or descriptor */
+    LPK_E_NOMEM = -2,      /* host or device allocation failed */
+    LPK_E_HIP = -3,        /* a HIP runtime call failed */
+    LPK_E_NONFINITE = -4,  /* a non-finite input where a finite one is required */
+    LPK_E_STATE = -5,      /* call order (e.g. render before scene upload) */
+    LPK_E_NODEVICE = -6    /* no usable gfx950 device */
+} lpk_status;
+
+/* src/signal/mod.rs:27-37 and the five implementations */
+typedef enum {
+    LPK_SIG_CW = 0,        /* src/signal/cw.rs:37-53 */
+    LPK_SIG_BPSK = 1,      /* src/signal/bpsk.rs:58-71 */
+    LPK_SIG_OOK = 2,       /* src/signal/ook.rs:60-74 */
+    LPK_SIG_QAM16 = 3,     /* src/signal/qam16.rs:96-109 */
+    LPK_SIG_CA = 4         /* src/signal/ca_code.rs:167-193 */
+} lpk_signal_kind;
+
+/* src/signal/ca_code.rs:112-121 */
+typedef enum { LPK_NAV_OFF = 0, LPK_NAV_DUMMY = 1, LPK_NAV_FULL = 2 } lpk_nav_mode;
+
+/* AnyTrajectory::{Stationary, Precomputed}, src/geometry/mod.rs:25-55 */
+typedef enum { LPK_TRAJ_STATIONARY = 0, LPK_TRAJ_GRID = 1 } lpk_traj_kind;
+
+/* PostOp, src/rx/mod.rs:62-79 */
+typedef enum { LPK_POST_GAIN = 0, LPK_POST_NONLINEARITY = 1, LPK_POST_ADC = 2 } lpk_post_kind;
+
+/* Arithmetic form of a render. */
+typedef enum {
+    LPK_FORM_PLAIN = 0,      /* the reference's f64 path, operation for operation */
+    LPK_FORM_PRECISION = 1   /* f64 geometry and phase, f32 signal path, compensated sum */
+} lpk_form;
+
+/* One transmitter, as Transmitter::from_config leaves it (src/pipeline.rs:297-341). */
+typedef struct {
+    /* trajectory: Stationary (src/geometry/stationary.rs:21-24) or the 10 ms
+     * PrecomputedTrajectory grid (src/geometry/precomputed.rs:23-70) */
+    int32_t traj_kind;           /* lpk_traj_kind */
+    int32_t signal_kind;         /* lpk_signal_kind */
+    double pos_ecef[3];          /* STATIONARY: ECEF metres */
+    double grid_t0_unix;         /* GRID: absolute time of node 0 (scene epoch - 0.5 s) */
+    double grid_dt_s;            /* GRID: node spacing */
+    uint64_t grid_n;             /* GRID: node count, >= 2 */
+    const double *grid_xyz;      /* GRID: (3, n) row-major, as ndarray Array2 (3, N) */
+    double carrier_hz;           /* SignalModel::carrier_hz */
+    double tx_power_w;           /* 10^(dBW/10), src/pipeline.rs:338 */
+    double bandwidth_hz;         /* effective occupied bandwidth, src/config/mod.rs:191-193 */
+    /* CW */
+    double baseband_offset_hz;
+    int32_t cw_anchored, _pad0;
+    double anchor_phase_rad;     /* ContinuousWave::anchor_phase */
+    /* BPSK / OOK / QAM16 (framed) */
+    double rate_hz;              /* bit or symbol rate */
+    double epoch_rel_s;          /* symbol-0 (framed) or chip-0 (C/A) tag, scene-epoch relative */
+    int32_t repeating, _pad1;    /* repeat_from: None (0) / Some (1) */
+    int64_t repeat_from;
+    uint64_t n_symbols;
+    const int8_t *bits;          /* BPSK +-1 or OOK 0/1, n_symbols entries */
+    const float *symbols;        /* QAM16: n_symbols (re, im) pairs */
+    /* C/A */
+    int32_t nav_mode;            /* lpk_nav_mode */
+    int32_t prn;
+    const int8_t *ca_code;       /* 1023 chips, +-1 (lpk_ca_code) */
+    uint64_t nav_seed;           /* DUMMY: seed already folded with the PRN (lpk_nav_fold_seed) */
+    const int8_t *nav_bits;      /* FULL: +-1 LNAV bits at 50 bps */
+    uint64_t n_nav_bits;
+    /* burst gate, src/signal/gate.rs:21-104 */
+    int32_t gated;
+    uint32_t n_bursts;
+    double gate_period_s;
+    double gate_phase_s;         /* in [0, period) */
+    const double *bursts;        /* n_bursts (start, end) pairs, end exclusive */
+} lpk_tx_desc;
+
+/* One stationary receiver with its clock (src/pipeline.rs:344-364,
+ * NodeClockSpec src/clock/mod.rs:20-46 without flicker). */
+typedef struct {
+    double pos_ecef[3];
+    double allan_deviation_1s;   /* white-FM sigma_y(1 s); 0 = no walk */
+    double frequency_offset_y0;
+    double initial_time_offset_s;
+    double master_rate_hz;       /* highest band rate of the receiver, src/pipeline.rs:864-866 */
+    uint64_t clock_seed;         /* src/pipeline.rs:825-827 */
+} lpk_rx_desc;
+
+typedef struct {
+    int32_t kind;                /* lpk_post_kind */
+    int32_t bits;                /* ADC bits */
+    double value;                /* GAIN: dB; NONLINEARITY: iip3 W; ADC: full-scale V */
+} lpk_post_op;
+
+/* One chunk [i0, i0 + clen) of one receiver band (src/pipeline.rs:911-990). */
+typedef struct {
+    uint32_t rx;                 /* receiver index */
+    uint32_t n_active;
+    const uint32_t *active;      /* in-band transmitter indices, in summation order */
+    double fs_hz;                /* band sample rate */
+    double lo_hz;                /* band LO */
+    double epoch_unix;           /* scene epoch */
+    uint64_t i0;                 /* global index of the chunk's first sample */
+    uint64_t clen;               /* samples in the chunk */
+    const double *delta_t;       /* optional host array [clen]: receiver clock offset per
+                                  * sample (the caller's own ClockStream); NULL = device clock */
+    double noise_power_w;        /* kTB or override; 0 = no AWGN */
+    uint64_t noise_seed;         /* src/pipeline.rs:828-830 */
+    uint32_t n_post, _pad;
+    lpk_post_op post[4];         /* RxChain::post_sum in order */
+    void *out_dev;               /* device complex64 [clen]; NULL = internal staging */
+    void *out_host;              /* optional host complex64 [clen], filled by the render */
+    int16_t *adc_codes_host;     /* optional host int16 [2*clen]: ADC integer levels */
+} lpk_band_chunk;
+
+typedef struct lpk_ctx lpk_ctx;
+
+/* ---- context (the Backend slot, src/backend/mod.rs:14-18) ---- */
+int lpk_abi_version(void);
+/* Creates a context on HIP device `hip_device` with its own stream. */
+int lpk_ctx_create(int hip_device, lpk_ctx **out);
+void lpk_ctx_destroy(lpk_ctx *ctx);
+/* Text of the last error on this context (or of the last failed create when
+ * ctx is NULL). Valid until the next call on the context. */
+const char *lpk_last_error(const lpk_ctx *ctx);
+
+/* ---- scene upload (Transmitter::from_config / Receiver::from_config,
+ *      src/pipeline.rs:312-364): copies grids, tables and constants to HBM
+ *      once; replaces any previous scene of the context. ---- */
+int lpk_scene_upload(lpk_ctx *ctx, const lpk_tx_desc *tx, uint32_t n_tx,
+                     const lpk_rx_desc *rx, uint32_t n_rx);
+
+/* ---- chunk render (the chunk body of generate_band, src/pipeline.rs:915-983):
+ *      every job of the call runs in one kernel launch. Asynchronous. ---- */
+int lpk_render_chunk(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form);
+int lpk_sync(lpk_ctx *ctx);
+
+/* Diagnostic render of one job that also records, for active slot `slot`,
+ * the solved emission time, range (src/channel/mod.rs:97-98), f64 carrier
+ * phase (:257-258) and the clock offset per sample (host arrays of clen
+ * doubles; any may be NULL). Synchronous. Same kernels as lpk_render_chunk. */
+int lpk_render_taps(lpk_ctx *ctx, const lpk_band_chunk *job, int form, uint32_t slot,
+                    double *t_emit, double *range, double *phase, double *delta_t);
+
+/* Time of the last lpk_render_chunk kernel on the context's stream, in ms,
+ * measured with HIP events around the launch (valid after lpk_sync). */
+int lpk_last_kernel_ms(lpk_ctx *ctx, double *ms);
+
+/* ---- device memory helpers for callers that keep output in HBM ---- */
+int lpk_device_alloc(lpk_ctx *ctx, size_t bytes, void **dev);
+int lpk_device_free(lpk_ctx *ctx, void *dev);
+int lpk_copy_to_host(lpk_ctx *ctx, void *host, const void *dev, size_t bytes);
+/* FNV-1a 64 of `bytes` bytes of device memory (the hashing sink of
+ * examples/sky_throughput.rs:73-82), computed on the host after a copy. */
+int lpk_device_fnv1a(lpk_ctx *ctx, const void *dev, size_t bytes, uint64_t *hash);
+
+/* ---- host-side table builders used by scene setup (pure host code) ---- */
+/* src/geometry/ecef.rs:10-20 */
+void lpk_geodetic_to_ecef(double lat_deg, double lon_deg, double h_m, double out[3]);
+/* PrecomputedTrajectory::sample over a CircularKeplerian orbit
+ * (src/geometry/precomputed.rs:47-70, src/geometry/kepler.rs:27-85).
+ * lpk_grid_size gives n; xyz must hold 3*n doubles. */
+uint64_t lpk_grid_size(double duration_s, double grid_dt_s);
+int lpk_grid_fill_kepler(double a_m, double inc_deg, double raan_deg, double nu0_deg,
+                         double orbit_epoch_unix, double scene_epoch_unix, double duration_s,
+                         double grid_dt_s, double *t0_out, double *xyz, uint64_t n);
+/* src/signal/ca_code.rs:66-78 */
+int lpk_ca_code(int prn, int8_t out[1023]);
+/* src/signal/ca_code.rs:146-151 */
+uint64_t lpk_nav_fold_seed(uint64_t nav_seed, uint64_t prn);
+/* BpskFrame/OokFrame/Qam16Frame::random (src/signal/bpsk.rs:27-41,
+ * ook.rs:26-44, qam16.rs:62-80): sync word then seeded payload. */
+int lpk_payload_bpsk(const int8_t *sync, uint64_t n_sync, uint64_t n_payload, uint64_t seed, int8_t *out);
+int lpk_payload_ook(const uint8_t *sync, uint64_t n_sync, uint64_t n_payload, uint64_t seed, int8_t *out);
+int lpk_payload_qam16(const uint8_t *sync_bits, uint64_t n_sync_bits, uint64_t n_payload,
+                      uint64_t seed, float *out_iq);
+/* src/signal/mod.rs:50-101, src/signal/cw.rs:26-34 */
+int lpk_elapsed_periods(double elapsed_s, double rate_hz, int64_t *whole, double *frac);
+int lpk_frame_anchor(double render_epoch, double signal_epoch, double rate_hz, uint64_t n_symbols,
+                     double *epoch_rel_s, int64_t *symbol_offset);
+int lpk_cw_anchor_phase(double render_epoch, double signal_epoch, double offset_hz, double *phase);
+/* src/pipeline.rs:70-75 */
+int lpk_bands_overlap(double tx_carrier_hz, double tx_bw_hz, double band_lo_hz, double band_bw_hz);
+/* src/channel/mod.rs:310-312 */
+double lpk_thermal_noise_power_w(double temperature_k, double bandwidth_hz);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

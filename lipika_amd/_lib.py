@@ -1,0 +1,153 @@
+"""ctypes binding of include/lpk.h (the HIP backend's C ABI).
+
+The library is ``lipika_amd/liblpk.so``, built in place by
+``python -m lipika_amd.build``. There is no fallback: if the library is
+missing or the device is not a gfx950, every render raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblpk.so")
+
+LPK_OK = 0
+STATUS = {
+    0: "LPK_OK", -1: "LPK_E_INVALID", -2: "LPK_E_NOMEM", -3: "LPK_E_HIP",
+    -4: "LPK_E_NONFINITE", -5: "LPK_E_STATE", -6: "LPK_E_NODEVICE",
+}
+SIG_CW, SIG_BPSK, SIG_OOK, SIG_QAM16, SIG_CA = range(5)
+NAV_OFF, NAV_DUMMY, NAV_FULL = range(3)
+TRAJ_STATIONARY, TRAJ_GRID = range(2)
+POST_GAIN, POST_NONLINEARITY, POST_ADC = range(3)
+FORM_PLAIN, FORM_PRECISION = 0, 1
+
+_d = C.c_double
+_u64 = C.c_uint64
+_i64 = C.c_int64
+_i32 = C.c_int32
+_u32 = C.c_uint32
+
+
+class TxDesc(C.Structure):
+    _fields_ = [
+        ("traj_kind", _i32), ("signal_kind", _i32),
+        ("pos_ecef", _d * 3),
+        ("grid_t0_unix", _d), ("grid_dt_s", _d), ("grid_n", _u64),
+        ("grid_xyz", C.POINTER(_d)),
+        ("carrier_hz", _d), ("tx_power_w", _d), ("bandwidth_hz", _d),
+        ("baseband_offset_hz", _d), ("cw_anchored", _i32), ("_pad0", _i32),
+        ("anchor_phase_rad", _d),
+        ("rate_hz", _d), ("epoch_rel_s", _d),
+        ("repeating", _i32), ("_pad1", _i32), ("repeat_from", _i64),
+        ("n_symbols", _u64),
+        ("bits", C.POINTER(C.c_int8)), ("symbols", C.POINTER(C.c_float)),
+        ("nav_mode", _i32), ("prn", _i32),
+        ("ca_code", C.POINTER(C.c_int8)), ("nav_seed", _u64),
+        ("nav_bits", C.POINTER(C.c_int8)), ("n_nav_bits", _u64),
+        ("gated", _i32), ("n_bursts", _u32),
+        ("gate_period_s", _d), ("gate_phase_s", _d),
+        ("bursts", C.POINTER(_d)),
+    ]
+
+
+class RxDesc(C.Structure):
+    _fields_ = [
+        ("pos_ecef", _d * 3),
+        ("allan_deviation_1s", _d), ("frequency_offset_y0", _d),
+        ("initial_time_offset_s", _d), ("master_rate_hz", _d),
+        ("clock_seed", _u64),
+    ]
+
+
+class PostOp(C.Structure):
+    _fields_ = [("kind", _i32), ("bits", _i32), ("value", _d)]
+
+
+class BandChunk(C.Structure):
+    _fields_ = [
+        ("rx", _u32), ("n_active", _u32), ("active", C.POINTER(_u32)),
+        ("fs_hz", _d), ("lo_hz", _d), ("epoch_unix", _d),
+        ("i0", _u64), ("clen", _u64),
+        ("delta_t", C.POINTER(_d)),
+        ("noise_power_w", _d), ("noise_seed", _u64),
+        ("n_post", _u32), ("_pad", _u32),
+        ("post", PostOp * 4),
+        ("out_dev", C.c_void_p), ("out_host", C.c_void_p),
+        ("adc_codes_host", C.POINTER(C.c_int16)),
+    ]
+
+
+class LpkError(RuntimeError):
+    """A non-zero lpk_status, with the library's error text."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_LIB = None
+
+
+def load() -> C.CDLL:
+    """Load liblpk.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m lipika_amd.build`")
+    lib = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    sig = {
+        "lpk_abi_version": (_i32, []),
+        "lpk_ctx_create": (_i32, [C.c_int, P(C.c_void_p)]),
+        "lpk_ctx_destroy": (None, [C.c_void_p]),
+        "lpk_last_error": (C.c_char_p, [C.c_void_p]),
+        "lpk_scene_upload": (_i32, [C.c_void_p, P(TxDesc), _u32, P(RxDesc), _u32]),
+        "lpk_render_chunk": (_i32, [C.c_void_p, P(BandChunk), _u32, C.c_int]),
+        "lpk_sync": (_i32, [C.c_void_p]),
+        "lpk_render_taps": (_i32, [C.c_void_p, P(BandChunk), C.c_int, _u32, P(_d), P(_d), P(_d), P(_d)]),
+        "lpk_last_kernel_ms": (_i32, [C.c_void_p, P(_d)]),
+        "lpk_device_alloc": (_i32, [C.c_void_p, C.c_size_t, P(C.c_void_p)]),
+        "lpk_device_free": (_i32, [C.c_void_p, C.c_void_p]),
+        "lpk_copy_to_host": (_i32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
+        "lpk_device_fnv1a": (_i32, [C.c_void_p, C.c_void_p, C.c_size_t, P(_u64)]),
+        "lpk_geodetic_to_ecef": (None, [_d, _d, _d, P(_d)]),
+        "lpk_grid_size": (_u64, [_d, _d]),
+        "lpk_grid_fill_kepler": (_i32, [_d, _d, _d, _d, _d, _d, _d, _d, P(_d), P(_d), _u64]),
+        "lpk_ca_code": (_i32, [C.c_int, P(C.c_int8)]),
+        "lpk_nav_fold_seed": (_u64, [_u64, _u64]),
+        "lpk_payload_bpsk": (_i32, [P(C.c_int8), _u64, _u64, _u64, P(C.c_int8)]),
+        "lpk_payload_ook": (_i32, [P(C.c_uint8), _u64, _u64, _u64, P(C.c_int8)]),
+        "lpk_payload_qam16": (_i32, [P(C.c_uint8), _u64, _u64, _u64, P(C.c_float)]),
+        "lpk_elapsed_periods": (_i32, [_d, _d, P(_i64), P(_d)]),
+        "lpk_frame_anchor": (_i32, [_d, _d, _d, _u64, P(_d), P(_i64)]),
+        "lpk_cw_anchor_phase": (_i32, [_d, _d, _d, P(_d)]),
+        "lpk_bands_overlap": (_i32, [_d, _d, _d, _d]),
+        "lpk_thermal_noise_power_w": (_d, [_d, _d]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _LIB = lib
+    return lib
+
+
+# Every symbol include/lpk.h declares (checked by the CPU tests).
+EXPORTED = (
+    "lpk_abi_version", "lpk_ctx_create", "lpk_ctx_destroy", "lpk_last_error",
+    "lpk_scene_upload", "lpk_render_chunk", "lpk_sync", "lpk_render_taps",
+    "lpk_last_kernel_ms", "lpk_device_alloc", "lpk_device_free", "lpk_copy_to_host",
+    "lpk_device_fnv1a", "lpk_geodetic_to_ecef", "lpk_grid_size", "lpk_grid_fill_kepler",
+    "lpk_ca_code", "lpk_nav_fold_seed", "lpk_payload_bpsk", "lpk_payload_ook",
+    "lpk_payload_qam16", "lpk_elapsed_periods", "lpk_frame_anchor", "lpk_cw_anchor_phase",
+    "lpk_bands_overlap", "lpk_thermal_noise_power_w",
+)
+
+
+def check(rc: int, ctx=None) -> None:
+    if rc != LPK_OK:
+        msg = load().lpk_last_error(ctx)
+        raise LpkError(rc, msg.decode() if msg else "")

@@ -1,0 +1,653 @@
+// lpk_api.hip -- the C ABI of include/lpk.h: contexts, scene upload, the
+// per-call job table and the launches. No exception leaves this file.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/lpk.h"
+#include "lpk_device.hpp"
+
+namespace lpk {
+hipError_t launch_render(int form, const JobDev *jobs, int n_jobs, const LinkDev *links,
+                         const TxDev *txs, uint32_t n_tiles, double *tap_t_emit, double *tap_range,
+                         double *tap_phase, double *tap_delta_t, uint32_t tap_slot,
+                         hipStream_t stream);
+hipError_t launch_walk_extend(uint64_t seed, double sigma, int64_t first_block, int64_t n_blocks,
+                              double *base, double *blocksum_scratch, hipStream_t stream);
+hipError_t launch_walk_materialize(uint64_t seed, double sigma, int64_t k_lo, int64_t k_hi,
+                                   const double *base, double *out, hipStream_t stream);
+}  // namespace lpk
+
+using namespace lpk;
+
+namespace {
+
+std::string g_create_error;
+
+struct Fail {
+    int code;
+    std::string msg;
+};
+
+// Growable device buffer (never shrinks; reallocation waits for the stream).
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    void *ensure(size_t bytes, hipStream_t s) {
+        if (bytes <= cap && p) return p;
+        size_t want = std::max<size_t>(bytes, 256);
+        if (p) {
+            (void)hipStreamSynchronize(s);
+            (void)hipFree(p);
+            p = nullptr;
+            cap = 0;
+        }
+        if (hipMalloc(&p, want) != hipSuccess) {
+            p = nullptr;
+            throw Fail{LPK_E_NOMEM, "device allocation of " + std::to_string(want) + " bytes failed"};
+        }
+        cap = want;
+        return p;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+// Pinned host staging slot, reusable once its event has completed.
+struct PinSlot {
+    void *p = nullptr;
+    size_t cap = 0;
+    hipEvent_t done = nullptr;
+};
+
+struct RxState {
+    lpk_rx_desc d{};
+    int walk = 0;
+    double sigma_step = 0.0;
+    double *base = nullptr;     // base[b] = eps(b*kWalkBlock - 1), base[0] = 0
+    int64_t base_cap = 0;       // entries
+    int64_t nb_known = 0;       // base[0..nb_known] valid
+    DevBuf eps;                 // materialised window for the current call
+};
+
+}  // namespace
+
+struct lpk_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+    // scene
+    std::vector<void *> scene_allocs;
+    std::vector<TxDev> tx_host;
+    std::vector<double> tx_carrier;
+    TxDev *tx_dev = nullptr;
+    uint32_t n_tx = 0;
+    std::vector<RxState> rx;
+    bool have_scene = false;
+    bool timed = false;
+    // per-call buffers
+    DevBuf jobs, links, dt_stage, out_stage, codes_stage, blocksum;
+    PinSlot pin[4];
+    int pin_next = 0;
+};
+
+namespace {
+
+void hip_check(hipError_t e, const char *what) {
+    if (e != hipSuccess) throw Fail{LPK_E_HIP, std::string(what) + ": " + hipGetErrorString(e)};
+}
+
+int fail(lpk_ctx *ctx, const Fail &f) {
+    if (ctx) ctx->err = f.msg;
+    return f.code;
+}
+
+template <class F>
+int guarded(lpk_ctx *ctx, F &&body) {
+    try {
+        body();
+        return LPK_OK;
+    } catch (const Fail &f) {
+        return fail(ctx, f);
+    } catch (const std::bad_alloc &) {
+        return fail(ctx, Fail{LPK_E_NOMEM, "host allocation failed"});
+    } catch (...) {
+        return fail(ctx, Fail{LPK_E_INVALID, "unexpected internal error"});
+    }
+}
+
+void *upload(lpk_ctx *ctx, const void *host, size_t bytes) {
+    if (!bytes) return nullptr;
+    void *d = nullptr;
+    if (hipMalloc(&d, bytes) != hipSuccess) throw Fail{LPK_E_NOMEM, "scene table allocation failed"};
+    ctx->scene_allocs.push_back(d);
+    hip_check(hipMemcpy(d, host, bytes, hipMemcpyHostToDevice), "scene table copy");
+    return d;
+}
+
+void free_scene(lpk_ctx *ctx) {
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (void *p : ctx->scene_allocs) (void)hipFree(p);
+    ctx->scene_allocs.clear();
+    for (auto &r : ctx->rx) {
+        if (r.base) (void)hipFree(r.base);
+        r.base = nullptr;
+        r.eps.release();
+    }
+    ctx->rx.clear();
+    ctx->tx_host.clear();
+    ctx->tx_carrier.clear();
+    ctx->tx_dev = nullptr;
+    ctx->n_tx = 0;
+    ctx->have_scene = false;
+}
+
+bool is_fin(double x) { return std::isfinite(x); }
+
+TxDev make_tx(lpk_ctx *ctx, const lpk_tx_desc &d, uint32_t idx) {
+    auto bad = [&](const char *why) {
+        return Fail{LPK_E_INVALID, "transmitter " + std::to_string(idx) + ": " + why};
+    };
+    TxDev T{};
+    T.traj_kind = d.traj_kind;
+    T.signal_kind = d.signal_kind;
+    if (d.traj_kind == LPK_TRAJ_STATIONARY) {
+        for (int k = 0; k < 3; ++k) T.pos[k] = d.pos_ecef[k];
+    } else if (d.traj_kind == LPK_TRAJ_GRID) {
+        if (d.grid_n < 2 || !d.grid_xyz) throw bad("grid needs at least 2 nodes and a position table");
+        if (!(d.grid_dt_s > 0.0) || !is_fin(d.grid_dt_s) || !is_fin(d.grid_t0_unix))
+            throw Fail{LPK_E_NONFINITE, "transmitter " + std::to_string(idx) + ": grid start/step"};
+        T.grid_t0 = d.grid_t0_unix;
+        T.grid_dt = d.grid_dt_s;
+        T.grid_n = int64_t(d.grid_n);
+        T.grid = static_cast<const double *>(upload(ctx, d.grid_xyz, 3 * d.grid_n * sizeof(double)));
+    } else {
+        throw bad("unknown trajectory kind");
+    }
+    if (!is_fin(d.carrier_hz) || !(d.carrier_hz > 0.0)) throw bad("carrier_hz must be positive and finite");
+    if (!is_fin(d.tx_power_w) || d.tx_power_w < 0.0) throw bad("tx_power_w must be finite and >= 0");
+    T.carrier = d.carrier_hz;
+    // src/channel/mod.rs:208-210, :239-243
+    const double wavelength_m = kSpeedOfLight / d.carrier_hz;
+    const double tx_amplitude_v = std::sqrt(d.tx_power_w);
+    T.neg_tau_src = -kTau * d.carrier_hz;
+    T.amp_factor = (wavelength_m / (2.0 * kTau)) * tx_amplitude_v;
+    switch (d.signal_kind) {
+    case LPK_SIG_CW:
+        T.omega = kTau * d.baseband_offset_hz;  // src/signal/cw.rs:42
+        T.cw_anchored = d.cw_anchored ? 1 : 0;
+        T.anchor_phase = d.anchor_phase_rad;
+        break;
+    case LPK_SIG_BPSK:
+    case LPK_SIG_OOK:
+    case LPK_SIG_QAM16:
+        T.rate = d.rate_hz;
+        T.epoch_rel = d.epoch_rel_s;
+        T.repeating = d.repeating ? 1 : 0;
+        T.repeat_from = d.repeat_from;
+        T.n_symbols = int64_t(d.n_symbols);
+        if (d.n_symbols) {
+            if (d.signal_kind == LPK_SIG_QAM16) {
+                if (!d.symbols) throw bad("qam16 needs its symbol table");
+                T.symbols = static_cast<const float *>(upload(ctx, d.symbols, 2 * d.n_symbols * sizeof(float)));
+            } else {
+                if (!d.bits) throw bad("bpsk/ook needs its bit table");
+                T.bits = static_cast<const int8_t *>(upload(ctx, d.bits, d.n_symbols));
+            }
+        }
+        break;
+    case LPK_SIG_CA:
+        if (!d.ca_code) throw bad("ca_code needs its 1023-chip code");
+        T.epoch_rel = d.epoch_rel_s;
+        T.nav_mode = d.nav_mode;
+        if (d.nav_mode < LPK_NAV_OFF || d.nav_mode > LPK_NAV_FULL) throw bad("unknown nav mode");
+        T.ca_code = static_cast<const int8_t *>(upload(ctx, d.ca_code, 1023));
+        T.nav_seed = d.nav_seed;
+        if (d.nav_mode == LPK_NAV_FULL && d.n_nav_bits) {
+            if (!d.nav_bits) throw bad("full nav needs its bit stream");
+            T.nav_bits = static_cast<const int8_t *>(upload(ctx, d.nav_bits, d.n_nav_bits));
+            T.n_nav_bits = int64_t(d.n_nav_bits);
+        }
+        break;
+    default:
+        throw bad("unknown signal kind");
+    }
+    if (d.gated) {
+        // src/signal/gate.rs:30-52
+        if (!(d.gate_period_s > 0.0) || !is_fin(d.gate_period_s)) throw bad("gate period must be positive");
+        if (d.n_bursts && !d.bursts) throw bad("gate bursts missing");
+        for (uint32_t b = 0; b < d.n_bursts; ++b) {
+            const double s = d.bursts[2 * b], e = d.bursts[2 * b + 1];
+            if (!(s >= 0.0) || !(e > s) || !(e <= d.gate_period_s)) throw bad("gate burst outside its period");
+        }
+        T.gated = 1;
+        T.n_bursts = int32_t(d.n_bursts);
+        T.gate_period = d.gate_period_s;
+        T.gate_phase = d.gate_phase_s;
+        T.bursts = static_cast<const double *>(upload(ctx, d.bursts, 2 * d.n_bursts * sizeof(double)));
+    }
+    return T;
+}
+
+void *pinned_slot(lpk_ctx *ctx, size_t bytes) {
+    PinSlot &s = ctx->pin[ctx->pin_next];
+    ctx->pin_next = (ctx->pin_next + 1) & 3;
+    if (s.done) hip_check(hipEventSynchronize(s.done), "staging reuse");
+    if (bytes > s.cap) {
+        if (s.p) (void)hipHostFree(s.p);
+        s.p = nullptr;
+        s.cap = 0;
+        size_t want = std::max<size_t>(bytes, 4096);
+        if (hipHostMalloc(&s.p, want, 0) != hipSuccess) throw Fail{LPK_E_NOMEM, "pinned staging allocation failed"};
+        s.cap = want;
+    }
+    if (!s.done) hip_check(hipEventCreateWithFlags(&s.done, hipEventDisableTiming), "event create");
+    return s.p;
+}
+void pinned_mark(lpk_ctx *ctx, int slot_used) {
+    hip_check(hipEventRecord(ctx->pin[slot_used].done, ctx->stream), "event record");
+}
+
+// Bring the receiver's walk base table up to block b_hi, then materialise
+// eps over [k_lo, k_hi].
+void prepare_walk(lpk_ctx *ctx, RxState &R, int64_t k_lo, int64_t k_hi) {
+    const int64_t b_hi = k_hi / kWalkBlock;
+    if (b_hi + 1 > R.base_cap) {
+        int64_t cap = std::max<int64_t>(R.base_cap ? R.base_cap : 4096, 4096);
+        while (cap < b_hi + 2) cap *= 2;
+        double *nb = nullptr;
+        if (hipMalloc(&nb, size_t(cap) * sizeof(double)) != hipSuccess)
+            throw Fail{LPK_E_NOMEM, "clock walk table allocation failed"};
+        if (R.base) {
+            hip_check(hipMemcpyAsync(nb, R.base, size_t(R.nb_known + 1) * sizeof(double),
+                                     hipMemcpyDeviceToDevice, ctx->stream), "walk table copy");
+            hip_check(hipStreamSynchronize(ctx->stream), "walk table copy");
+            (void)hipFree(R.base);
+        } else {
+            const double zero = 0.0;
+            hip_check(hipMemcpy(nb, &zero, sizeof(double), hipMemcpyHostToDevice), "walk table init");
+            R.nb_known = 0;
+        }
+        R.base = nb;
+        R.base_cap = cap;
+    }
+    if (R.nb_known < b_hi) {
+        const int64_t nadd = b_hi - R.nb_known;
+        double *bs = static_cast<double *>(ctx->blocksum.ensure(size_t(nadd) * sizeof(double), ctx->stream));
+        hip_check(launch_walk_extend(R.d.clock_seed, R.sigma_step, R.nb_known, nadd, R.base, bs, ctx->stream),
+                  "clock walk extend");
+        R.nb_known = b_hi;
+    }
+    double *eps = static_cast<double *>(R.eps.ensure(size_t(k_hi - k_lo + 1) * sizeof(double), ctx->stream));
+    hip_check(launch_walk_materialize(R.d.clock_seed, R.sigma_step, k_lo, k_hi, R.base, eps, ctx->stream),
+              "clock walk materialise");
+}
+
+struct TapReq {
+    uint32_t slot;
+    double *t_emit, *range, *phase, *delta_t;
+};
+
+void render(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form, const TapReq *tap) {
+    if (!ctx->have_scene) throw Fail{LPK_E_STATE, "no scene uploaded"};
+    if (form != LPK_FORM_PLAIN && form != LPK_FORM_PRECISION) throw Fail{LPK_E_INVALID, "unknown form"};
+    if (n_jobs && !jobs) throw Fail{LPK_E_INVALID, "jobs is NULL"};
+    // ---- validation (src/pipeline.rs:759-767 and the descriptor contracts)
+    uint64_t total_links = 0, dt_total = 0, stage_total = 0, code_total = 0;
+    for (uint32_t q = 0; q < n_jobs; ++q) {
+        const lpk_band_chunk &B = jobs[q];
+        auto bad = [&](int code, const std::string &why) {
+            return Fail{code, "job " + std::to_string(q) + ": " + why};
+        };
+        if (B.rx >= ctx->rx.size()) throw bad(LPK_E_INVALID, "receiver index out of range");
+        if (!(B.fs_hz > 0.0) || !is_fin(B.fs_hz)) throw bad(LPK_E_INVALID, "fs_hz must be positive and finite");
+        if (!is_fin(B.lo_hz) || !is_fin(B.epoch_unix)) throw bad(LPK_E_NONFINITE, "lo_hz / epoch_unix");
+        if (!is_fin(B.noise_power_w) || B.noise_power_w < 0.0) throw bad(LPK_E_INVALID, "noise_power_w");
+        if (B.n_active && !B.active) throw bad(LPK_E_INVALID, "active list is NULL");
+        if (B.clen > (uint64_t(1) << 40) || B.i0 > (uint64_t(1) << 52)) throw bad(LPK_E_INVALID, "chunk too large");
+        for (uint32_t s = 0; s < B.n_active; ++s)
+            if (B.active[s] >= ctx->n_tx) throw bad(LPK_E_INVALID, "active transmitter index out of range");
+        if (B.n_post > 4) throw bad(LPK_E_INVALID, "at most 4 post-sum ops");
+        for (uint32_t p = 0; p < B.n_post; ++p) {
+            const lpk_post_op &P = B.post[p];
+            if (P.kind < LPK_POST_GAIN || P.kind > LPK_POST_ADC) throw bad(LPK_E_INVALID, "unknown post op");
+            if (P.kind == LPK_POST_ADC && (P.bits < 0 || P.bits > 16)) throw bad(LPK_E_INVALID, "ADC bits must be 0..16");
+            if (!is_fin(P.value)) throw bad(LPK_E_NONFINITE, "post op parameter");
+        }
+        if (B.delta_t)
+            for (uint64_t j = 0; j < B.clen; ++j)
+                if (!is_fin(B.delta_t[j])) throw bad(LPK_E_NONFINITE, "delta_t holds a non-finite value");
+        total_links += B.n_active;
+        if (B.delta_t) dt_total += B.clen;
+        if (!B.out_dev) stage_total += B.clen;
+        if (B.adc_codes_host) code_total += B.clen;
+    }
+    if (tap && n_jobs != 1) throw Fail{LPK_E_INVALID, "taps need exactly one job"};
+    if (tap && tap->slot >= jobs[0].n_active) throw Fail{LPK_E_INVALID, "tap slot out of range"};
+
+    // ---- order: heavy jobs first so their tiles are dispatched first
+    std::vector<uint32_t> order(n_jobs);
+    for (uint32_t q = 0; q < n_jobs; ++q) order[q] = q;
+    if (!tap)
+        std::stable_sort(order.begin(), order.end(),
+                         [&](uint32_t a, uint32_t b) { return jobs[a].n_active > jobs[b].n_active; });
+
+    // ---- clock walk windows per receiver
+    std::vector<int64_t> wlo(ctx->rx.size(), INT64_MAX), whi(ctx->rx.size(), -1);
+    for (uint32_t q = 0; q < n_jobs; ++q) {
+        const lpk_band_chunk &B = jobs[q];
+        RxState &R = ctx->rx[B.rx];
+        if (!R.walk || B.delta_t || B.clen == 0) continue;
+        const double M = R.d.master_rate_hz;
+        const double xa = (double(B.i0) / B.fs_hz) * M;
+        const double xb = (double(B.i0 + B.clen - 1) / B.fs_hz) * M;
+        if (!(xb < 4.0e15)) throw Fail{LPK_E_INVALID, "job " + std::to_string(q) + ": clock index range too large"};
+        const int64_t klo = xa <= 0.0 ? 0 : int64_t(std::floor(xa));
+        const int64_t khi = xb <= 0.0 ? 0 : int64_t(std::floor(xb)) + 1;
+        wlo[B.rx] = std::min(wlo[B.rx], klo);
+        whi[B.rx] = std::max(whi[B.rx], khi);
+    }
+    for (size_t r = 0; r < ctx->rx.size(); ++r)
+        if (whi[r] >= 0) prepare_walk(ctx, ctx->rx[r], wlo[r], whi[r]);
+
+    // ---- staging buffers
+    double *dt_dev = dt_total ? static_cast<double *>(ctx->dt_stage.ensure(dt_total * 8, ctx->stream)) : nullptr;
+    float *stage_dev = stage_total ? static_cast<float *>(ctx->out_stage.ensure(stage_total * 8, ctx->stream)) : nullptr;
+    int16_t *codes_dev = code_total ? static_cast<int16_t *>(ctx->codes_stage.ensure(code_total * 4, ctx->stream)) : nullptr;
+
+    // ---- job and link tables
+    const size_t job_bytes = size_t(std::max<uint32_t>(n_jobs, 1)) * sizeof(JobDev);
+    const size_t link_bytes = size_t(std::max<uint64_t>(total_links, 1)) * sizeof(LinkDev);
+    const int slot = ctx->pin_next;
+    char *pin = static_cast<char *>(pinned_slot(ctx, job_bytes + link_bytes));
+    JobDev *jh = reinterpret_cast<JobDev *>(pin);
+    LinkDev *lh = reinterpret_cast<LinkDev *>(pin + job_bytes);
+    JobDev *jd = static_cast<JobDev *>(ctx->jobs.ensure(job_bytes, ctx->stream));
+    LinkDev *ld = static_cast<LinkDev *>(ctx->links.ensure(link_bytes, ctx->stream));
+
+    uint64_t tiles = 0, link_off = 0, dt_off = 0, st_off = 0, cd_off = 0;
+    std::vector<float *> out_ptr(n_jobs, nullptr);
+    std::vector<int16_t *> code_ptr(n_jobs, nullptr);
+    for (uint32_t o = 0; o < n_jobs; ++o) {
+        const uint32_t q = order[o];
+        const lpk_band_chunk &B = jobs[q];
+        const RxState &R = ctx->rx[B.rx];
+        JobDev J{};
+        J.tile0 = uint32_t(tiles);
+        J.n_tiles = uint32_t((B.clen + kTile - 1) / kTile);
+        tiles += J.n_tiles;
+        if (tiles > 0x7FFFFFFFull) throw Fail{LPK_E_INVALID, "render call too large for one launch"};
+        J.link0 = uint32_t(link_off);
+        J.n_active = B.n_active;
+        for (uint32_t s = 0; s < B.n_active; ++s) {
+            LinkDev L{};
+            L.tx = B.active[s];
+            L.tau_if = kTau * (ctx->tx_carrier[L.tx] - B.lo_hz);  // src/channel/mod.rs:240
+            lh[link_off + s] = L;
+        }
+        link_off += B.n_active;
+        J.fs = B.fs_hz;
+        J.lo = B.lo_hz;
+        J.epoch = B.epoch_unix;
+        J.i0 = B.i0;
+        J.clen = B.clen;
+        for (int k = 0; k < 3; ++k) J.obs[k] = R.d.pos_ecef[k];
+        if (B.delta_t) {
+            hip_check(hipMemcpyAsync(dt_dev + dt_off, B.delta_t, B.clen * 8, hipMemcpyHostToDevice, ctx->stream),
+                      "delta_t upload");
+            J.delta_t = dt_dev + dt_off;
+            dt_off += B.clen;
+        }
+        J.c0 = R.d.initial_time_offset_s;
+        J.y0 = R.d.frequency_offset_y0;
+        J.master = R.d.master_rate_hz;
+        J.walk = (R.walk && !B.delta_t) ? 1 : 0;
+        if (J.walk) {
+            J.eps = static_cast<const double *>(R.eps.p);
+            J.eps_k0 = wlo[B.rx];
+        }
+        J.sigma = float(std::sqrt(B.noise_power_w / 2.0));  // src/channel/mod.rs:328
+        J.noise_seed = B.noise_seed;
+        J.n_post = int32_t(B.n_post);
+        for (uint32_t p = 0; p < B.n_post; ++p) {
+            const lpk_post_op &P = B.post[p];
+            PostDev D{};
+            D.kind = P.kind;
+            if (P.kind == LPK_POST_GAIN) {
+                D.a = std::pow(10.0f, float(P.value / 20.0));  // src/rx/mod.rs:86
+            } else if (P.kind == LPK_POST_NONLINEARITY) {
+                D.a = float(4.0 / 3.0 / P.value);  // src/rx/mod.rs:90
+            } else {
+                const float fsv = float(P.value);  // src/rx/mod.rs:94-96
+                const int bits = P.bits < 2 ? 2 : P.bits;
+                const float levels = float((1ULL << (bits - 1)) - 1);
+                D.a = fsv;
+                D.b = fsv / levels;
+            }
+            J.post[p] = D;
+        }
+        if (B.out_dev) {
+            J.out = static_cast<float *>(B.out_dev);
+        } else {
+            J.out = stage_dev + 2 * st_off;
+            st_off += B.clen;
+        }
+        out_ptr[q] = J.out;
+        if (B.adc_codes_host) {
+            J.codes = codes_dev + 2 * cd_off;
+            cd_off += B.clen;
+            code_ptr[q] = J.codes;
+        }
+        jh[o] = J;
+    }
+    hip_check(hipMemcpyAsync(jd, jh, job_bytes, hipMemcpyHostToDevice, ctx->stream), "job table upload");
+    hip_check(hipMemcpyAsync(ld, lh, link_bytes, hipMemcpyHostToDevice, ctx->stream), "link table upload");
+    pinned_mark(ctx, slot);
+
+    if (tiles) {
+        hip_check(hipEventRecord(ctx->ev0, ctx->stream), "event record");
+        hip_check(launch_render(form, jd, int(n_jobs), ld, ctx->tx_dev, uint32_t(tiles),
+                                tap ? tap->t_emit : nullptr, tap ? tap->range : nullptr,
+                                tap ? tap->phase : nullptr, tap ? tap->delta_t : nullptr,
+                                tap ? tap->slot : 0, ctx->stream),
+                  "render launch");
+        hip_check(hipEventRecord(ctx->ev1, ctx->stream), "event record");
+        ctx->timed = true;
+    }
+    for (uint32_t q = 0; q < n_jobs; ++q) {
+        const lpk_band_chunk &B = jobs[q];
+        if (B.out_host && B.clen)
+            hip_check(hipMemcpyAsync(B.out_host, out_ptr[q], B.clen * 8, hipMemcpyDeviceToHost, ctx->stream),
+                      "output download");
+        if (B.adc_codes_host && B.clen)
+            hip_check(hipMemcpyAsync(B.adc_codes_host, code_ptr[q], B.clen * 4, hipMemcpyDeviceToHost, ctx->stream),
+                      "ADC code download");
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int lpk_abi_version(void) { return LPK_ABI_VERSION; }
+
+int lpk_ctx_create(int hip_device, lpk_ctx **out) {
+    if (!out) return LPK_E_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        g_create_error = "no HIP device";
+        return LPK_E_NODEVICE;
+    }
+    if (hip_device < 0 || hip_device >= n) {
+        g_create_error = "device index out of range";
+        return LPK_E_INVALID;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, hip_device) != hipSuccess ||
+        std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        g_create_error = std::string("device is not gfx950: ") + prop.gcnArchName;
+        return LPK_E_NODEVICE;
+    }
+    lpk_ctx *ctx = new (std::nothrow) lpk_ctx();
+    if (!ctx) return LPK_E_NOMEM;
+    ctx->device = hip_device;
+    if (hipSetDevice(hip_device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
+        g_create_error = "HIP stream/event creation failed";
+        delete ctx;
+        return LPK_E_HIP;
+    }
+    *out = ctx;
+    return LPK_OK;
+}
+
+void lpk_ctx_destroy(lpk_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    free_scene(ctx);
+    ctx->jobs.release();
+    ctx->links.release();
+    ctx->dt_stage.release();
+    ctx->out_stage.release();
+    ctx->codes_stage.release();
+    ctx->blocksum.release();
+    for (auto &s : ctx->pin) {
+        if (s.p) (void)hipHostFree(s.p);
+        if (s.done) (void)hipEventDestroy(s.done);
+    }
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *lpk_last_error(const lpk_ctx *ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
+
+int lpk_scene_upload(lpk_ctx *ctx, const lpk_tx_desc *tx, uint32_t n_tx, const lpk_rx_desc *rx, uint32_t n_rx) {
+    if (!ctx) return LPK_E_INVALID;
+    if ((n_tx && !tx) || (n_rx && !rx)) return fail(ctx, Fail{LPK_E_INVALID, "descriptor array is NULL"});
+    return guarded(ctx, [&] {
+        hip_check(hipSetDevice(ctx->device), "set device");
+        free_scene(ctx);
+        std::vector<TxDev> host;
+        host.reserve(n_tx);
+        for (uint32_t i = 0; i < n_tx; ++i) {
+            host.push_back(make_tx(ctx, tx[i], i));
+            ctx->tx_carrier.push_back(tx[i].carrier_hz);
+        }
+        ctx->tx_dev = static_cast<TxDev *>(upload(ctx, host.data(), host.size() * sizeof(TxDev)));
+        ctx->tx_host = std::move(host);
+        ctx->n_tx = n_tx;
+        ctx->rx.resize(n_rx);
+        for (uint32_t r = 0; r < n_rx; ++r) {
+            const lpk_rx_desc &d = rx[r];
+            for (int k = 0; k < 3; ++k)
+                if (!is_fin(d.pos_ecef[k])) throw Fail{LPK_E_NONFINITE, "receiver position"};
+            if (!(d.master_rate_hz > 0.0) || !is_fin(d.master_rate_hz))
+                throw Fail{LPK_E_INVALID, "receiver master_rate_hz must be positive"};
+            if (!is_fin(d.allan_deviation_1s) || d.allan_deviation_1s < 0.0 || !is_fin(d.frequency_offset_y0) ||
+                !is_fin(d.initial_time_offset_s))
+                throw Fail{LPK_E_NONFINITE, "receiver clock parameters"};
+            RxState &R = ctx->rx[r];
+            R.d = d;
+            R.walk = d.allan_deviation_1s > 0.0;  // src/clock/mod.rs:274-279
+            R.sigma_step = R.walk ? d.allan_deviation_1s / std::sqrt(d.master_rate_hz) : 0.0;
+        }
+        ctx->have_scene = true;
+    });
+}
+
+int lpk_render_chunk(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form) {
+    if (!ctx) return LPK_E_INVALID;
+    return guarded(ctx, [&] {
+        hip_check(hipSetDevice(ctx->device), "set device");
+        render(ctx, jobs, n_jobs, form, nullptr);
+    });
+}
+
+int lpk_render_taps(lpk_ctx *ctx, const lpk_band_chunk *job, int form, uint32_t slot, double *t_emit,
+                    double *range, double *phase, double *delta_t) {
+    if (!ctx || !job) return LPK_E_INVALID;
+    DevBuf taps;
+    int rc = guarded(ctx, [&] {
+        hip_check(hipSetDevice(ctx->device), "set device");
+        const size_t n = job->clen;
+        double *d = static_cast<double *>(taps.ensure(std::max<size_t>(n, 1) * 32, ctx->stream));
+        TapReq req{slot, d, d + n, d + 2 * n, d + 3 * n};
+        render(ctx, job, 1, form, &req);
+        hip_check(hipStreamSynchronize(ctx->stream), "taps sync");
+        double *outs[4] = {t_emit, range, phase, delta_t};
+        for (int k = 0; k < 4; ++k)
+            if (outs[k] && n) hip_check(hipMemcpy(outs[k], d + k * n, n * 8, hipMemcpyDeviceToHost), "taps download");
+    });
+    taps.release();
+    return rc;
+}
+
+int lpk_sync(lpk_ctx *ctx) {
+    if (!ctx) return LPK_E_INVALID;
+    return guarded(ctx, [&] { hip_check(hipStreamSynchronize(ctx->stream), "stream sync"); });
+}
+
+int lpk_last_kernel_ms(lpk_ctx *ctx, double *ms) {
+    if (!ctx || !ms) return LPK_E_INVALID;
+    if (!ctx->timed) return fail(ctx, Fail{LPK_E_STATE, "no timed render yet"});
+    return guarded(ctx, [&] {
+        float f = 0.0f;
+        hip_check(hipEventElapsedTime(&f, ctx->ev0, ctx->ev1), "event elapsed");
+        *ms = f;
+    });
+}
+
+int lpk_device_alloc(lpk_ctx *ctx, size_t bytes, void **dev) {
+    if (!ctx || !dev) return LPK_E_INVALID;
+    *dev = nullptr;
+    return guarded(ctx, [&] {
+        hip_check(hipSetDevice(ctx->device), "set device");
+        if (hipMalloc(dev, std::max<size_t>(bytes, 1)) != hipSuccess) {
+            *dev = nullptr;
+            throw Fail{LPK_E_NOMEM, "device allocation failed"};
+        }
+    });
+}
+
+int lpk_device_free(lpk_ctx *ctx, void *dev) {
+    if (!ctx) return LPK_E_INVALID;
+    return guarded(ctx, [&] {
+        hip_check(hipStreamSynchronize(ctx->stream), "stream sync");
+        hip_check(hipFree(dev), "device free");
+    });
+}
+
+int lpk_copy_to_host(lpk_ctx *ctx, void *host, const void *dev, size_t bytes) {
+    if (!ctx || (bytes && (!host || !dev))) return LPK_E_INVALID;
+    return guarded(ctx, [&] {
+        hip_check(hipStreamSynchronize(ctx->stream), "stream sync");
+        hip_check(hipMemcpy(host, dev, bytes, hipMemcpyDeviceToHost), "copy to host");
+    });
+}
+
+int lpk_device_fnv1a(lpk_ctx *ctx, const void *dev, size_t bytes, uint64_t *hash) {
+    if (!ctx || !hash || (bytes && !dev)) return LPK_E_INVALID;
+    return guarded(ctx, [&] {
+        std::vector<unsigned char> h(bytes);
+        hip_check(hipStreamSynchronize(ctx->stream), "stream sync");
+        if (bytes) hip_check(hipMemcpy(h.data(), dev, bytes, hipMemcpyDeviceToHost), "copy to host");
+        uint64_t x = 0xCBF29CE484222325ULL;
+        for (unsigned char c : h) x = (x ^ c) * 0x100000001B3ULL;
+        *hash = x;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,106 @@
+// lpk_device.hpp -- device-side descriptors shared by the kernels and the
+// host launcher. Everything here lives in HBM once per scene (TxDev) or once
+// per render call (JobDev, LinkDev).
+#pragma once
+
+#include <cstdint>
+
+namespace lpk {
+
+// Physical constants, as the reference defines them.
+constexpr double kSpeedOfLight = 299792458.0;          // src/lib.rs:53
+constexpr double kInvC = 1.0 / kSpeedOfLight;           // src/channel/mod.rs:66
+constexpr double kEmissionInitialLag = 0.07;            // src/channel/mod.rs:23
+constexpr double kEarthRotation = 7.2921151467e-5;      // src/channel/mod.rs:27
+constexpr double kTau = 6.283185307179586;              // std::f64::consts::TAU
+constexpr double kCaChipRate = 1023000.0;               // src/signal/ca_code.rs:23
+constexpr int kCaCodeLength = 1023;                     // src/signal/ca_code.rs:24
+constexpr double kNavBitRate = 50.0;                    // src/signal/ca_code.rs:27
+constexpr int64_t kNavSubframeBits = 300;               // src/signal/ca_code.rs:29
+constexpr int kNewtonIterations = 2;                    // src/pipeline.rs:56
+
+// Philox4x64-10 stream tags: one key word separates the clock walk from the
+// noise stream drawn with the same seed.
+constexpr uint64_t kClockTag = 0x636C6F636B77616CULL;   // "clockwal"
+constexpr uint64_t kAwgnTag = 0x6177676E6E6F6973ULL;    // "awgnnois"
+
+// Clock walk: blocks of kWalkBlock master-grid increments; each of the
+// kWalkThreads threads of a block owns kWalkPerThread consecutive ones.
+constexpr int kWalkThreads = 256;
+constexpr int kWalkPerThread = 32;
+constexpr int64_t kWalkBlock = int64_t(kWalkThreads) * kWalkPerThread;
+
+// Render tile: one workgroup of kTile lanes, one output sample per lane.
+constexpr int kTile = 256;
+
+struct TxDev {
+    int32_t traj_kind;       // 0 stationary, 1 grid
+    int32_t signal_kind;     // lpk_signal_kind
+    double pos[3];
+    double grid_t0, grid_dt;
+    int64_t grid_n;
+    const double *grid;      // (3, n)
+    double carrier;
+    double neg_tau_src;      // -TAU * f_src            (src/channel/mod.rs:239)
+    double amp_factor;       // lambda/(2 TAU) * sqrt(P) (src/channel/mod.rs:243)
+    // CW
+    double omega;            // TAU * baseband offset  (src/signal/cw.rs:42)
+    double anchor_phase;
+    int32_t cw_anchored;
+    // framed
+    int32_t repeating;
+    double rate, epoch_rel;
+    int64_t repeat_from;
+    int64_t n_symbols;
+    const int8_t *bits;
+    const float *symbols;    // interleaved
+    // C/A
+    int32_t nav_mode;
+    int32_t gated;
+    const int8_t *ca_code;
+    uint64_t nav_seed;
+    const int8_t *nav_bits;
+    int64_t n_nav_bits;
+    // gate
+    int32_t n_bursts, _pad;
+    double gate_period, gate_phase;
+    const double *bursts;
+};
+
+// One (band, active transmitter) pair of a render call, in summation order.
+struct LinkDev {
+    uint32_t tx;
+    uint32_t _pad;
+    double tau_if;           // TAU * (f_src - f_lo)   (src/channel/mod.rs:240)
+};
+
+struct PostDev {
+    int32_t kind;            // lpk_post_kind
+    int32_t _pad;
+    float a, b;              // GAIN: g; NONLINEARITY: k; ADC: full scale, lsb
+};
+
+struct JobDev {
+    uint32_t tile0;          // first tile of this job in the launch
+    uint32_t n_tiles;
+    uint32_t link0, n_active;
+    double fs, lo, epoch;
+    uint64_t i0, clen;
+    double obs[3];
+    // clock
+    const double *delta_t;   // host-provided offsets (device copy) or null
+    const double *eps;       // materialised walk, eps[k - eps_k0]
+    int64_t eps_k0;
+    double c0, y0, master;
+    int32_t walk;
+    // AWGN and post-sum
+    int32_t n_post;
+    float sigma;
+    int32_t _pad;
+    uint64_t noise_seed;
+    PostDev post[4];
+    float *out;              // interleaved complex64
+    int16_t *codes;          // optional ADC codes (2 per sample)
+};
+
+}  // namespace lpk

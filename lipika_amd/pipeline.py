@@ -1,0 +1,571 @@
+"""Host-side mirror of the reference's scene orchestration
+(src/pipeline.rs): scene preparation (Transmitter/Receiver::from_config),
+per-(receiver, band) work units, seeds, band-overlap gating, and the chunk
+loop -- with the chunk body running on the GPU through the C ABI
+(include/lpk.h). Names, arguments and error behaviour follow the reference:
+``run_scene`` (:443), ``render_window`` (:709), ``WindowOptions`` (:624).
+
+There is no CPU fallback: without ``liblpk.so`` or a gfx950 device every
+render raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+from enum import Enum
+from typing import Callable, List, Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from .config import (
+    Bpsk, CaCode, CircularKeplerian, Cw, Ook, Qam16, ReceiverBandConfig, ReceiverConfig,
+    SceneConfig, SceneMeta, SkyConfig, StationaryEcef, StationaryGeodetic, TransmitterConfig,
+    CA_CHIP_RATE_HZ,
+)
+
+NEWTON_ITERATIONS = 2                 # src/pipeline.rs:56
+TX_TRAJECTORY_GRID_DT_S = 0.01        # src/pipeline.rs:62
+DEFAULT_CHUNK_SAMPLES = 1 << 20       # src/pipeline.rs:367
+DEFAULT_WINDOW_CHUNK_SAMPLES = 1 << 16  # src/pipeline.rs:637
+DEFAULT_SIGNAL_EPOCH_UNIX = 0.0       # src/pipeline.rs:139
+CA_CODE_LENGTH = 1023
+U64 = (1 << 64) - 1
+
+
+class TimeBase(Enum):
+    """src/pipeline.rs:124-135"""
+    SCENE = 0
+    ABSOLUTE = 1
+
+
+def rust_round(x: float) -> int:
+    """f64::round (half away from zero) for the sample count, src/pipeline.rs:856."""
+    r = math.floor(x)
+    if x - r >= 0.5:
+        r += 1
+    return int(r)
+
+
+def clock_seed(base: int, r_idx: int) -> int:
+    """src/pipeline.rs:825-827"""
+    return (base + 0xC10C_0000_0000_0000 + r_idx) & U64
+
+
+def noise_seed(base: int, work_idx: int) -> int:
+    """src/pipeline.rs:828-830"""
+    return (base + work_idx) & U64
+
+
+# ---------------------------------------------------------------- preparation
+@dataclass
+class PreparedTx:
+    """A transmitter as Transmitter::from_config leaves it (src/pipeline.rs:297-341)."""
+    name: str
+    carrier_hz: float
+    tx_power_w: float
+    bandwidth_hz: float
+    signal_kind: int
+    traj_kind: int
+    pos: np.ndarray = field(default_factory=lambda: np.zeros(3))
+    grid_t0: float = 0.0
+    grid_dt: float = 0.0
+    grid: Optional[np.ndarray] = None          # (3, n) float64
+    baseband_offset_hz: float = 0.0
+    cw_anchored: bool = False
+    anchor_phase: float = 0.0
+    rate_hz: float = 0.0
+    epoch_rel_s: float = 0.0
+    repeating: bool = False
+    repeat_from: int = 0
+    bits: Optional[np.ndarray] = None          # int8
+    symbols: Optional[np.ndarray] = None       # float32 (n, 2)
+    nav_mode: int = _lib.NAV_OFF
+    prn: int = 0
+    ca_code: Optional[np.ndarray] = None       # int8 (1023)
+    nav_seed: int = 0
+    gated: bool = False
+    gate_period: float = 0.0
+    gate_phase: float = 0.0
+    bursts: Optional[np.ndarray] = None        # float64 (k, 2): (start, end)
+
+    @property
+    def n_symbols(self) -> int:
+        if self.signal_kind == _lib.SIG_QAM16:
+            return 0 if self.symbols is None else len(self.symbols)
+        return 0 if self.bits is None else len(self.bits)
+
+
+@dataclass
+class PreparedBand:
+    band: ReceiverBandConfig
+    n: int
+    active: np.ndarray            # uint32 transmitter indices, summation order
+    noise_power_w: float
+    post: list                    # [(kind, bits, value)]
+
+
+@dataclass
+class PreparedRx:
+    name: str
+    pos: np.ndarray
+    clock: object
+    master_rate_hz: float
+    bands: List[PreparedBand]
+
+
+@dataclass
+class PreparedScene:
+    meta: SceneMeta
+    transmitters: List[PreparedTx]
+    receivers: List[PreparedRx]
+
+
+def _rem_euclid(a: float, b: float) -> float:
+    r = math.fmod(a, b)
+    return r + abs(b) if r < 0.0 else r
+
+
+def station_ecef(traj) -> np.ndarray:
+    lib = _lib.load()
+    if isinstance(traj, StationaryGeodetic):
+        out = (C.c_double * 3)()
+        lib.lpk_geodetic_to_ecef(traj.lat_deg, traj.lon_deg, traj.h_m, out)
+        return np.array(out[:], dtype=np.float64)
+    if isinstance(traj, StationaryEcef):
+        return np.array([traj.x_m, traj.y_m, traj.z_m], dtype=np.float64)
+    raise NotImplementedError("only stationary receivers are on the GPU path")
+
+
+def _check(rc: int) -> None:
+    _lib.check(rc)
+
+
+def build_transmitter(cfg: TransmitterConfig, meta: SceneMeta, time_base: TimeBase) -> PreparedTx:
+    """Transmitter::from_config + build_signal (src/pipeline.rs:142-253, :312-341)."""
+    lib = _lib.load()
+    sig = cfg.kind
+    kind = {Cw: _lib.SIG_CW, Bpsk: _lib.SIG_BPSK, Ook: _lib.SIG_OOK, Qam16: _lib.SIG_QAM16,
+            CaCode: _lib.SIG_CA}[type(sig)]
+    tx = PreparedTx(name=cfg.name, carrier_hz=float(cfg.carrier_hz),
+                    tx_power_w=math.pow(10.0, cfg.tx_power_dbw / 10.0),   # src/pipeline.rs:338
+                    bandwidth_hz=float(cfg.effective_bandwidth_hz()), signal_kind=kind,
+                    traj_kind=_lib.TRAJ_STATIONARY)
+    traj = cfg.trajectory
+    if isinstance(traj, (StationaryGeodetic, StationaryEcef)):
+        tx.pos = station_ecef(traj)
+    elif isinstance(traj, CircularKeplerian):
+        n = int(lib.lpk_grid_size(meta.duration_s, TX_TRAJECTORY_GRID_DT_S))
+        grid = np.empty((3, n), dtype=np.float64)
+        t0 = C.c_double()
+        _check(lib.lpk_grid_fill_kepler(
+            traj.semi_major_axis_m, traj.inclination_deg, traj.raan_deg, traj.true_anomaly_deg_at_epoch,
+            traj.epoch_unix, meta.epoch_unix, meta.duration_s, TX_TRAJECTORY_GRID_DT_S, C.byref(t0),
+            grid.ctypes.data_as(C.POINTER(C.c_double)), n))
+        tx.traj_kind = _lib.TRAJ_GRID
+        tx.grid, tx.grid_t0, tx.grid_dt = grid, t0.value, TX_TRAJECTORY_GRID_DT_S
+    else:
+        raise NotImplementedError(f"trajectory {traj!r} is not on the GPU path")
+
+    render_epoch = meta.epoch_unix
+    anchor_epoch = cfg.signal_epoch_unix
+    if anchor_epoch is None and time_base == TimeBase.ABSOLUTE:
+        anchor_epoch = DEFAULT_SIGNAL_EPOCH_UNIX
+
+    def frame_anchor(rate: float, n_sym: int):
+        rel, off = C.c_double(), C.c_int64()
+        _check(lib.lpk_frame_anchor(render_epoch, anchor_epoch, rate, n_sym, C.byref(rel), C.byref(off)))
+        return rel.value, off.value
+
+    if isinstance(sig, Cw):
+        tx.baseband_offset_hz = float(sig.baseband_offset_hz)
+        if anchor_epoch is not None:
+            ph = C.c_double()
+            _check(lib.lpk_cw_anchor_phase(render_epoch, anchor_epoch, sig.baseband_offset_hz, C.byref(ph)))
+            tx.cw_anchored, tx.anchor_phase = True, ph.value
+    elif isinstance(sig, (Bpsk, Ook, Qam16)):
+        if isinstance(sig, Qam16):
+            sync = np.asarray(sig.sync_word, dtype=np.int64)
+            if np.any((sync < 0) | (sync > 1)):
+                raise ValueError(f"qam16 sync_word entries must be 0 or 1 (transmitter {cfg.name!r})")
+            if len(sync) % 4:
+                raise ValueError(f"qam16 sync_word must hold a whole number of symbols (transmitter {cfg.name!r})")
+            s8 = sync.astype(np.uint8)
+            n_sym = len(s8) // 4 + sig.payload_symbols
+            out = np.empty((n_sym, 2), dtype=np.float32)
+            _check(lib.lpk_payload_qam16(s8.ctypes.data_as(C.POINTER(C.c_uint8)), len(s8), sig.payload_symbols,
+                                         sig.payload_seed & U64, out.ctypes.data_as(C.POINTER(C.c_float))))
+            tx.symbols, rate = out, float(sig.symbol_rate_baud)
+        elif isinstance(sig, Ook):
+            sync = np.asarray(sig.sync_word, dtype=np.int64)
+            if np.any((sync < 0) | (sync > 1)):
+                raise ValueError(f"ook sync_word entries must be 0 or 1 (transmitter {cfg.name!r})")
+            s8 = sync.astype(np.uint8)
+            out = np.empty(len(s8) + sig.payload_bits, dtype=np.int8)
+            _check(lib.lpk_payload_ook(s8.ctypes.data_as(C.POINTER(C.c_uint8)), len(s8), sig.payload_bits,
+                                       sig.payload_seed & U64, out.ctypes.data_as(C.POINTER(C.c_int8))))
+            tx.bits, rate = out, float(sig.bit_rate_bps)
+        else:
+            s8 = np.asarray(sig.sync_word, dtype=np.int8)
+            out = np.empty(len(s8) + sig.payload_bits, dtype=np.int8)
+            _check(lib.lpk_payload_bpsk(s8.ctypes.data_as(C.POINTER(C.c_int8)), len(s8), sig.payload_bits,
+                                        sig.payload_seed & U64, out.ctypes.data_as(C.POINTER(C.c_int8))))
+            tx.bits, rate = out, float(sig.bit_rate_bps)
+        tx.rate_hz = rate
+        tx.epoch_rel_s = 0.0   # bit 0 at the scene epoch
+        if anchor_epoch is not None:
+            tx.epoch_rel_s, tx.repeat_from = frame_anchor(rate, tx.n_symbols)
+            tx.repeating = True
+    else:  # C/A
+        code = np.empty(CA_CODE_LENGTH, dtype=np.int8)
+        if not 1 <= sig.prn <= 32:
+            raise ValueError(f"PRN must be 1..32, got {sig.prn}")
+        _check(lib.lpk_ca_code(sig.prn, code.ctypes.data_as(C.POINTER(C.c_int8))))
+        tx.ca_code, tx.prn = code, sig.prn
+        if time_base == TimeBase.SCENE:
+            tx.epoch_rel_s = sig.epoch_unix - meta.epoch_unix
+            if sig.nav == "off":
+                tx.nav_mode = _lib.NAV_OFF
+            elif sig.nav == "dummy":
+                tx.nav_mode = _lib.NAV_DUMMY
+                tx.nav_seed = int(lib.lpk_nav_fold_seed(sig.nav_seed & U64, sig.prn))
+            else:
+                raise NotImplementedError("nav = \"full\" (LNAV from ephemeris) is not on the GPU path")
+        else:
+            if sig.nav != "off":
+                raise ValueError(f"transmitter '{cfg.name}': ca_code with nav = \"{sig.nav}\" is not supported "
+                                 "in a window render; use nav = \"off\" (pure code)")
+            code_rate = CA_CHIP_RATE_HZ / CA_CODE_LENGTH
+            rel, off = C.c_double(), C.c_int64()
+            _check(lib.lpk_frame_anchor(render_epoch, sig.epoch_unix, code_rate, 1, C.byref(rel), C.byref(off)))
+            tx.epoch_rel_s = rel.value
+    if cfg.gate is not None:
+        g = cfg.gate
+        if not (g.period_s > 0.0) or not math.isfinite(g.period_s):
+            raise ValueError(f"transmitter '{cfg.name}': gate period_s must be positive and finite, got {g.period_s}")
+        bursts = []
+        for off_s, dur_s in g.bursts:
+            if not (off_s >= 0.0) or not (dur_s > 0.0) or not (off_s + dur_s <= g.period_s):
+                raise ValueError(f"transmitter '{cfg.name}': gate burst (offset_s = {off_s}, duration_s = {dur_s}) "
+                                 "must lie within one period")
+            bursts.append((off_s, off_s + dur_s))
+        phase = render_epoch - g.epoch_unix                          # src/signal/gate.rs:56-66
+        if not math.isfinite(phase):
+            raise ValueError(f"transmitter '{cfg.name}': gate phase_s must be finite")
+        tx.gated, tx.gate_period = True, float(g.period_s)
+        tx.gate_phase = _rem_euclid(phase, g.period_s)
+        tx.bursts = np.asarray(bursts, dtype=np.float64).reshape(-1, 2)
+    return tx
+
+
+def _post_ops(band: ReceiverBandConfig) -> list:
+    """RxChain::from_config, src/rx/mod.rs:117-137."""
+    ops = []
+    fe = band.frontend
+    if fe is None:
+        return ops
+    if fe.antenna is not None:
+        raise NotImplementedError("antenna patterns (and the horizon gate) are not on the GPU path")
+    if fe.gain_db is not None:
+        ops.append((_lib.POST_GAIN, 0, float(fe.gain_db)))
+    if fe.nonlinearity is not None:
+        ops.append((_lib.POST_NONLINEARITY, 0, math.pow(10.0, (fe.nonlinearity.iip3_dbm - 30.0) / 10.0)))
+    if fe.adc is not None:
+        ops.append((_lib.POST_ADC, int(fe.adc.bits), float(fe.adc.full_scale_v)))
+    return ops
+
+
+def prepare(scene: SceneConfig, time_base: TimeBase = TimeBase.SCENE) -> PreparedScene:
+    """src/pipeline.rs:801-819, plus the per-band set-up of generate_band (:853-894)."""
+    lib = _lib.load()
+    meta = scene.scene
+    txs = [build_transmitter(t, meta, time_base) for t in scene.transmitters]
+    rxs = []
+    for r in scene.receivers:
+        if not r.bands:
+            raise ValueError(f"receiver '{r.name}' has no bands; at least one [[receiver.band]] required")
+        if r.clock.flicker_floor > 0.0:
+            raise NotImplementedError("flicker FM is not on the GPU path")
+        pos = station_ecef(r.trajectory)
+        master = 0.0
+        for b in r.bands:
+            master = max(master, float(b.sample_rate_hz))
+        bands = []
+        for b in r.bands:
+            n = rust_round(b.sample_rate_hz * meta.duration_s)
+            bw = b.effective_bandwidth_hz()
+            active = [i for i, t in enumerate(txs)
+                      if lib.lpk_bands_overlap(t.carrier_hz, t.bandwidth_hz, b.lo_carrier_hz, bw)]
+            p = b.noise_power_w if b.noise_power_w is not None else \
+                float(lib.lpk_thermal_noise_power_w(b.noise_temperature_k, b.sample_rate_hz))
+            bands.append(PreparedBand(b, n, np.asarray(active, dtype=np.uint32), float(p), _post_ops(b)))
+        rxs.append(PreparedRx(r.name, pos, r.clock, master, bands))
+    return PreparedScene(meta, txs, rxs)
+
+
+# ---------------------------------------------------------------- backend
+def _ptr(a: Optional[np.ndarray], ctype):
+    if a is None:
+        return C.POINTER(ctype)()
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+class Backend:
+    """One HIP context (one device, one stream): the ``Backend`` slot of
+    src/backend/mod.rs:14-18 filled with the gfx950 implementation."""
+
+    def __init__(self, device: int = 0):
+        self.lib = _lib.load()
+        h = C.c_void_p()
+        _lib.check(self.lib.lpk_ctx_create(device, C.byref(h)), None)
+        self.h = h
+        self._keep: list = []
+        self.scene: Optional[PreparedScene] = None
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.lpk_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def check(self, rc: int) -> None:
+        _lib.check(rc, self.h)
+
+    def upload(self, scene: PreparedScene, rng_seed_base: int) -> None:
+        n_tx, n_rx = len(scene.transmitters), len(scene.receivers)
+        tx_arr = (_lib.TxDesc * max(n_tx, 1))()
+        keep = []
+        for i, t in enumerate(scene.transmitters):
+            d = tx_arr[i]
+            d.traj_kind, d.signal_kind = t.traj_kind, t.signal_kind
+            d.pos_ecef[:] = [float(v) for v in t.pos]
+            if t.grid is not None:
+                g = np.ascontiguousarray(t.grid, dtype=np.float64)
+                keep.append(g)
+                d.grid_t0_unix, d.grid_dt_s, d.grid_n = t.grid_t0, t.grid_dt, g.shape[1]
+                d.grid_xyz = _ptr(g, C.c_double)
+            d.carrier_hz, d.tx_power_w, d.bandwidth_hz = t.carrier_hz, t.tx_power_w, t.bandwidth_hz
+            d.baseband_offset_hz, d.cw_anchored, d.anchor_phase_rad = t.baseband_offset_hz, int(t.cw_anchored), t.anchor_phase
+            d.rate_hz, d.epoch_rel_s = t.rate_hz, t.epoch_rel_s
+            d.repeating, d.repeat_from, d.n_symbols = int(t.repeating), t.repeat_from, t.n_symbols
+            if t.bits is not None:
+                b = np.ascontiguousarray(t.bits, dtype=np.int8)
+                keep.append(b)
+                d.bits = _ptr(b, C.c_int8)
+            if t.symbols is not None:
+                s = np.ascontiguousarray(t.symbols, dtype=np.float32)
+                keep.append(s)
+                d.symbols = _ptr(s, C.c_float)
+            d.nav_mode, d.prn, d.nav_seed = t.nav_mode, t.prn, t.nav_seed & U64
+            if t.ca_code is not None:
+                c = np.ascontiguousarray(t.ca_code, dtype=np.int8)
+                keep.append(c)
+                d.ca_code = _ptr(c, C.c_int8)
+            if t.gated:
+                bb = np.ascontiguousarray(t.bursts, dtype=np.float64)
+                keep.append(bb)
+                d.gated, d.n_bursts, d.gate_period_s, d.gate_phase_s = 1, len(bb), t.gate_period, t.gate_phase
+                d.bursts = _ptr(bb, C.c_double)
+        rx_arr = (_lib.RxDesc * max(n_rx, 1))()
+        for r_idx, r in enumerate(scene.receivers):
+            d = rx_arr[r_idx]
+            d.pos_ecef[:] = [float(v) for v in r.pos]
+            d.allan_deviation_1s = r.clock.allan_deviation_1s
+            d.frequency_offset_y0 = r.clock.frequency_offset_y0
+            d.initial_time_offset_s = r.clock.initial_time_offset_s
+            d.master_rate_hz = r.master_rate_hz
+            d.clock_seed = clock_seed(rng_seed_base, r_idx)
+        self.check(self.lib.lpk_scene_upload(self.h, tx_arr, n_tx, rx_arr, n_rx))
+        del keep  # tables are copied by lpk_scene_upload
+        self.scene = scene
+
+    def job(self, r_idx: int, b_idx: int, work_idx: int, rng_seed_base: int, i0: int, clen: int,
+            out_host: Optional[np.ndarray] = None, out_dev: Optional[int] = None,
+            codes: Optional[np.ndarray] = None, delta_t: Optional[np.ndarray] = None) -> tuple:
+        """One lpk_band_chunk for samples [i0, i0+clen) of band b_idx of receiver r_idx.
+        Returns (descriptor, keep-alive list)."""
+        sc = self.scene
+        pb = sc.receivers[r_idx].bands[b_idx]
+        j = _lib.BandChunk()
+        keep = [pb.active]
+        j.rx, j.n_active = r_idx, len(pb.active)
+        j.active = _ptr(pb.active, C.c_uint32)
+        j.fs_hz, j.lo_hz, j.epoch_unix = pb.band.sample_rate_hz, pb.band.lo_carrier_hz, sc.meta.epoch_unix
+        j.i0, j.clen = i0, clen
+        if delta_t is not None:
+            dt = np.ascontiguousarray(delta_t, dtype=np.float64)
+            keep.append(dt)
+            j.delta_t = _ptr(dt, C.c_double)
+        j.noise_power_w, j.noise_seed = pb.noise_power_w, noise_seed(rng_seed_base, work_idx)
+        j.n_post = len(pb.post)
+        for k, (kind, bits, value) in enumerate(pb.post):
+            j.post[k].kind, j.post[k].bits, j.post[k].value = kind, bits, value
+        if out_dev is not None:
+            j.out_dev = out_dev
+        if out_host is not None:
+            assert out_host.dtype == np.complex64 and out_host.flags.c_contiguous and len(out_host) >= clen
+            j.out_host = out_host.ctypes.data
+        if codes is not None:
+            assert codes.dtype == np.int16 and codes.flags.c_contiguous and codes.size >= 2 * clen
+            j.adc_codes_host = _ptr(codes, C.c_int16)
+        return j, keep
+
+    def render(self, jobs: Sequence, form: int = _lib.FORM_PLAIN) -> None:
+        arr = (_lib.BandChunk * max(len(jobs), 1))(*jobs)
+        self.check(self.lib.lpk_render_chunk(self.h, arr, len(jobs), form))
+
+    def sync(self) -> None:
+        self.check(self.lib.lpk_sync(self.h))
+
+    def last_kernel_ms(self) -> float:
+        ms = C.c_double()
+        self.check(self.lib.lpk_last_kernel_ms(self.h, C.byref(ms)))
+        return ms.value
+
+    def taps(self, job, slot: int, form: int = _lib.FORM_PLAIN):
+        n = job.clen
+        te, rg, ph, dt = (np.empty(n, dtype=np.float64) for _ in range(4))
+        self.check(self.lib.lpk_render_taps(self.h, C.byref(job), form, slot, _ptr(te, C.c_double),
+                                            _ptr(rg, C.c_double), _ptr(ph, C.c_double), _ptr(dt, C.c_double)))
+        return te, rg, ph, dt
+
+
+_FORMS = {"plain": _lib.FORM_PLAIN, "precision": _lib.FORM_PRECISION}
+
+
+def form_code(form) -> int:
+    return _FORMS[form] if isinstance(form, str) else int(form)
+
+
+# ---------------------------------------------------------------- run_scene
+@dataclass
+class ObserverOutput:
+    """src/output/mod.rs (name, epoch, rate, LO, IQ, descriptor)."""
+    name: str
+    epoch_unix: float
+    sample_rate_hz: float
+    lo_carrier_hz: float
+    iq: np.ndarray
+    descriptor: dict
+
+
+def _units(sc: PreparedScene):
+    w = 0
+    for r_idx, r in enumerate(sc.receivers):
+        for b_idx, _ in enumerate(r.bands):
+            yield w, r_idx, b_idx
+            w += 1
+
+
+def run_scene(scene: SceneConfig, rng_seed_base: int, chunk_samples: int = DEFAULT_CHUNK_SAMPLES,
+              form="plain", backend: Optional[Backend] = None) -> List[ObserverOutput]:
+    """Every (receiver, band) of the scene in memory (src/pipeline.rs:443-485).
+    All bands' chunk c run in one launch."""
+    chunk_samples = max(int(chunk_samples), 1)
+    own = backend is None
+    be = backend or Backend()
+    try:
+        sc = prepare(scene, TimeBase.SCENE)
+        be.upload(sc, rng_seed_base)
+        units = list(_units(sc))
+        outs = {w: np.empty(sc.receivers[r].bands[b].n, dtype=np.complex64) for w, r, b in units}
+        n_chunks = max([math.ceil(sc.receivers[r].bands[b].n / chunk_samples) for _, r, b in units] + [0])
+        for c in range(n_chunks):
+            jobs, keep = [], []
+            for w, r, b in units:
+                n = sc.receivers[r].bands[b].n
+                i0 = c * chunk_samples
+                if i0 >= n:
+                    continue
+                clen = min(chunk_samples, n - i0)
+                j, k = be.job(r, b, w, rng_seed_base, i0, clen, out_host=outs[w][i0:i0 + clen])
+                jobs.append(j)
+                keep.append(k)
+            be.render(jobs, form_code(form))
+        be.sync()
+        result = []
+        for w, r, b in units:
+            rx = sc.receivers[r]
+            pb = rx.bands[b]
+            desc = {
+                "receiver": rx.name, "band_id": pb.band.band_id, "duration_s": sc.meta.duration_s,
+                "band": {"lo_carrier_hz": pb.band.lo_carrier_hz, "sample_rate_hz": pb.band.sample_rate_hz,
+                         "bandwidth_hz": pb.band.effective_bandwidth_hz(),
+                         "noise_temperature_k": pb.band.noise_temperature_k, "noise_power_w": pb.noise_power_w},
+                "contributing_transmitters": [sc.transmitters[i].name for i in pb.active],
+            }
+            result.append(ObserverOutput(f"{rx.name}__{pb.band.band_id}", sc.meta.epoch_unix,
+                                         pb.band.sample_rate_hz, pb.band.lo_carrier_hz, outs[w], desc))
+        return result
+    finally:
+        if own:
+            be.close()
+
+
+# ---------------------------------------------------------------- render_window
+@dataclass
+class WindowOptions:
+    """src/pipeline.rs:623-643"""
+    seed: int = 0
+    chunk_samples: int = DEFAULT_WINDOW_CHUNK_SAMPLES
+    n_threads: int = 1   # accepted for interface parity; the GPU render ignores it
+
+
+@dataclass
+class WindowSummary:
+    """src/pipeline.rs:646-669"""
+    n_samples_window: int
+    n_samples_emitted: int
+    stopped_early: bool
+    contributing_transmitters: List[str]
+    visible_transmitters: List[str]
+
+
+BREAK = "break"   # a sink returns BREAK to stop the render (ControlFlow::Break)
+
+
+def render_window(sky: SkyConfig, receiver: ReceiverConfig, band: ReceiverBandConfig, start_unix: float,
+                  duration_s: float, options: WindowOptions, sink: Callable[[np.ndarray], object],
+                  form="plain", backend: Optional[Backend] = None) -> WindowSummary:
+    """One receiver, one band, one absolute-time window (src/pipeline.rs:709-794).
+    ``sink`` is called on the calling thread with each chunk in order."""
+    if not math.isfinite(start_unix):
+        raise ValueError(f"window start_unix must be finite, got {start_unix}")
+    if not (duration_s >= 0.0) or not math.isfinite(duration_s):
+        raise ValueError(f"window duration_s must be finite and non-negative, got {duration_s}")
+    if not (band.sample_rate_hz > 0.0) or not math.isfinite(band.sample_rate_hz):
+        raise ValueError(f"band sample_rate_hz must be positive, got {band.sample_rate_hz}")
+    scene = SceneConfig(SceneMeta(start_unix, duration_s, sky.gps_utc_offset_s), list(sky.transmitters),
+                        [ReceiverConfig(receiver.name, receiver.trajectory, receiver.clock, [band])])
+    chunk = max(int(options.chunk_samples), 1)
+    own = backend is None
+    be = backend or Backend()
+    try:
+        sc = prepare(scene, TimeBase.ABSOLUTE)
+        be.upload(sc, options.seed)
+        pb = sc.receivers[0].bands[0]
+        n = pb.n
+        i0, stopped = 0, False
+        buf = np.empty(min(chunk, max(n, 1)), dtype=np.complex64)
+        while i0 < n:
+            clen = min(chunk, n - i0)
+            j, keep = be.job(0, 0, 0, options.seed, i0, clen, out_host=buf[:clen])
+            be.render([j], form_code(form))
+            be.sync()
+            flow = sink(buf[:clen].copy())
+            i0 += clen
+            if flow == BREAK and i0 < n:
+                stopped = True
+                break
+        names = [sc.transmitters[i].name for i in pb.active]
+        return WindowSummary(rust_round(band.sample_rate_hz * duration_s), i0, stopped, names,
+                             list(names) if i0 > 0 else [])
+    finally:
+        if own:
+            be.close()
