@@ -183,6 +183,13 @@ int lpk_render_taps(lpk_ctx *ctx, const lpk_band_chunk *job, int form, uint32_t 
 /* Time of the last lpk_render_chunk kernel on the context's stream, in ms,
  * measured with HIP events around the launch (valid after lpk_sync). */
 int lpk_last_kernel_ms(lpk_ctx *ctx, double *ms);
+/* Kernel-time accounting for benchmarks (the reference's profile::time
+ * sections, src/profile.rs:27-36, for the GPU path): lpk_timing_reset waits
+ * for the stream and clears the record; every later render launch is
+ * bracketed by HIP events; lpk_timing_read waits for the stream and returns
+ * the summed render-kernel time and the number of launches since the reset. */
+int lpk_timing_reset(lpk_ctx *ctx);
+int lpk_timing_read(lpk_ctx *ctx, double *total_ms, uint64_t *n_launches);
 
 /* ---- device memory helpers for callers that keep output in HBM ---- */
 int lpk_device_alloc(lpk_ctx *ctx, size_t bytes, void **dev);

@@ -109,6 +109,8 @@ def load() -> C.CDLL:
         "lpk_sync": (_i32, [C.c_void_p]),
         "lpk_render_taps": (_i32, [C.c_void_p, P(BandChunk), C.c_int, _u32, P(_d), P(_d), P(_d), P(_d)]),
         "lpk_last_kernel_ms": (_i32, [C.c_void_p, P(_d)]),
+        "lpk_timing_reset": (_i32, [C.c_void_p]),
+        "lpk_timing_read": (_i32, [C.c_void_p, P(_d), P(_u64)]),
         "lpk_device_alloc": (_i32, [C.c_void_p, C.c_size_t, P(C.c_void_p)]),
         "lpk_device_free": (_i32, [C.c_void_p, C.c_void_p]),
         "lpk_copy_to_host": (_i32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
@@ -139,7 +141,7 @@ def load() -> C.CDLL:
 EXPORTED = (
     "lpk_abi_version", "lpk_ctx_create", "lpk_ctx_destroy", "lpk_last_error",
     "lpk_scene_upload", "lpk_render_chunk", "lpk_sync", "lpk_render_taps",
-    "lpk_last_kernel_ms", "lpk_device_alloc", "lpk_device_free", "lpk_copy_to_host",
+    "lpk_last_kernel_ms", "lpk_timing_reset", "lpk_timing_read", "lpk_device_alloc", "lpk_device_free", "lpk_copy_to_host",
     "lpk_device_fnv1a", "lpk_geodetic_to_ecef", "lpk_grid_size", "lpk_grid_fill_kepler",
     "lpk_ca_code", "lpk_nav_fold_seed", "lpk_payload_bpsk", "lpk_payload_ook",
     "lpk_payload_qam16", "lpk_elapsed_periods", "lpk_frame_anchor", "lpk_cw_anchor_phase",

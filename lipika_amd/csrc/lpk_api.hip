@@ -84,7 +84,10 @@ struct RxState {
 struct lpk_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;   // the last launch's pair (owned by tpool)
+    // event pairs around every render launch since the last timing reset
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> tpool;
+    size_t tused = 0;
     std::string err;
     // scene
     std::vector<void *> scene_allocs;
@@ -455,6 +458,17 @@ void render(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form,
     pinned_mark(ctx, slot);
 
     if (tiles) {
+        if (ctx->tused == ctx->tpool.size() && ctx->tused < 65536) {
+            std::pair<hipEvent_t, hipEvent_t> pr{nullptr, nullptr};
+            hip_check(hipEventCreate(&pr.first), "event create");
+            hip_check(hipEventCreate(&pr.second), "event create");
+            ctx->tpool.push_back(pr);
+        }
+        if (ctx->tused < ctx->tpool.size()) {
+            ctx->ev0 = ctx->tpool[ctx->tused].first;
+            ctx->ev1 = ctx->tpool[ctx->tused].second;
+            ++ctx->tused;
+        }
         hip_check(hipEventRecord(ctx->ev0, ctx->stream), "event record");
         hip_check(launch_render(form, jd, int(n_jobs), ld, ctx->tx_dev, uint32_t(tiles),
                                 tap ? tap->t_emit : nullptr, tap ? tap->range : nullptr,
@@ -502,8 +516,7 @@ int lpk_ctx_create(int hip_device, lpk_ctx **out) {
     lpk_ctx *ctx = new (std::nothrow) lpk_ctx();
     if (!ctx) return LPK_E_NOMEM;
     ctx->device = hip_device;
-    if (hipSetDevice(hip_device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
+    if (hipSetDevice(hip_device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         g_create_error = "HIP stream/event creation failed";
         delete ctx;
         return LPK_E_HIP;
@@ -526,8 +539,10 @@ void lpk_ctx_destroy(lpk_ctx *ctx) {
         if (s.p) (void)hipHostFree(s.p);
         if (s.done) (void)hipEventDestroy(s.done);
     }
-    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
-    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    for (auto &pr : ctx->tpool) {
+        (void)hipEventDestroy(pr.first);
+        (void)hipEventDestroy(pr.second);
+    }
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -607,6 +622,29 @@ int lpk_last_kernel_ms(lpk_ctx *ctx, double *ms) {
         float f = 0.0f;
         hip_check(hipEventElapsedTime(&f, ctx->ev0, ctx->ev1), "event elapsed");
         *ms = f;
+    });
+}
+
+int lpk_timing_reset(lpk_ctx *ctx) {
+    if (!ctx) return LPK_E_INVALID;
+    return guarded(ctx, [&] {
+        hip_check(hipStreamSynchronize(ctx->stream), "stream sync");
+        ctx->tused = 0;
+    });
+}
+
+int lpk_timing_read(lpk_ctx *ctx, double *total_ms, uint64_t *n_launches) {
+    if (!ctx || !total_ms || !n_launches) return LPK_E_INVALID;
+    return guarded(ctx, [&] {
+        hip_check(hipStreamSynchronize(ctx->stream), "stream sync");
+        double t = 0.0;
+        for (size_t k = 0; k < ctx->tused; ++k) {
+            float f = 0.0f;
+            hip_check(hipEventElapsedTime(&f, ctx->tpool[k].first, ctx->tpool[k].second), "event elapsed");
+            t += f;
+        }
+        *total_ms = t;
+        *n_launches = ctx->tused;
     });
 }
 

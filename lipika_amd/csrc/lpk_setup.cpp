@@ -77,10 +77,12 @@ extern "C" {
 
 void lpk_geodetic_to_ecef(double lat_deg, double lon_deg, double h_m, double out[3]) {
     const double lat = radians(lat_deg), lon = radians(lon_deg);
-    const double sl = std::sin(lat), cl = std::cos(lat);
+    double sl, cl, slon, clon;
+    ::sincos(lat, &sl, &cl);
+    ::sincos(lon, &slon, &clon);
     const double n = kWgs84A / std::sqrt(1.0 - kWgs84E2 * sl * sl);
-    out[0] = (n + h_m) * cl * std::cos(lon);
-    out[1] = (n + h_m) * cl * std::sin(lon);
+    out[0] = (n + h_m) * cl * clon;
+    out[1] = (n + h_m) * cl * slon;
     out[2] = (n * (1.0 - kWgs84E2) + h_m) * sl;
 }
 
@@ -99,19 +101,22 @@ int lpk_grid_fill_kepler(double a_m, double inc_deg, double raan_deg, double nu0
     // CircularKeplerian::eci_at + eci_to_ecef, src/geometry/kepler.rs:27-85
     const double mean_motion = std::sqrt(kGM / (a_m * a_m * a_m));
     const double nu0 = radians(nu0_deg);
-    const double si = std::sin(radians(inc_deg)), ci = std::cos(radians(inc_deg));
-    const double so = std::sin(radians(raan_deg)), co = std::cos(radians(raan_deg));
+    double si, ci, so, co;
+    ::sincos(radians(inc_deg), &si, &ci);
+    ::sincos(radians(raan_deg), &so, &co);
     for (uint64_t i = 0; i < n; ++i) {
         const double t = start + double(i) * grid_dt_s;
         const double dt = t - orbit_epoch_unix;
         const double nu = nu0 + mean_motion * dt;
-        const double snu = std::sin(nu), cnu = std::cos(nu);
+        double snu, cnu;
+        ::sincos(nu, &snu, &cnu);
         const double xp = a_m * cnu, yp = a_m * snu;
         const double y1 = yp * ci, z1 = yp * si;
         const double xe = xp * co - y1 * so;
         const double ye = xp * so + y1 * co;
         const double th = kOmegaE * (t - orbit_epoch_unix);
-        const double s = std::sin(th), c = std::cos(th);
+        double s, c;
+        ::sincos(th, &s, &c);
         xyz[i] = xe * c + ye * s;
         xyz[n + i] = -xe * s + ye * c;
         xyz[2 * n + i] = z1;

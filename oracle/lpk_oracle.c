@@ -7,6 +7,7 @@
  * contracts to FMA) does. Transcendentals (sin, cos, sqrt, pow, log) come from
  * glibc libm, which is also what the reference's f64::sin etc. call on Linux.
  */
+#define _GNU_SOURCE
 #include "lpk_oracle.h"
 
 #include <math.h>
@@ -68,11 +69,13 @@ static double to_radians(double d) { return d * (PI / 180.0); }
 void orc_geodetic_to_ecef(double lat_deg, double lon_deg, double h_m, double out[3]) {
     double lat = to_radians(lat_deg);
     double lon = to_radians(lon_deg);
-    double sin_lat = sin(lat);
-    double cos_lat = cos(lat);
+    double sin_lat, cos_lat;
+    sincos(lat, &sin_lat, &cos_lat);
+    double sin_lon, cos_lon;
+    sincos(lon, &sin_lon, &cos_lon);
     double n = WGS84_A / sqrt(1.0 - WGS84_E2 * sin_lat * sin_lat);
-    out[0] = (n + h_m) * cos_lat * cos(lon);
-    out[1] = (n + h_m) * cos_lat * sin(lon);
+    out[0] = (n + h_m) * cos_lat * cos_lon;
+    out[1] = (n + h_m) * cos_lat * sin_lon;
     out[2] = (n * (1.0 - WGS84_E2) + h_m) * sin_lat;
 }
 
@@ -83,10 +86,12 @@ void orc_kepler_position(double a_m, double inc_deg, double raan_deg, double nu0
     double n = sqrt(GM_EARTH / (a_m * a_m * a_m));   /* a.powi(3) = a*(a*a) = (a*a)*a */
     double dt = t_unix - epoch_unix;
     double nu = to_radians(nu0_deg) + n * dt;
-    double s_nu = sin(nu), c_nu = cos(nu);
+    double s_nu, c_nu;
+    sincos(nu, &s_nu, &c_nu);
     double inc = to_radians(inc_deg), raan = to_radians(raan_deg);
-    double s_i = sin(inc), c_i = cos(inc);
-    double s_o = sin(raan), c_o = cos(raan);
+    double s_i, c_i, s_o, c_o;
+    sincos(inc, &s_i, &c_i);
+    sincos(raan, &s_o, &c_o);
     double xp = r * c_nu;
     double yp = r * s_nu;
     double y1 = yp * c_i;
@@ -95,7 +100,8 @@ void orc_kepler_position(double a_m, double inc_deg, double raan_deg, double nu0
     double y_eci = xp * s_o + y1 * c_o;
     double z_eci = z1;
     double theta = EARTH_ROTATION_RATE_RAD_S * (t_unix - epoch_unix);
-    double s = sin(theta), c = cos(theta);
+    double s, c;
+    sincos(theta, &s, &c);
     out[0] = x_eci * c + y_eci * s;
     out[1] = -x_eci * s + y_eci * c;
     out[2] = z_eci;
@@ -301,8 +307,7 @@ static void baseband_one(const orc_tx *tx, double t, float *re, float *im) {
         float phase;
         if (!tx->cw_anchored) phase = (float)(omega * t);
         else phase = (float)rem_euclid_f64(omega * t + tx->anchor_phase_rad, TAU);
-        *re = cosf(phase);
-        *im = sinf(phase);
+        sincosf(phase, im, re);
         return;
     }
     case ORC_SIG_BPSK:
@@ -489,8 +494,10 @@ void orc_awgn_normals(uint64_t noise_seed, uint64_t i, float *z_re, float *z_im)
     float u2 = (float)(w[1] >> 40) * 0x1.0p-24f;
     float r = sqrtf(-2.0f * logf(u1));
     float th = 6.28318530717958647692f * u2;
-    *z_re = r * cosf(th);
-    *z_im = r * sinf(th);
+    float s_th, c_th;
+    sincosf(th, &s_th, &c_th);
+    *z_re = r * c_th;
+    *z_im = r * s_th;
 }
 
 /* ---------------------------------------------------------------- the clock */
@@ -602,7 +609,8 @@ static void tx_contribution(const orc_tx *tx, const double obs[3], const double 
         double r = ranges[i];
         double delay = r * inv_c;
         double phase = neg_tau_src * (delay + delta_t[i]) + tau_if * t_local[i];
-        double sp = sin(phase), cp = cos(phase);
+        double sp, cp;
+        sincos(phase, &sp, &cp);
         float amplitude = (float)(amp_factor / r);
         float bre = bb[2 * i], bim = bb[2 * i + 1];
         float cpf = (float)cp, spf = (float)sp;

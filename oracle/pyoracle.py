@@ -52,6 +52,11 @@ class OrcBand(C.Structure):
                 ("post", OrcPost * 4)]
 
 
+class OrcClock(C.Structure):
+    _fields_ = [("c0", _d), ("y0", _d), ("sigma_step", _d), ("master", _d), ("walk", C.c_int),
+                ("seed", _u64), ("frontier", _i64), ("eps_hi", _d), ("eps_lo", _d)]
+
+
 _LIB = None
 
 
@@ -92,6 +97,9 @@ def load() -> C.CDLL:
         "orc_clock_normal": (_d, [_u64, _u64]),
         "orc_awgn_normals": (None, [_u64, _u64, P(C.c_float), P(C.c_float)]),
         "orc_thermal_noise_power_w": (_d, [_d, _d]),
+        "orc_clock_init": (None, [P(OrcClock), P(OrcRx)]),
+        "orc_clock_delta_t_at": (_d, [P(OrcClock), _d]),
+        "orc_clock_delta_t_at_master_index": (_d, [P(OrcClock), _i64]),
         "orc_post_ops": (None, [P(OrcPost), _u32, P(C.c_float), _u64, P(C.c_int16)]),
         "orc_render_band": (C.c_int, [P(OrcTx), _u32, P(OrcRx), P(OrcBand), _u64, _u64, P(C.c_float),
                                       P(C.c_int16), _u32, P(_d), P(_d), P(_d), P(_d)]),

@@ -1,0 +1,324 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU FP64 oracle on
+identical prepared scenes, plus the reference's end-to-end invariants run on
+the GPU path. Tolerances:
+  - solved range / emission time / f64 carrier phase (taps): phase within
+    1e-6 rad and delay within 1 ps of the oracle (the plain form matches to
+    the last bit in practice);
+  - noise-free IQ: f32 ulp-bounded (|d| <= 1e-6 of the band's peak), and
+    >= 99 % of samples bit-identical;
+  - ADC codes: identical except where the oracle's pre-quantisation value is
+    within 1e-5 LSB of a rounding boundary;
+  - AWGN: same Philox draws, Box-Muller in f32 on both sides, so within a few
+    f32 ulps of sigma; statistics against the reference's bounds.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from lipika_amd import _lib, workloads
+from lipika_amd.config import AdcConfig, FrontendConfig, NonlinearityConfig, NodeClockSpec
+from lipika_amd.pipeline import DEFAULT_CHUNK_SAMPLES, prepare, run_scene
+from oracle.pyoracle import OracleScene
+
+import util
+
+pytestmark = pytest.mark.gpu
+
+C_LIGHT = 299792458.0
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def gpu_band(be, sc, w, r, b, seed, chunk=DEFAULT_CHUNK_SAMPLES, n=None, form=_lib.FORM_PLAIN, codes=False):
+    """Render unit w (receiver r, band b) of an uploaded scene chunk by chunk."""
+    pb = sc.receivers[r].bands[b]
+    n = pb.n if n is None else n
+    out = np.empty(n, dtype=np.complex64)
+    cd = np.empty(2 * n, dtype=np.int16) if codes else None
+    for i0 in range(0, n, chunk):
+        clen = min(chunk, n - i0)
+        j, keep = be.job(r, b, w, seed, i0, clen, out_host=out[i0:i0 + clen],
+                         codes=cd[2 * i0:2 * (i0 + clen)] if codes else None)
+        be.render([j], form)
+    be.sync()
+    return out, cd
+
+
+def unit_index(sc, r, b):
+    w = 0
+    for ri, rx in enumerate(sc.receivers):
+        for bi, _ in enumerate(rx.bands):
+            if (ri, bi) == (r, b):
+                return w
+            w += 1
+    raise KeyError
+
+
+def noise_free(scene):
+    for rx in scene.receivers:
+        for b in rx.bands:
+            b.noise_power_w = 0.0
+    return scene
+
+
+def compare_iq(gpu, ref, label):
+    peak = float(np.max(np.abs(ref))) if len(ref) else 0.0
+    d = np.abs(gpu.astype(np.complex128) - ref.astype(np.complex128))
+    same = np.mean((gpu.view(np.uint32) == ref.view(np.uint32)).reshape(-1, 2).all(axis=1)) if len(ref) else 1.0
+    _say(f"{label}: n={len(ref)} peak={peak:.3e} max|d|={d.max() if len(d) else 0:.3e} bit-identical={same:.6f}")
+    assert np.all(np.isfinite(gpu))
+    assert d.max() <= 1e-6 * peak + 1e-30
+    assert same >= 0.99
+    return same
+
+
+def compare_taps(be, orc_scene, sc, w, r, b, seed, slot, n, form=_lib.FORM_PLAIN, label=""):
+    _, _, taps = orc_scene.render_band(w, n=n, tap_slot=slot)
+    j, keep = be.job(r, b, w, seed, 0, n)
+    te, rg, ph, dt = be.taps(j, slot, form)
+    dphi = np.max(np.abs(ph - taps[2]))
+    ddel = np.max(np.abs(rg - taps[1])) / C_LIGHT
+    dte = np.max(np.abs(te - taps[0]))
+    ddt = np.max(np.abs(dt - taps[3]))
+    _say(f"{label} taps slot {slot}: max|dphi|={dphi:.3e} rad  max|ddelay|={ddel:.3e} s  "
+         f"max|dt_emit|={dte:.3e} s  max|d delta_t|={ddt:.3e} s")
+    assert dphi <= 1e-6
+    assert ddel <= 1e-12
+    assert dte <= 1e-12
+    assert ddt <= 1e-18
+    return dphi, ddel
+
+
+# ---------------------------------------------------------------- scenes vs oracle
+
+def test_config1_full_scene_noise_free(backend):
+    seed = 2026
+    scene = noise_free(workloads.scene1())
+    sc = prepare(scene)
+    backend.upload(sc, seed)
+    orc = OracleScene(sc, seed)
+    g, _ = gpu_band(backend, sc, 0, 0, 0, seed)
+    o, _, _ = orc.render_band(0)
+    compare_iq(g, o, "config 1 (10 s CW LEO)")
+    compare_taps(backend, orc, sc, 0, 0, 0, seed, 0, 200_000, label="config 1")
+
+
+def test_config2_prefix_both_bands(backend):
+    seed = 2026
+    scene = noise_free(workloads.scene2(duration_s=0.3))
+    sc = prepare(scene)
+    backend.upload(sc, seed)
+    orc = OracleScene(sc, seed)
+    for b, name in ((0, "UHF BPSK 48 kHz"), (1, "L1 6xC/A 2.048 MHz")):
+        g, _ = gpu_band(backend, sc, b, 0, b, seed, chunk=100_000)
+        o, _, _ = orc.render_band(b)
+        compare_iq(g, o, f"config 2 prefix, {name}")
+    for slot in range(6):
+        compare_taps(backend, orc, sc, 1, 0, 1, seed, slot, 150_000, label="config 2 L1")
+
+
+def test_all_signal_kinds_with_clock_walk(backend):
+    seed = 7
+    scene = util.all_kinds_scene(duration_s=0.01)
+    sc = prepare(scene)
+    backend.upload(sc, seed)
+    orc = OracleScene(sc, seed)
+    g, _ = gpu_band(backend, sc, 0, 0, 0, seed, chunk=30_000)
+    o, _, _ = orc.render_band(0)
+    compare_iq(g, o, "all kinds, 8 MHz")
+    for slot in range(5):
+        compare_taps(backend, orc, sc, 0, 0, 0, seed, slot, 40_000, label="all kinds")
+
+
+def test_awgn_matches_oracle_draws(backend):
+    seed = 99
+    scene = util.all_kinds_scene(duration_s=0.005, noise_power_w=None)
+    sc = prepare(scene)
+    backend.upload(sc, seed)
+    orc = OracleScene(sc, seed)
+    g, _ = gpu_band(backend, sc, 0, 0, 0, seed)
+    o, _, _ = orc.render_band(0)
+    sigma = math.sqrt(sc.receivers[0].bands[0].noise_power_w / 2.0)
+    d = np.abs(g.astype(np.complex128) - o.astype(np.complex128))
+    _say(f"AWGN: sigma={sigma:.3e} max|d|={d.max():.3e} ({d.max() / sigma:.2e} sigma)")
+    assert d.max() <= 1e-5 * sigma
+
+
+def test_front_end_ops_and_adc_codes(backend):
+    seed = 3
+    fe = FrontendConfig(gain_db=110.0, nonlinearity=NonlinearityConfig(30.0), adc=AdcConfig(8, 1.0))
+    scene = util.all_kinds_scene(duration_s=0.004, noise_power_w=None, frontend=fe)
+    sc = prepare(scene)
+    backend.upload(sc, seed)
+    orc = OracleScene(sc, seed)
+    g, gc = gpu_band(backend, sc, 0, 0, 0, seed, codes=True)
+    o, oc, _ = orc.render_band(0, codes=True)
+    lsb = 1.0 / 127.0
+    mism = np.nonzero(gc != oc)[0]
+    _say(f"ADC: {len(mism)} of {len(oc)} codes differ; codes span [{oc.min()}, {oc.max()}]")
+    assert oc.min() < -20 and oc.max() > 20          # the ADC is exercised, not idle
+    assert len(mism) <= 1e-3 * len(oc)
+    for k in mism:  # only at rounding boundaries: the codes differ by one level
+        assert abs(int(gc[k]) - int(oc[k])) == 1
+    flat_o = o.view(np.float32)
+    assert np.all(np.abs(flat_o) <= 1.0 + 1e-6)
+    assert np.all(np.abs(np.round(flat_o / lsb) * lsb - flat_o) < 1e-6)
+
+
+# ---------------------------------------------------------------- precision form
+
+def test_precision_form_error_vs_fp64_oracle(backend):
+    seed = 5
+    scene = noise_free(workloads.scene2(duration_s=0.1))
+    sc = prepare(scene)
+    backend.upload(sc, seed)
+    orc = OracleScene(sc, seed)
+    g, _ = gpu_band(backend, sc, 1, 0, 1, seed, form=_lib.FORM_PRECISION)
+    o, _, _ = orc.render_band(1)
+    peak = float(np.max(np.abs(o)))
+    d = np.abs(g.astype(np.complex128) - o.astype(np.complex128))
+    _say(f"precision form L1: max|d|/peak={d.max() / peak:.3e}")
+    # a 1e-6 rad phase error moves a unit phasor by 1e-6 of its amplitude
+    assert d.max() <= 2e-6 * peak
+    for slot in range(6):
+        compare_taps(backend, orc, sc, 1, 0, 1, seed, slot, 50_000, form=_lib.FORM_PRECISION,
+                     label="precision form")
+
+
+# ---------------------------------------------------------------- reference invariants on the GPU
+
+def test_chunk_size_invariance_bytes(backend):
+    """tests/streaming_chunk_invariance.rs:78-149 on the GPU path."""
+    scene = workloads.scene2(duration_s=0.25)
+    a = run_scene(scene, 11, chunk_samples=DEFAULT_CHUNK_SAMPLES, backend=backend)
+    for chunk in (65_536, 12_345):
+        b = run_scene(scene, 11, chunk_samples=chunk, backend=backend)
+        for x, y in zip(a, b):
+            assert x.iq.tobytes() == y.iq.tobytes(), (x.name, chunk)
+
+
+def test_additivity_both_equals_a_plus_b_minus_noise(backend, kats):
+    """tests/multi_transmitter_summation.rs:188-236"""
+    seed = 99
+    both = run_scene(util.two_tx_scene(("alpha", "beta")), seed, backend=backend)[0].iq
+    a = run_scene(util.two_tx_scene(("alpha",)), seed, backend=backend)[0].iq
+    b = run_scene(util.two_tx_scene(("beta",)), seed, backend=backend)[0].iq
+    n = run_scene(util.two_tx_scene(()), seed, backend=backend)[0].iq
+    err = np.max(np.abs(both - (a + b - n)))
+    _say(f"additivity: max|both - (a+b-n)| = {err:.3e}, mean|both| = {np.mean(np.abs(both)):.3e}")
+    assert err < kats["additivity"]["max_abs_err"]
+
+
+def test_clock_offset_130ns_recovered(backend, kats):
+    """tests/multi_transmitter_summation.rs:134-156"""
+    outs = {o.name: o.iq for o in run_scene(util.cw_two_clocks_scene(), 7, backend=backend)}
+    a = outs["rxb__b"].astype(np.complex128)
+    b = outs["rxa__b"].astype(np.complex128)
+    acc = np.sum(a * np.conj(b))
+    rec = -math.atan2(acc.imag, acc.real) / (2 * math.pi * 1e6)
+    _say(f"recovered clock offset {rec * 1e9:.4f} ns")
+    assert abs(rec - 1.3e-7) < kats["clock_offset_recovery"]["tolerance_s"]
+
+
+def test_noise_only_band_statistics(backend, kats):
+    """tests/multi_transmitter_summation.rs:175-186, tests/window_render.rs:431-458"""
+    k = kats["noise_power"]
+    iq = run_scene(util.two_tx_scene(()), 12345, backend=backend)[0].iq.astype(np.complex128)
+    p = np.mean(np.abs(iq) ** 2)
+    sigma = math.sqrt(k["power_w"] / 2)
+    lag1 = np.real(np.mean(iq[1:] * np.conj(iq[:-1]))) / p
+    _say(f"noise: power/1e-15={p / k['power_w']:.4f} |mean|/sigma={abs(np.mean(iq.real)) / sigma:.4f} lag1={lag1:.4f}")
+    assert abs(p / k["power_w"] - 1) < k["tolerance"]
+    assert abs(np.mean(iq.real)) < k["mean_sigma"] * sigma and abs(np.mean(iq.imag)) < k["mean_sigma"] * sigma
+    assert abs(lag1) < k["lag1"]
+
+
+def test_full_config2_scale_properties(backend):
+    """Config 2 at its full size (60 s, 125.76 M output samples) on the GPU:
+    determinism, chunk invariance through the FNV-1a sink, per-band power."""
+    from lipika_amd import workloads as wl
+    seed = 2026
+    scene = wl.scene2()
+    sc = prepare(scene)
+    backend.upload(sc, seed)
+    lib = _lib.load()
+    import ctypes as C
+    hashes = []
+    for chunk in (DEFAULT_CHUNK_SAMPLES, 3 << 18, DEFAULT_CHUNK_SAMPLES):
+        hs = []
+        for w, (r, b) in enumerate([(0, 0), (0, 1)]):
+            n = sc.receivers[r].bands[b].n
+            dev = C.c_void_p()
+            backend.check(lib.lpk_device_alloc(backend.h, n * 8, C.byref(dev)))
+            jobs, keep = [], []
+            for i0 in range(0, n, chunk):
+                j, k = backend.job(r, b, w, seed, i0, min(chunk, n - i0), out_dev=dev.value + i0 * 8)
+                jobs.append(j)
+                keep.append(k)
+            backend.render(jobs)
+            h = C.c_uint64()
+            backend.check(lib.lpk_device_fnv1a(backend.h, dev, n * 8, C.byref(h)))
+            if chunk == DEFAULT_CHUNK_SAMPLES and len(hashes) == 0:
+                tail = np.empty(1 << 16, dtype=np.complex64)
+                backend.check(lib.lpk_copy_to_host(backend.h, tail.ctypes.data, dev.value + (n - len(tail)) * 8,
+                                                   len(tail) * 8))
+                p = np.mean(np.abs(tail.astype(np.complex128)) ** 2)
+                pn = sc.receivers[r].bands[b].noise_power_w
+                _say(f"band {b}: tail power / kTB = {p / pn:.4f}")
+                assert np.all(np.isfinite(tail)) and 0.9 < p / pn < 3.0
+            backend.check(lib.lpk_device_free(backend.h, dev))
+            hs.append(h.value)
+        hashes.append(hs)
+        _say(f"chunk {chunk}: fnv1a {[hex(x) for x in hs]}")
+    assert hashes[0] == hashes[1] == hashes[2]
+
+
+# ---------------------------------------------------------------- error paths (forced from the host)
+
+def test_errors_are_codes_not_crashes(backend):
+    import ctypes as C
+    scene = workloads.scene1(duration_s=0.01)
+    sc = prepare(scene)
+    backend.upload(sc, 1)
+    j, keep = backend.job(0, 0, 0, 1, 0, 100, out_host=np.empty(100, dtype=np.complex64))
+    bad = np.array([5], dtype=np.uint32)
+    j.active = bad.ctypes.data_as(C.POINTER(C.c_uint32))
+    with pytest.raises(_lib.LpkError) as e:
+        backend.render([j])
+    assert e.value.code == -1
+    j2, keep2 = backend.job(0, 0, 0, 1, 0, 4, out_host=np.empty(4, dtype=np.complex64),
+                            delta_t=np.array([0.0, np.nan, 0.0, 0.0]))
+    with pytest.raises(_lib.LpkError) as e:
+        backend.render([j2])
+    assert e.value.code == -4
+    j3, keep3 = backend.job(0, 0, 0, 1, 0, 4, out_host=np.empty(4, dtype=np.complex64))
+    j3.fs_hz = 0.0
+    with pytest.raises(_lib.LpkError):
+        backend.render([j3])
+    # the context still works afterwards
+    j4, keep4 = backend.job(0, 0, 0, 1, 0, 100, out_host=np.empty(100, dtype=np.complex64))
+    backend.render([j4])
+    backend.sync()
+
+
+def test_host_delta_t_path_equals_device_clock(backend):
+    """A caller-supplied clock offset array (the reference's own ClockStream)
+    gives the same IQ as the device walk when it holds the same values."""
+    seed = 21
+    scene = workloads.scene2(duration_s=0.02)
+    sc = prepare(scene)
+    backend.upload(sc, seed)
+    n = sc.receivers[0].bands[1].n
+    j, keep = backend.job(0, 1, 1, seed, 0, n)
+    _, _, _, dt = backend.taps(j, 0)
+    a = np.empty(n, dtype=np.complex64)
+    b = np.empty(n, dtype=np.complex64)
+    ja, ka = backend.job(0, 1, 1, seed, 0, n, out_host=a)
+    jb, kb = backend.job(0, 1, 1, seed, 0, n, out_host=b, delta_t=dt)
+    backend.render([ja])
+    backend.render([jb])
+    backend.sync()
+    assert a.tobytes() == b.tobytes()

@@ -98,3 +98,11 @@ def all_kinds_scene(duration_s=0.02, noise_power_w=0.0, clock=None, frontend=Non
     band = ReceiverBandConfig("ism_915", 915e6, 8e6, 290.0, noise_power_w=noise_power_w, frontend=frontend)
     return SceneConfig(SceneMeta(EPOCH, duration_s), txs,
                        [lab_receiver([band], clock or NodeClockSpec(allan_deviation_1s=1e-11))])
+
+
+def pyoracle_clock(orc, rx):
+    """An initialised oracle ClockStream for receiver struct `rx`."""
+    c = pyoracle.OrcClock()
+    keep.append(rx)
+    orc.orc_clock_init(C.byref(c), C.byref(rx))
+    return c
