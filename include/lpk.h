@@ -172,6 +172,12 @@ int lpk_scene_upload(lpk_ctx *ctx, const lpk_tx_desc *tx, uint32_t n_tx,
  *      every job of the call runs in one kernel launch. Asynchronous. ---- */
 int lpk_render_chunk(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form);
 int lpk_sync(lpk_ctx *ctx);
+/* Forget every receiver's clock-walk progress, so the next render regenerates
+ * the walk from master index 0, as a fresh ClockStream does
+ * (src/clock/mod.rs:268-295, one per band in src/pipeline.rs:869-872). The
+ * values produced are unchanged; only the work is redone. Asynchronous: it
+ * changes host bookkeeping only, and the stream orders the work after it. */
+int lpk_clock_rewind(lpk_ctx *ctx);
 
 /* Diagnostic render of one job that also records, for active slot `slot`,
  * the solved emission time, range (src/channel/mod.rs:97-98), f64 carrier

@@ -107,6 +107,7 @@ def load() -> C.CDLL:
         "lpk_scene_upload": (_i32, [C.c_void_p, P(TxDesc), _u32, P(RxDesc), _u32]),
         "lpk_render_chunk": (_i32, [C.c_void_p, P(BandChunk), _u32, C.c_int]),
         "lpk_sync": (_i32, [C.c_void_p]),
+        "lpk_clock_rewind": (_i32, [C.c_void_p]),
         "lpk_render_taps": (_i32, [C.c_void_p, P(BandChunk), C.c_int, _u32, P(_d), P(_d), P(_d), P(_d)]),
         "lpk_last_kernel_ms": (_i32, [C.c_void_p, P(_d)]),
         "lpk_timing_reset": (_i32, [C.c_void_p]),
@@ -140,7 +141,7 @@ def load() -> C.CDLL:
 # Every symbol include/lpk.h declares (checked by the CPU tests).
 EXPORTED = (
     "lpk_abi_version", "lpk_ctx_create", "lpk_ctx_destroy", "lpk_last_error",
-    "lpk_scene_upload", "lpk_render_chunk", "lpk_sync", "lpk_render_taps",
+    "lpk_scene_upload", "lpk_render_chunk", "lpk_sync", "lpk_clock_rewind", "lpk_render_taps",
     "lpk_last_kernel_ms", "lpk_timing_reset", "lpk_timing_read", "lpk_device_alloc", "lpk_device_free", "lpk_copy_to_host",
     "lpk_device_fnv1a", "lpk_geodetic_to_ecef", "lpk_grid_size", "lpk_grid_fill_kepler",
     "lpk_ca_code", "lpk_nav_fold_seed", "lpk_payload_bpsk", "lpk_payload_ook",

@@ -18,10 +18,10 @@ hipError_t launch_render(int form, const JobDev *jobs, int n_jobs, const LinkDev
                          const TxDev *txs, uint32_t n_tiles, double *tap_t_emit, double *tap_range,
                          double *tap_phase, double *tap_delta_t, uint32_t tap_slot,
                          hipStream_t stream);
-hipError_t launch_walk_extend(uint64_t seed, double sigma, int64_t first_block, int64_t n_blocks,
-                              double *base, double *blocksum_scratch, hipStream_t stream);
-hipError_t launch_walk_materialize(uint64_t seed, double sigma, int64_t k_lo, int64_t k_hi,
-                                   const double *base, double *out, hipStream_t stream);
+hipError_t launch_walk_gen(uint64_t seed, double sigma, int64_t first_block, int64_t n_blocks, float *local,
+                           double *toff, double *blocksum, hipStream_t stream);
+hipError_t launch_walk_carry(double *base, int64_t first_block, int64_t n_blocks, const double *blocksum,
+                             hipStream_t stream);
 }  // namespace lpk
 
 using namespace lpk;
@@ -76,7 +76,7 @@ struct RxState {
     double *base = nullptr;     // base[b] = eps(b*kWalkBlock - 1), base[0] = 0
     int64_t base_cap = 0;       // entries
     int64_t nb_known = 0;       // base[0..nb_known] valid
-    DevBuf eps;                 // materialised window for the current call
+    DevBuf local, toff;         // the current call's window (f32 per index, f64 per 32)
 };
 
 }  // namespace
@@ -145,7 +145,8 @@ void free_scene(lpk_ctx *ctx) {
     for (auto &r : ctx->rx) {
         if (r.base) (void)hipFree(r.base);
         r.base = nullptr;
-        r.eps.release();
+        r.local.release();
+        r.toff.release();
     }
     ctx->rx.clear();
     ctx->tx_host.clear();
@@ -261,10 +262,10 @@ void pinned_mark(lpk_ctx *ctx, int slot_used) {
     hip_check(hipEventRecord(ctx->pin[slot_used].done, ctx->stream), "event record");
 }
 
-// Bring the receiver's walk base table up to block b_hi, then materialise
-// eps over [k_lo, k_hi].
+// Bring the receiver's walk base table up to the block of k_hi and build the
+// walk window covering [k_lo, k_hi] (whole blocks).
 void prepare_walk(lpk_ctx *ctx, RxState &R, int64_t k_lo, int64_t k_hi) {
-    const int64_t b_hi = k_hi / kWalkBlock;
+    const int64_t b_hi = k_hi >> kWalkBlockShift;
     if (b_hi + 1 > R.base_cap) {
         int64_t cap = std::max<int64_t>(R.base_cap ? R.base_cap : 4096, 4096);
         while (cap < b_hi + 2) cap *= 2;
@@ -284,16 +285,30 @@ void prepare_walk(lpk_ctx *ctx, RxState &R, int64_t k_lo, int64_t k_hi) {
         R.base = nb;
         R.base_cap = cap;
     }
-    if (R.nb_known < b_hi) {
-        const int64_t nadd = b_hi - R.nb_known;
-        double *bs = static_cast<double *>(ctx->blocksum.ensure(size_t(nadd) * sizeof(double), ctx->stream));
-        hip_check(launch_walk_extend(R.d.clock_seed, R.sigma_step, R.nb_known, nadd, R.base, bs, ctx->stream),
-                  "clock walk extend");
-        R.nb_known = b_hi;
+    const int64_t b_lo = k_lo >> kWalkBlockShift;
+    const int64_t n_win = b_hi - b_lo + 1;
+    double *bs = static_cast<double *>(
+        ctx->blocksum.ensure(size_t(std::max<int64_t>(n_win, b_lo - R.nb_known)) * sizeof(double), ctx->stream));
+    // blocks between the frontier and the window: block sums only
+    if (R.nb_known < b_lo) {
+        const int64_t gap = b_lo - R.nb_known;
+        hip_check(launch_walk_gen(R.d.clock_seed, R.sigma_step, R.nb_known, gap, nullptr, nullptr, bs, ctx->stream),
+                  "clock walk block sums");
+        hip_check(launch_walk_carry(R.base, R.nb_known, gap, bs, ctx->stream), "clock walk carry");
+        R.nb_known = b_lo;
     }
-    double *eps = static_cast<double *>(R.eps.ensure(size_t(k_hi - k_lo + 1) * sizeof(double), ctx->stream));
-    hip_check(launch_walk_materialize(R.d.clock_seed, R.sigma_step, k_lo, k_hi, R.base, eps, ctx->stream),
-              "clock walk materialise");
+    // the window: thread-local prefixes, thread offsets and block sums
+    float *loc = static_cast<float *>(R.local.ensure(size_t(n_win) * size_t(kWalkBlock) * sizeof(float), ctx->stream));
+    double *toff = static_cast<double *>(
+        R.toff.ensure(size_t(n_win) * size_t(kWalkBlock >> kWalkSubShift) * sizeof(double), ctx->stream));
+    hip_check(launch_walk_gen(R.d.clock_seed, R.sigma_step, b_lo, n_win, loc, toff, bs, ctx->stream),
+              "clock walk window");
+    if (R.nb_known <= b_hi) {
+        const int64_t first = R.nb_known;
+        hip_check(launch_walk_carry(R.base, first, b_hi + 1 - first, bs + (first - b_lo), ctx->stream),
+                  "clock walk carry");
+        R.nb_known = b_hi + 1;
+    }
 }
 
 struct TapReq {
@@ -387,7 +402,7 @@ void render(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form,
         const RxState &R = ctx->rx[B.rx];
         JobDev J{};
         J.tile0 = uint32_t(tiles);
-        J.n_tiles = uint32_t((B.clen + kTile - 1) / kTile);
+        J.n_tiles = B.clen ? uint32_t(((B.i0 + B.clen - 1) >> kTileShift) - (B.i0 >> kTileShift) + 1) : 0u;
         tiles += J.n_tiles;
         if (tiles > 0x7FFFFFFFull) throw Fail{LPK_E_INVALID, "render call too large for one launch"};
         J.link0 = uint32_t(link_off);
@@ -416,8 +431,10 @@ void render(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form,
         J.master = R.d.master_rate_hz;
         J.walk = (R.walk && !B.delta_t) ? 1 : 0;
         if (J.walk) {
-            J.eps = static_cast<const double *>(R.eps.p);
-            J.eps_k0 = wlo[B.rx];
+            J.walk_base = R.base;
+            J.walk_toff = static_cast<const double *>(R.toff.p);
+            J.walk_local = static_cast<const float *>(R.local.p);
+            J.eps_k0 = (wlo[B.rx] >> kWalkBlockShift) << kWalkBlockShift;
         }
         J.sigma = float(std::sqrt(B.noise_power_w / 2.0));  // src/channel/mod.rs:328
         J.noise_seed = B.noise_seed;
@@ -608,6 +625,14 @@ int lpk_render_taps(lpk_ctx *ctx, const lpk_band_chunk *job, int form, uint32_t 
     });
     taps.release();
     return rc;
+}
+
+int lpk_clock_rewind(lpk_ctx *ctx) {
+    if (!ctx) return LPK_E_INVALID;
+    // Host bookkeeping only: later walk kernels rewrite the table in stream
+    // order, after every earlier render has read it.
+    for (auto &r : ctx->rx) r.nb_known = 0;
+    return LPK_OK;
 }
 
 int lpk_sync(lpk_ctx *ctx) {

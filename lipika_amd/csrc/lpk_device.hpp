@@ -26,12 +26,18 @@ constexpr uint64_t kAwgnTag = 0x6177676E6E6F6973ULL;    // "awgnnois"
 
 // Clock walk: blocks of kWalkBlock master-grid increments; each of the
 // kWalkThreads threads of a block owns kWalkPerThread consecutive ones.
+// epsilon(k) = base[k >> 16] (f64) + toff[(k - k0) >> 5] (f64) + local[k - k0] (f32).
 constexpr int kWalkThreads = 256;
-constexpr int kWalkPerThread = 32;
-constexpr int64_t kWalkBlock = int64_t(kWalkThreads) * kWalkPerThread;
+constexpr int kWalkPerThread = 256;
+constexpr int kWalkSubShift = 5;
+constexpr int kWalkBlockShift = 16;
+constexpr int64_t kWalkBlock = int64_t(1) << kWalkBlockShift;
+static_assert(kWalkThreads * kWalkPerThread == (1 << kWalkBlockShift), "walk block shape");
 
-// Render tile: one workgroup of kTile lanes, one output sample per lane.
+// Render tile: one workgroup of kTile lanes, one output sample per lane; tiles
+// sit at absolute multiples of kTile in a band's sample index.
 constexpr int kTile = 256;
+constexpr int kTileShift = 8;
 
 struct TxDev {
     int32_t traj_kind;       // 0 stationary, 1 grid
@@ -89,8 +95,10 @@ struct JobDev {
     double obs[3];
     // clock
     const double *delta_t;   // host-provided offsets (device copy) or null
-    const double *eps;       // materialised walk, eps[k - eps_k0]
-    int64_t eps_k0;
+    const double *walk_base; // per 65 536-index block, absolute block index
+    const double *walk_toff; // per 256 indices of the window
+    const float *walk_local; // per index of the window
+    int64_t eps_k0;          // first master index of the window (block aligned)
     double c0, y0, master;
     int32_t walk;
     // AWGN and post-sum

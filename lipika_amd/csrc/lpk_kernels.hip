@@ -1,21 +1,32 @@
 // lpk_kernels.hip -- gfx950 kernels of the IQ-generation backend.
 //
 // render_kernel<FORM>: one workgroup renders one tile of kTile consecutive
-// output samples of one (receiver, band, chunk) job; each lane owns one
-// sample and walks the job's in-band transmitters in summation order:
+// output samples (tiles sit at absolute multiples of kTile in the band's
+// sample index) of one (receiver, band, chunk) job. Every tile of every job of
+// a render call is one launch. Each lane owns one sample and walks the job's
+// in-band transmitters in summation order:
 //   light-time + Sagnac solve (src/channel/mod.rs:70-99)
-//   -> s(t_emit) (src/signal/*.rs) -> FSPL amplitude and f64 carrier phase
-//   -> f32 mix (src/channel/mod.rs:251-268) -> ordered f32 sum
+//   -> s(t_emit) (src/signal/*.rs) -> FSPL amplitude and carrier phase
+//   -> f32 mix (src/channel/mod.rs:251-268) -> sum over transmitters
 //   (src/pipeline.rs:965-976) -> AWGN (src/channel/mod.rs:323-334)
 //   -> post-sum ops (src/rx/mod.rs:83-101) -> one coalesced complex64 store.
-// Every tile of every job of a render call is one launch.
 //
-// walk_*: the receiver clock's white-FM walk (src/clock/mod.rs:299-307) as a
-// fixed-shape block scan over the master grid, so every value depends only on
-// (seed, index), never on how the render was chunked.
+// FORM 0 (plain) is the reference's f64 path operation for operation. FORM 1
+// (precision) keeps the solve in f64 but evaluates it per tile and
+// transmitter: the smooth two-iteration map is fitted by a quadratic in the
+// reception time through three anchor samples, and each lane adds the exact
+// effect of the reference's rounding of its absolute query times (first
+// order, through the range rates); a tile whose query times cross a
+// trajectory grid node takes the exact per-sample solve instead. The phase
+// is formed in f64 in the reference's operation order and reduced exactly;
+// sincos, mix and a compensated sum run in f32.
 //
-// All code is compiled with -ffp-contract=off: the plain form keeps the
-// reference's rounding sequence (Rust never fuses to FMA).
+// walk_gen_kernel: the receiver clock's white-FM walk (src/clock/mod.rs:
+// 299-307) on the master grid, as fixed 65 536-index blocks; values depend
+// only on (seed, index), never on how the render is chunked.
+//
+// Everything is compiled with -ffp-contract=off so the plain form keeps the
+// reference's rounding; the precision form asks for its FMAs explicitly.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -34,19 +45,19 @@ __device__ __forceinline__ int64_t sat_i64(double x) {
     return static_cast<int64_t>(x);
 }
 
-// i64::rem_euclid for n > 0; a 32-bit path covers every index a real scene
+// i64::rem_euclid for n > 0, with a 32-bit path for every index a real scene
 // produces.
 __device__ __forceinline__ int64_t rem_euclid(int64_t a, int64_t n) {
     if (a >= INT32_MIN && a <= INT32_MAX && n <= INT32_MAX) {
-        int32_t r = static_cast<int32_t>(a) % static_cast<int32_t>(n);
+        const int32_t r = static_cast<int32_t>(a) % static_cast<int32_t>(n);
         return r < 0 ? r + n : r;
     }
-    int64_t r = a % n;
+    const int64_t r = a % n;
     return r < 0 ? r + n : r;
 }
 
 __device__ __forceinline__ double rem_euclid_f64(double a, double b) {
-    double r = fmod(a, b);
+    const double r = fmod(a, b);
     return r < 0.0 ? r + fabs(b) : r;
 }
 
@@ -69,40 +80,79 @@ __device__ __forceinline__ void philox4x64(uint64_t c[4], uint64_t k0, uint64_t 
     }
 }
 
-// Standard normal for master-grid index k of a clock walk (f64 Box-Muller).
-__device__ __forceinline__ double clock_normal(uint64_t seed, uint64_t k) {
-    uint64_t w[4] = {k, 0, 0, 0};
-    philox4x64(w, seed, kClockTag);
-    const double u1 = (static_cast<double>(w[0] >> 11) + 0.5) * 0x1.0p-53;
-    const double u2 = static_cast<double>(w[1] >> 11) * 0x1.0p-53;
-    return sqrt(-2.0 * log(u1)) * cos(kTau * u2);
-}
+// Box-Muller pair from one 64-bit Philox word: u1 = (m1 + 1/2) 2^-24 from bits
+// 63..40 and the angle u2 = m2 2^-24 turns from bits 39..16. The pair is
+// evaluated in f64 with correctly rounded operations only (+ - * / sqrt, no
+// FMA, no libm), so every IEEE-754 machine yields the same bits; each value is
+// then rounded once to f32.
+//   ln u1: u1 = n 2^-25 with n = 2 m1 + 1 odd; n = f 2^e with f in
+//   [sqrt(1/2), sqrt(2)] split exactly; ln f = 2 atanh(s), s = (f-1)/(f+1),
+//   |s| <= 0.1716, by its series through s^13 (truncation < 2e-13 relative).
+//   sin/cos of 2 pi u2: the quadrant (top 2 bits) and the reflection about
+//   pi/4 are exact integer steps; sin and cos of the remaining angle
+//   (<= pi/4) come from their Taylor series through the 13th and 14th powers.
+__device__ __forceinline__ void box_muller(uint64_t w, float &zc, float &zs) {
+    const uint32_t n = (static_cast<uint32_t>(w >> 40) << 1) | 1u;
+    int e = 31 - __clz(static_cast<int>(n));
+    double f = ldexp(static_cast<double>(n), -e);
+    if (f > 1.4142135623730951) {
+        f = f * 0.5;
+        e += 1;
+    }
+    const double s = (f - 1.0) / (f + 1.0);
+    const double z = s * s;
+    double p = 1.0 / 13.0;
+    p = p * z + 1.0 / 11.0;
+    p = p * z + 1.0 / 9.0;
+    p = p * z + 1.0 / 7.0;
+    p = p * z + 1.0 / 5.0;
+    p = p * z + 1.0 / 3.0;
+    p = p * z + 1.0;
+    const double ln_u1 = static_cast<double>(e - 25) * 0.6931471805599453 + (2.0 * s) * p;
+    const double r = sqrt(-2.0 * ln_u1);
 
-// Two standard normals for global sample i of a band's noise (f32 Box-Muller).
-__device__ __forceinline__ void awgn_normals(uint64_t seed, uint64_t i, float &zr, float &zi) {
-    uint64_t w[4] = {i, 0, 0, 0};
-    philox4x64(w, seed, kAwgnTag);
-    const float u1 = (static_cast<float>(w[0] >> 40) + 0.5f) * 0x1.0p-24f;
-    const float u2 = static_cast<float>(w[1] >> 40) * 0x1.0p-24f;
-    const float r = sqrtf(-2.0f * logf(u1));
-    const float th = 6.28318530717958647692f * u2;
-    float s, c;
-    sincosf(th, &s, &c);
-    zr = r * c;
-    zi = r * s;
+    const uint32_t m2 = static_cast<uint32_t>(w >> 16) & 0xFFFFFFu;
+    const uint32_t quad = m2 >> 22, v = m2 & 0x3FFFFFu;
+    const bool flip = v > 0x200000u;
+    const double th = static_cast<double>(flip ? 0x400000u - v : v) * (1.5707963267948966 * 0x1.0p-22);
+    const double t2 = th * th;
+    double ps = 1.0 / 6227020800.0;
+    ps = ps * t2 - 1.0 / 39916800.0;
+    ps = ps * t2 + 1.0 / 362880.0;
+    ps = ps * t2 - 1.0 / 5040.0;
+    ps = ps * t2 + 1.0 / 120.0;
+    ps = ps * t2 - 1.0 / 6.0;
+    double pc = -1.0 / 87178291200.0;
+    pc = pc * t2 + 1.0 / 479001600.0;
+    pc = pc * t2 - 1.0 / 3628800.0;
+    pc = pc * t2 + 1.0 / 40320.0;
+    pc = pc * t2 - 1.0 / 720.0;
+    pc = pc * t2 + 1.0 / 24.0;
+    pc = pc * t2 - 0.5;
+    const double sn = th + (th * t2) * ps;
+    const double cs = 1.0 + t2 * pc;
+    const double a = flip ? cs : sn, b = flip ? sn : cs;  // sin, cos within the quadrant
+    double sv, cv;
+    if (quad == 0) { sv = a; cv = b; }
+    else if (quad == 1) { sv = b; cv = -a; }
+    else if (quad == 2) { sv = -a; cv = -b; }
+    else { sv = -b; cv = a; }
+    zc = static_cast<float>(r * cv);
+    zs = static_cast<float>(r * sv);
 }
 
 // ---------------------------------------------------------------- geometry
 
 // PrecomputedTrajectory::position_ecef (src/geometry/precomputed.rs:111-121):
-// the clamp of floor(f) to [0, n-2] is done in f64, which gives exactly
-// Rust's saturating cast + clamp (NaN -> 0 included).
+// the clamp of floor(f) to [0, n-2] is done in f64, which is exactly Rust's
+// saturating cast + clamp (NaN -> 0 included). `cell` gets the node index.
 __device__ __forceinline__ void tx_position(const TxDev &T, double t_abs, double &x, double &y,
-                                            double &z) {
+                                            double &z, int64_t &cell) {
     if (T.traj_kind == 0) {
         x = T.pos[0];
         y = T.pos[1];
         z = T.pos[2];
+        cell = 0;
         return;
     }
     const double f = (t_abs - T.grid_t0) / T.grid_dt;
@@ -116,17 +166,23 @@ __device__ __forceinline__ void tx_position(const TxDev &T, double t_abs, double
     x = g[lo] * omf + g[lo + 1] * frac;
     y = g[n + lo] * omf + g[n + lo + 1] * frac;
     z = g[2 * n + lo] * omf + g[2 * n + lo + 1] * frac;
+    cell = lo;
 }
 
-// solve_emission_times_for, one sample (src/channel/mod.rs:70-99).
+// solve_emission_times_for, one sample (src/channel/mod.rs:70-99). cell0 and
+// cell1 are the grid nodes the two iterations' queries fell in.
 __device__ __forceinline__ void solve_emission(const TxDev &T, const double *obs, double tu,
-                                               double epoch, double &t_emit, double &r) {
+                                               double epoch, double &t_emit, double &r,
+                                               int64_t &cell0, int64_t &cell1) {
     double t = tu - kEmissionInitialLag;
     r = 0.0;
 #pragma unroll
     for (int it = 0; it < kNewtonIterations; ++it) {
         double sx, sy, sz;
-        tx_position(T, epoch + t, sx, sy, sz);
+        int64_t cell;
+        tx_position(T, epoch + t, sx, sy, sz, cell);
+        if (it == 0) cell0 = cell;
+        else cell1 = cell;
         const double wtau = kEarthRotation * (tu - t);
         const double sx_rot = sx + wtau * sy;
         const double sy_rot = sy - wtau * sx;
@@ -181,11 +237,10 @@ __device__ __forceinline__ void baseband(const TxDev &T, double t, float &re, fl
     im = 0.0f;
     if (T.gated && !gate_on(T, t)) return;
     switch (T.signal_kind) {
-    case 0: {  // CW, src/signal/cw.rs:41-52
+    case 0: {  // CW, src/signal/cw.rs:41-52 (cos, sin of one f32 phase: one sincosf)
         const float ph = T.cw_anchored ? static_cast<float>(rem_euclid_f64(T.omega * t + T.anchor_phase, kTau))
                                        : static_cast<float>(T.omega * t);
-        re = cosf(ph);
-        im = sinf(ph);
+        sincosf(ph, &im, &re);
         return;
     }
     case 1:    // BPSK, src/signal/bpsk.rs:58-71
@@ -225,21 +280,27 @@ __device__ __forceinline__ void baseband(const TxDev &T, double t, float &re, fl
 
 // ---------------------------------------------------------------- clock
 
-// ClockStream::delta_t_at (src/clock/mod.rs:329-346) over the materialised
-// walk eps[k - eps_k0] = epsilon(k); clock start epoch 0.0.
+// epsilon(k) of the materialised walk window (k >= 0).
+__device__ __forceinline__ double walk_eps(const JobDev &J, int64_t k) {
+    const int64_t rel = k - J.eps_k0;
+    return J.walk_base[k >> kWalkBlockShift] + J.walk_toff[rel >> kWalkSubShift] +
+           static_cast<double>(J.walk_local[rel]);
+}
+
+// ClockStream::delta_t_at (src/clock/mod.rs:329-346); clock start epoch 0.0.
 __device__ __forceinline__ double clock_delta(const JobDev &J, double t) {
     const double elapsed = t - 0.0;
     const double x = elapsed * J.master;
     if (x <= 0.0) {
-        const double e0 = J.walk ? J.eps[0 - J.eps_k0] : 0.0;
+        const double e0 = J.walk ? walk_eps(J, 0) : 0.0;
         return (e0 + 0.0) + (J.c0 + J.y0 * 0.0);
     }
     const int64_t k0 = static_cast<int64_t>(floor(x));
     const double frac = x - static_cast<double>(k0);
     double elo = 0.0, ehi = 0.0;
     if (J.walk) {
-        elo = J.eps[k0 - J.eps_k0];
-        ehi = J.eps[k0 + 1 - J.eps_k0];
+        elo = walk_eps(J, k0);
+        ehi = walk_eps(J, k0 + 1);
     }
     const double eps = elo * (1.0 - frac) + ehi * frac;
     return (eps + 0.0) + (J.c0 + J.y0 * elapsed);
@@ -251,6 +312,88 @@ struct TapOut {
     double *t_emit, *range, *phase, *delta_t;
     uint32_t slot, on;
 };
+
+// Per (tile, transmitter) data of the precision form. The solve is fitted
+// in two parts: the smooth map (trajectory queried at unrounded times), a
+// quadratic in tu through three anchor samples, plus the first-order effect
+// of the reference's rounding of the absolute query time epoch + t
+// (src/channel/mod.rs:79), which the lanes compute exactly per sample.
+struct Anchor {
+    double r0s[3], r1s[3];  // smooth ranges of iterations 1 and 2 at the anchors
+    double q0d1, q0c2;      // r0s(tu) = r0s_m + x (q0d1 + q0c2 y)
+    double q1d1, q1c2;      // r1s(tu) likewise
+    double rd0;             // d r0 / d (query time) at the middle anchor
+    double rd1q;            // d r1 / d (query time), position only
+    double k01;             // (d r1 / d t1, position and Sagnac) * rd0 / c
+    double r1_mid;          // range at the middle sample (model, or exact when not smooth)
+    double r1x_mid;         // exact (reference arithmetic) range at the middle sample
+    double inv_rm;          // 1 / r1_mid
+    int64_t cell[2][2];     // grid nodes of both iterations at the first and last anchors
+    float amp_m;            // (float)(amp_factor / r1_mid)
+    int smooth;             // the fit is valid across the tile
+};
+constexpr int kAnchorGroup = 64;  // transmitters per anchor pass
+
+// The two-iteration solve with the trajectory queried at unrounded times
+// ((epoch - t0) + t instead of epoch + t); also returns the node cells and
+// the range rates the rounding correction needs.
+__device__ __forceinline__ void solve_smooth(const TxDev &T, const double *obs, double tu, double eoff,
+                                             double &r0, double &r1, int64_t &c0, int64_t &c1,
+                                             double &rd0, double &rd1q, double &rd1t) {
+    double t = tu - kEmissionInitialLag;
+    double rr[2], rq[2], rt[2];
+    int64_t cc[2];
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+        double sx, sy, sz, vx = 0.0, vy = 0.0, vz = 0.0;
+        int64_t cell = 0;
+        if (T.traj_kind == 0) {
+            sx = T.pos[0];
+            sy = T.pos[1];
+            sz = T.pos[2];
+        } else {
+            const double f = (eoff + t) / T.grid_dt;
+            const double flo = fmin(fmax(floor(f), 0.0), static_cast<double>(T.grid_n - 2));
+            const int64_t lo = static_cast<int64_t>(flo);
+            double frac = f - flo;
+            frac = frac < 0.0 ? 0.0 : (frac > 1.0 ? 1.0 : frac);
+            const double *g = T.grid;
+            const int64_t n = T.grid_n;
+            const double ax = g[lo], ay = g[n + lo], az = g[2 * n + lo];
+            const double bx = g[lo + 1], by = g[n + lo + 1], bz = g[2 * n + lo + 1];
+            sx = fma(bx - ax, frac, ax);
+            sy = fma(by - ay, frac, ay);
+            sz = fma(bz - az, frac, az);
+            const double idt = 1.0 / T.grid_dt;
+            vx = (bx - ax) * idt;
+            vy = (by - ay) * idt;
+            vz = (bz - az) * idt;
+            cell = lo;
+        }
+        const double w = kEarthRotation * (tu - t);
+        const double dx = fma(w, sy, sx) - obs[0];
+        const double dy = fma(-w, sx, sy) - obs[1];
+        const double dz = sz - obs[2];
+        const double r = sqrt(fma(dx, dx, fma(dy, dy, dz * dz)));
+        const double ir = 1.0 / r;
+        // d/d(query time) of the rotated position: V + w J V; of the Sagnac
+        // angle: -omega J S, with J (x, y, z) = (y, -x, 0)
+        const double pq = (dx * fma(w, vy, vx) + dy * fma(-w, vx, vy) + dz * vz) * ir;
+        const double pw = (dx * sy - dy * sx) * ir * -kEarthRotation;
+        rr[it] = r;
+        rq[it] = pq;
+        rt[it] = pq + pw;
+        cc[it] = cell;
+        t = tu - r * kInvC;
+    }
+    r0 = rr[0];
+    r1 = rr[1];
+    c0 = cc[0];
+    c1 = cc[1];
+    rd0 = rq[0];
+    rd1q = rq[1];
+    rd1t = rt[1];
+}
 
 // Workgroups are dealt round-robin to the 8 XCDs; give each XCD a contiguous
 // run of tiles so neighbouring tiles of a job share that XCD's L2.
@@ -264,6 +407,10 @@ __global__ void __launch_bounds__(kTile) render_kernel(const JobDev *__restrict_
                                                        const LinkDev *__restrict__ links,
                                                        const TxDev *__restrict__ txs,
                                                        uint32_t n_tiles, TapOut tap) {
+    __shared__ uint64_t s_noise[kTile];              // one Philox word per sample
+    __shared__ double s_anc_t[3][3];                  // tu, t_local, delta_t at the anchors
+    __shared__ Anchor s_anc[FORM ? kAnchorGroup : 1];
+
     const uint32_t tile = xcd_tile(blockIdx.x, n_tiles);
     int lo = 0, hi = n_jobs - 1;
     while (lo < hi) {
@@ -272,76 +419,195 @@ __global__ void __launch_bounds__(kTile) render_kernel(const JobDev *__restrict_
         else hi = mid - 1;
     }
     const JobDev &J = jobs[lo];
-    const uint64_t j = static_cast<uint64_t>(tile - J.tile0) * kTile + threadIdx.x;
-    if (j >= J.clen) return;
-    const uint64_t i = J.i0 + j;
+    const uint64_t tile_start = ((J.i0 >> kTileShift) + (tile - J.tile0)) << kTileShift;
+    const uint64_t i = tile_start + threadIdx.x;
+    const bool valid = i >= J.i0 && i < J.i0 + J.clen;
+    const uint64_t j = i - J.i0;
 
-    // src/pipeline.rs:918-926
-    const double t_local = static_cast<double>(i) / J.fs;
-    const double dlt = J.delta_t ? J.delta_t[j] : clock_delta(J, t_local);
+    // AWGN draws: the first wavefront computes the tile's kTile/4 Philox
+    // blocks (4 samples per block) into LDS; consumed after the transmitter loop.
+    const bool noisy = J.sigma > 0.0f;
+    if (noisy && threadIdx.x < kTile / 4) {
+        uint64_t w[4] = {(tile_start >> 2) + threadIdx.x, 0, 0, 0};
+        philox4x64(w, J.noise_seed, kAwgnTag);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) s_noise[4 * threadIdx.x + q] = w[q];
+    }
+
+    // src/pipeline.rs:918-926 (clamped to a valid sample on the tile's edge lanes)
+    const uint64_t first = J.i0 > tile_start ? J.i0 : tile_start;
+    const uint64_t end = (J.i0 + J.clen) < tile_start + kTile ? (J.i0 + J.clen) : tile_start + kTile;
+    const uint64_t ic = valid ? i : first;
+    const double t_local = static_cast<double>(ic) / J.fs;
+    const double dlt = J.delta_t ? J.delta_t[ic - J.i0] : clock_delta(J, t_local);
     const double tu = t_local - dlt;
 
     float acc_re = 0.0f, acc_im = 0.0f;
     float cmp_re = 0.0f, cmp_im = 0.0f;  // Neumaier compensation (precision form)
-    for (uint32_t s = 0; s < J.n_active; ++s) {
-        const LinkDev L = links[J.link0 + s];
-        const TxDev &T = txs[L.tx];
-        double t_emit, r;
-        solve_emission(T, J.obs, tu, J.epoch, t_emit, r);
-        float bre, bim;
-        baseband(T, t_emit, bre, bim);
-        // src/channel/mod.rs:255-267
-        const double delay = r * kInvC;
-        const double phase = T.neg_tau_src * (delay + dlt) + L.tau_if * t_local;
-        float cpf, spf;
-        if (FORM == 0) {
+
+    if (FORM == 0) {
+        for (uint32_t s = 0; s < J.n_active; ++s) {
+            const LinkDev L = links[J.link0 + s];
+            const TxDev &T = txs[L.tx];
+            double t_emit, r;
+            int64_t c0, c1;
+            solve_emission(T, J.obs, tu, J.epoch, t_emit, r, c0, c1);
+            float bre, bim;
+            baseband(T, t_emit, bre, bim);
+            // src/channel/mod.rs:255-267
+            const double delay = r * kInvC;
+            const double phase = T.neg_tau_src * (delay + dlt) + L.tau_if * t_local;
             double sp, cp;
             sincos(phase, &sp, &cp);
-            cpf = static_cast<float>(cp);
-            spf = static_cast<float>(sp);
-        } else {
-            // Range-reduce the f64 phase to about [-pi, pi] with a three-part
-            // Cody-Waite split of 2*pi (24 + 24 + 53 significant bits, so the
-            // first two products are exact for |k| < 2^29), then f32 sincos.
-            const double k = rint(phase * 0.15915494309189535);
-            double rr = phase - k * 6.2831854820251465;
-            rr = rr - k * -1.7484555314695172e-07;
-            rr = rr - k * -6.8604979977715316e-15;
-            sincosf(static_cast<float>(rr), &spf, &cpf);
+            const float cpf = static_cast<float>(cp), spf = static_cast<float>(sp);
+            const float amplitude = static_cast<float>(T.amp_factor / r);
+            acc_re = acc_re + amplitude * (bre * cpf - bim * spf);
+            acc_im = acc_im + amplitude * (bre * spf + bim * cpf);
+            if (tap.on && s == tap.slot && valid) {
+                tap.t_emit[j] = t_emit;
+                tap.range[j] = r;
+                tap.phase[j] = phase;
+            }
         }
-        const float amplitude = static_cast<float>(T.amp_factor / r);
-        const float cre = amplitude * (bre * cpf - bim * spf);
-        const float cim = amplitude * (bre * spf + bim * cpf);
-        if (FORM == 0) {
-            acc_re = acc_re + cre;
-            acc_im = acc_im + cim;
-        } else {
-            // Neumaier TwoSum accumulation across transmitters.
-            const float tr = acc_re + cre, ti = acc_im + cim;
-            cmp_re += fabsf(acc_re) >= fabsf(cre) ? (acc_re - tr) + cre : (cre - tr) + acc_re;
-            cmp_im += fabsf(acc_im) >= fabsf(cim) ? (acc_im - ti) + cim : (cim - ti) + acc_im;
-            acc_re = tr;
-            acc_im = ti;
+    } else {
+        const int a_first = static_cast<int>(first - tile_start);
+        const int a_last = static_cast<int>(end - tile_start) - 1;
+        const int a_mid = (a_first + a_last) >> 1;
+        // a lane may be several anchors when the tile holds fewer than 3 samples
+        const int me = static_cast<int>(threadIdx.x);
+        for (int a = 0; a < 3; ++a) {
+            if (me == (a == 0 ? a_first : (a == 1 ? a_mid : a_last))) {
+                s_anc_t[a][0] = tu;
+                s_anc_t[a][1] = t_local;
+                s_anc_t[a][2] = dlt;
+            }
         }
-        if (tap.on && s == tap.slot && lo == 0) {
-            tap.t_emit[j] = t_emit;
-            tap.range[j] = r;
-            tap.phase[j] = phase;
+        __syncthreads();
+        const double tu_a = s_anc_t[0][0], tu_m = s_anc_t[1][0];
+        const double x = tu - tu_m, y = tu - tu_a;
+        // rounding of the iteration-1 query time, exact (Sterbenz) per sample
+        const double t0q = tu - kEmissionInitialLag;
+        const double del0 = ((J.epoch + t0q) - J.epoch) - t0q;
+        const bool fit_ok = a_last - a_first >= 2;
+        for (uint32_t g0 = 0; g0 < J.n_active; g0 += kAnchorGroup) {
+            const uint32_t ng = J.n_active - g0 < kAnchorGroup ? J.n_active - g0 : kAnchorGroup;
+            // smooth solves at the three anchors of every transmitter of the
+            // group, and the exact solve at the middle one
+            for (uint32_t q = threadIdx.x; q < 4 * ng; q += kTile) {
+                const uint32_t s = q >> 2, a = q & 3;
+                const LinkDev L = links[J.link0 + g0 + s];
+                const TxDev &T = txs[L.tx];
+                Anchor &A = s_anc[s];
+                if (a == 3) {
+                    double te, r;
+                    int64_t c0, c1;
+                    solve_emission(T, J.obs, s_anc_t[1][0], J.epoch, te, r, c0, c1);
+                    A.r1x_mid = r;
+                } else {
+                    const double eoff = T.traj_kind ? J.epoch - T.grid_t0 : 0.0;
+                    double r0, r1, rd0, rd1q, rd1t;
+                    int64_t c0, c1;
+                    solve_smooth(T, J.obs, s_anc_t[a][0], eoff, r0, r1, c0, c1, rd0, rd1q, rd1t);
+                    A.r0s[a] = r0;
+                    A.r1s[a] = r1;
+                    if (a != 1) {
+                        A.cell[a >> 1][0] = c0;
+                        A.cell[a >> 1][1] = c1;
+                    } else {
+                        A.rd0 = rd0;
+                        A.rd1q = rd1q;
+                        A.k01 = rd1t * rd0 * kInvC;
+                    }
+                }
+            }
+            __syncthreads();
+            for (uint32_t s = threadIdx.x; s < ng; s += kTile) {
+                Anchor &A = s_anc[s];
+                const LinkDev L = links[J.link0 + g0 + s];
+                const TxDev &T = txs[L.tx];
+                const bool same = A.cell[0][0] == A.cell[1][0] && A.cell[0][1] == A.cell[1][1];
+                A.smooth = fit_ok && (T.traj_kind == 0 || same);
+                double r1m = A.r1x_mid;
+                if (A.smooth) {
+                    const double xa = s_anc_t[0][0] - s_anc_t[1][0], xb = s_anc_t[2][0] - s_anc_t[1][0];
+                    const double a0 = (A.r0s[1] - A.r0s[0]) / (-xa), b0 = (A.r0s[2] - A.r0s[1]) / xb;
+                    const double a1 = (A.r1s[1] - A.r1s[0]) / (-xa), b1 = (A.r1s[2] - A.r1s[1]) / xb;
+                    A.q0d1 = a0;
+                    A.q0c2 = (b0 - a0) / (xb - xa);
+                    A.q1d1 = a1;
+                    A.q1c2 = (b1 - a1) / (xb - xa);
+                    // the model at the middle sample (x = 0)
+                    const double t0m = s_anc_t[1][0] - kEmissionInitialLag;
+                    const double d0m = ((J.epoch + t0m) - J.epoch) - t0m;
+                    const double r0m = fma(A.rd0, d0m, A.r0s[1]);
+                    const double t1m = fma(-r0m, kInvC, s_anc_t[1][0]);
+                    const double d1m = ((J.epoch + t1m) - J.epoch) - t1m;
+                    r1m = fma(A.rd1q, d1m, fma(-A.k01, d0m, A.r1s[1]));
+                }
+                A.r1_mid = r1m;
+                A.inv_rm = 1.0 / r1m;
+                A.amp_m = static_cast<float>(T.amp_factor / r1m);
+            }
+            __syncthreads();
+            for (uint32_t s = 0; s < ng; ++s) {
+                const LinkDev L = links[J.link0 + g0 + s];
+                const TxDev &T = txs[L.tx];
+                const Anchor &A = s_anc[s];
+                double r;
+                if (A.smooth) {
+                    const double r0 = fma(A.rd0, del0, fma(x, fma(A.q0c2, y, A.q0d1), A.r0s[1]));
+                    const double t1 = fma(-r0, kInvC, tu);
+                    const double del1 = ((J.epoch + t1) - J.epoch) - t1;
+                    r = fma(A.rd1q, del1, fma(-A.k01, del0, fma(x, fma(A.q1c2, y, A.q1d1), A.r1s[1])));
+                } else {
+                    double te;
+                    int64_t c0, c1;
+                    solve_emission(T, J.obs, tu, J.epoch, te, r, c0, c1);
+                }
+                const double dr = r - A.r1_mid;
+                const double t_emit = fma(-r, kInvC, tu);
+                float bre, bim;
+                baseband(T, t_emit, bre, bim);
+                // src/channel/mod.rs:256-258 in the reference's operation order,
+                // so its f64 rounding is followed; then an exact Cody-Waite
+                // reduction (24 + 24 + 53-bit split of 2 pi, exact for |k| < 2^29)
+                const double phase = T.neg_tau_src * (r * kInvC + dlt) + L.tau_if * t_local;
+                const double kk = rint(phase * 0.15915494309189535);
+                const double rr = fma(-kk, -6.8604979977715316e-15,
+                                      fma(-kk, -1.7484555314695172e-07, fma(-kk, 6.2831854820251465, phase)));
+                const float fr = static_cast<float>(rr * 0.15915494309189535);
+                float spf, cpf;
+                sincospif(2.0f * fr, &spf, &cpf);
+                const float amplitude = A.amp_m * (1.0f - static_cast<float>(dr * A.inv_rm));
+                const float cre = amplitude * (bre * cpf - bim * spf);
+                const float cim = amplitude * (bre * spf + bim * cpf);
+                const float tr = acc_re + cre, ti = acc_im + cim;
+                cmp_re += fabsf(acc_re) >= fabsf(cre) ? (acc_re - tr) + cre : (cre - tr) + acc_re;
+                cmp_im += fabsf(acc_im) >= fabsf(cim) ? (acc_im - ti) + cim : (cim - ti) + acc_im;
+                acc_re = tr;
+                acc_im = ti;
+                if (tap.on && g0 + s == tap.slot && valid) {
+                    tap.t_emit[j] = t_emit;
+                    tap.range[j] = r;
+                    tap.phase[j] = phase;
+                }
+            }
+            __syncthreads();
         }
-    }
-    if (FORM != 0) {
         acc_re = acc_re + cmp_re;
         acc_im = acc_im + cmp_im;
     }
-    if (tap.on && lo == 0) tap.delta_t[j] = dlt;
+    if (tap.on && valid) tap.delta_t[j] = dlt;
 
     // AWGN, src/channel/mod.rs:328-333 (sigma = sqrt(P/2) as f32)
-    if (J.sigma > 0.0f) {
+    if (noisy) {
+        __syncthreads();
         float zr, zi;
-        awgn_normals(J.noise_seed, i, zr, zi);
+        box_muller(s_noise[threadIdx.x], zr, zi);
         acc_re += J.sigma * zr;
         acc_im += J.sigma * zi;
     }
+    if (!valid) return;
     // RxChain::apply_post_sum, src/rx/mod.rs:142-146 and :83-101
     int16_t code_re = 0, code_im = 0;
     for (int p = 0; p < J.n_post; ++p) {
@@ -370,18 +636,48 @@ __global__ void __launch_bounds__(kTile) render_kernel(const JobDev *__restrict_
 
 // ---------------------------------------------------------------- clock walk
 
-// Sum of the kWalkPerThread increments a thread owns, sequentially.
-__device__ __forceinline__ double walk_thread_sum(uint64_t seed, double sigma, int64_t k0) {
-    double s = 0.0;
-    for (int q = 0; q < kWalkPerThread; ++q) s = s + (0.0 + sigma * clock_normal(seed, k0 + q));
-    return s;
-}
-
-// Fixed-tree inclusive scan of one double per thread over a 256-thread block
-// (Hillis-Steele in LDS); the same tree on every call.
-__device__ __forceinline__ double block_inclusive_scan(double v, double *lds) {
+// One 65 536-index block of the walk per workgroup: each thread owns 256
+// consecutive increments (64 Philox blocks, 8 normals per block: master index
+// k uses block k >> 3, word (k >> 1) & 3, and the cos (even k) or sin (odd k)
+// branch of its Box-Muller pair) and sums them sequentially in f64. With
+// `local` set it stores, per index, the prefix since the start of its
+// 32-index sub-block as f32 (a value a few sigma*sqrt(32) in size), and per
+// sub-block the f64 offset of that start within the walk block (toff); a
+// fixed-tree scan of the 256 thread totals places each thread in the block,
+// and the block total goes to blocksum.
+__global__ void __launch_bounds__(kWalkThreads) walk_gen_kernel(uint64_t seed, double sigma,
+                                                                int64_t first_block, float *local,
+                                                                double *toff, double *blocksum) {
+    __shared__ double lds[kWalkThreads];
+    constexpr int kSub = 1 << kWalkSubShift;
+    constexpr int kSubsPerThread = kWalkPerThread / kSub;
     const int t = threadIdx.x;
-    lds[t] = v;
+    const int64_t b = first_block + blockIdx.x;
+    const int64_t k0 = (b << kWalkBlockShift) + int64_t(t) * kWalkPerThread;
+    float *out = local ? local + (int64_t(blockIdx.x) << kWalkBlockShift) + int64_t(t) * kWalkPerThread
+                       : nullptr;
+    double s = 0.0, sub0 = 0.0;
+    double subs[kSubsPerThread];
+#pragma unroll
+    for (int m = 0; m < kSubsPerThread; ++m) {
+        subs[m] = s;  // thread-local prefix before sub-block m
+        sub0 = s;
+        for (int q = m * kSub; q < (m + 1) * kSub; q += 8) {
+            uint64_t w[4] = {static_cast<uint64_t>((k0 + q) >> 3), 0, 0, 0};
+            philox4x64(w, seed, kClockTag);
+#pragma unroll
+            for (int p = 0; p < 4; ++p) {
+                float zc, zs;
+                box_muller(w[p], zc, zs);
+                s = s + (0.0 + sigma * static_cast<double>(zc));
+                if (out) out[q + 2 * p] = static_cast<float>(s - sub0);
+                s = s + (0.0 + sigma * static_cast<double>(zs));
+                if (out) out[q + 2 * p + 1] = static_cast<float>(s - sub0);
+            }
+        }
+    }
+    // fixed-tree (Hillis-Steele) inclusive scan of the thread totals
+    lds[t] = s;
     __syncthreads();
     for (int off = 1; off < kWalkThreads; off <<= 1) {
         const double add = t >= off ? lds[t - off] : 0.0;
@@ -389,20 +685,16 @@ __device__ __forceinline__ double block_inclusive_scan(double v, double *lds) {
         if (t >= off) lds[t] = lds[t] + add;
         __syncthreads();
     }
-    return lds[t];
+    if (toff) {
+        const double excl = t ? lds[t - 1] : 0.0;
+        double *dst = toff + (int64_t(blockIdx.x) << (kWalkBlockShift - kWalkSubShift)) + int64_t(t) * kSubsPerThread;
+#pragma unroll
+        for (int m = 0; m < kSubsPerThread; ++m) dst[m] = excl + subs[m];
+    }
+    if (t == kWalkThreads - 1) blocksum[blockIdx.x] = lds[t];
 }
 
-__global__ void __launch_bounds__(kWalkThreads) walk_blocksum_kernel(uint64_t seed, double sigma,
-                                                                     int64_t first_block,
-                                                                     double *blocksum) {
-    __shared__ double lds[kWalkThreads];
-    const int64_t b = first_block + blockIdx.x;
-    const double s = walk_thread_sum(seed, sigma, b * kWalkBlock + int64_t(threadIdx.x) * kWalkPerThread);
-    const double inc = block_inclusive_scan(s, lds);
-    if (threadIdx.x == kWalkThreads - 1) blocksum[blockIdx.x] = inc;
-}
-
-// base[b + 1] = base[b] + blocksum[b], left to right (one lane).
+// base[b + 1] = base[b] + blocksum[b], strictly left to right (one lane).
 __global__ void walk_carry_kernel(double *base, int64_t first_block, int64_t n_blocks,
                                   const double *blocksum) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -410,37 +702,6 @@ __global__ void walk_carry_kernel(double *base, int64_t first_block, int64_t n_b
     for (int64_t q = 0; q < n_blocks; ++q) {
         acc = acc + blocksum[q];
         base[first_block + q + 1] = acc;
-    }
-}
-
-// eps(k) for k in [k_lo, k_hi]: out[k - k_lo] = base[b] + (prefix of thread
-// totals before this thread + running sum of this thread's increments up to k).
-__global__ void __launch_bounds__(kWalkThreads) walk_materialize_kernel(uint64_t seed, double sigma,
-                                                                        int64_t k_lo, int64_t k_hi,
-                                                                        int64_t first_block,
-                                                                        const double *base,
-                                                                        double *out) {
-    __shared__ double lds[kWalkThreads];
-    // Increments staged in LDS, [q][thread] so a wavefront's accesses are
-    // consecutive (64 KiB per workgroup).
-    __shared__ double incr[kWalkPerThread * kWalkThreads];
-    const int t = threadIdx.x;
-    const int64_t b = first_block + blockIdx.x;
-    const int64_t k0 = b * kWalkBlock + int64_t(t) * kWalkPerThread;
-    double s = 0.0;
-    for (int q = 0; q < kWalkPerThread; ++q) {
-        const double v = 0.0 + sigma * clock_normal(seed, k0 + q);
-        incr[q * kWalkThreads + t] = v;
-        s = s + v;
-    }
-    block_inclusive_scan(s, lds);
-    const double before = t ? lds[t - 1] : 0.0;
-    const double bb = base[b];
-    double p = 0.0;
-    for (int q = 0; q < kWalkPerThread; ++q) {
-        p = p + incr[q * kWalkThreads + t];
-        const int64_t k = k0 + q;
-        if (k >= k_lo && k <= k_hi) out[k - k_lo] = bb + (before + p);
     }
 }
 
@@ -465,21 +726,18 @@ hipError_t launch_render(int form, const JobDev *jobs, int n_jobs, const LinkDev
     return hipGetLastError();
 }
 
-hipError_t launch_walk_extend(uint64_t seed, double sigma, int64_t first_block, int64_t n_blocks,
-                              double *base, double *blocksum_scratch, hipStream_t stream) {
+hipError_t launch_walk_gen(uint64_t seed, double sigma, int64_t first_block, int64_t n_blocks, float *local,
+                           double *toff, double *blocksum, hipStream_t stream) {
     if (n_blocks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(walk_blocksum_kernel, dim3(uint32_t(n_blocks)), dim3(kWalkThreads), 0, stream,
-                       seed, sigma, first_block, blocksum_scratch);
-    hipLaunchKernelGGL(walk_carry_kernel, dim3(1), dim3(64), 0, stream, base, first_block, n_blocks,
-                       blocksum_scratch);
+    hipLaunchKernelGGL(walk_gen_kernel, dim3(uint32_t(n_blocks)), dim3(kWalkThreads), 0, stream, seed, sigma,
+                       first_block, local, toff, blocksum);
     return hipGetLastError();
 }
 
-hipError_t launch_walk_materialize(uint64_t seed, double sigma, int64_t k_lo, int64_t k_hi,
-                                   const double *base, double *out, hipStream_t stream) {
-    const int64_t b0 = k_lo / kWalkBlock, b1 = k_hi / kWalkBlock;
-    hipLaunchKernelGGL(walk_materialize_kernel, dim3(uint32_t(b1 - b0 + 1)), dim3(kWalkThreads), 0,
-                       stream, seed, sigma, k_lo, k_hi, b0, base, out);
+hipError_t launch_walk_carry(double *base, int64_t first_block, int64_t n_blocks, const double *blocksum,
+                             hipStream_t stream) {
+    if (n_blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(walk_carry_kernel, dim3(1), dim3(64), 0, stream, base, first_block, n_blocks, blocksum);
     return hipGetLastError();
 }
 

@@ -473,31 +473,81 @@ void orc_philox4x64(const uint64_t ctr_in[4], const uint64_t key_in[2], uint64_t
 #define LPK_CLOCK_TAG 0x636C6F636B77616CULL   /* "clockwal" */
 #define LPK_AWGN_TAG 0x6177676E6E6F6973ULL    /* "awgnnois" */
 
-/* Standard normal for master-grid index k of a receiver's clock walk:
- * Box-Muller in f64 on two 53-bit uniforms of Philox4x64-10(ctr = (k,0,0,0),
- * key = (seed, "clockwal")). */
+/* Box-Muller pair from one 64-bit Philox word: u1 = (m1 + 1/2) 2^-24 from bits
+ * 63..40, angle u2 = m2 2^-24 turns from bits 39..16. The project defines the
+ * pair with correctly rounded f64 operations only (+ - * / sqrt, no FMA, no
+ * libm), so any IEEE-754 machine reproduces its bits; each value is rounded
+ * once to f32. */
+void orc_box_muller(uint64_t w, float *zc, float *zs) {
+    /* ln of u1 = n 2^-25, n = 2 m1 + 1: exact split n = f 2^e with f in
+     * [sqrt(1/2), sqrt(2)], then ln f = 2 atanh((f-1)/(f+1)) by its series. */
+    uint32_t n = ((uint32_t)(w >> 40) << 1) | 1u;
+    int e = 31 - __builtin_clz(n);
+    double f = ldexp((double)n, -e);
+    if (f > 1.4142135623730951) {
+        f = f * 0.5;
+        e += 1;
+    }
+    double s = (f - 1.0) / (f + 1.0), z = s * s;
+    double p = 1.0 / 13.0;
+    p = p * z + 1.0 / 11.0;
+    p = p * z + 1.0 / 9.0;
+    p = p * z + 1.0 / 7.0;
+    p = p * z + 1.0 / 5.0;
+    p = p * z + 1.0 / 3.0;
+    p = p * z + 1.0;
+    double ln_u1 = (double)(e - 25) * 0.6931471805599453 + (2.0 * s) * p;
+    double r = sqrt(-2.0 * ln_u1);
+    /* angle 2 pi m2 2^-24: quadrant and reflection about pi/4 exactly in
+     * integers, then the Taylor series of sin and cos on [0, pi/4]. */
+    uint32_t m2 = (uint32_t)(w >> 16) & 0xFFFFFFu;
+    uint32_t quad = m2 >> 22, v = m2 & 0x3FFFFFu;
+    int flip = v > 0x200000u;
+    double th = (double)(flip ? 0x400000u - v : v) * (1.5707963267948966 * 0x1.0p-22);
+    double t2 = th * th;
+    double ps = 1.0 / 6227020800.0;
+    ps = ps * t2 - 1.0 / 39916800.0;
+    ps = ps * t2 + 1.0 / 362880.0;
+    ps = ps * t2 - 1.0 / 5040.0;
+    ps = ps * t2 + 1.0 / 120.0;
+    ps = ps * t2 - 1.0 / 6.0;
+    double pc = -1.0 / 87178291200.0;
+    pc = pc * t2 + 1.0 / 479001600.0;
+    pc = pc * t2 - 1.0 / 3628800.0;
+    pc = pc * t2 + 1.0 / 40320.0;
+    pc = pc * t2 - 1.0 / 720.0;
+    pc = pc * t2 + 1.0 / 24.0;
+    pc = pc * t2 - 0.5;
+    double sn = th + (th * t2) * ps, cs = 1.0 + t2 * pc;
+    double a = flip ? cs : sn, b = flip ? sn : cs, sv, cv;
+    switch (quad) {
+    case 0: sv = a; cv = b; break;
+    case 1: sv = b; cv = -a; break;
+    case 2: sv = -a; cv = -b; break;
+    default: sv = -b; cv = a; break;
+    }
+    *zc = (float)(r * cv);
+    *zs = (float)(r * sv);
+}
+
+/* Standard normal for master-grid index k of a receiver's clock walk: block
+ * k >> 3 of Philox4x64-10 (ctr = (k >> 3, 0, 0, 0), key = (seed, "clockwal")),
+ * word (k >> 1) & 3, the cos branch for even k and the sin branch for odd k. */
 double orc_clock_normal(uint64_t clock_seed, uint64_t k) {
-    uint64_t ctr[4] = {k, 0, 0, 0}, key[2] = {clock_seed, LPK_CLOCK_TAG}, w[4];
+    uint64_t ctr[4] = {k >> 3, 0, 0, 0}, key[2] = {clock_seed, LPK_CLOCK_TAG}, w[4];
     orc_philox4x64(ctr, key, w);
-    double u1 = ((double)(w[0] >> 11) + 0.5) * 0x1.0p-53;
-    double u2 = (double)(w[1] >> 11) * 0x1.0p-53;
-    return sqrt(-2.0 * log(u1)) * cos(TAU * u2);
+    float zc, zs;
+    orc_box_muller(w[(k >> 1) & 3], &zc, &zs);
+    return (double)((k & 1) ? zs : zc);
 }
 
 /* Two standard normals (re, im) for global sample i of a band's noise stream:
- * Box-Muller in f32 on two 24-bit uniforms of Philox4x64-10(ctr = (i,0,0,0),
- * key = (seed, "awgnnois")). */
+ * block i >> 2 of Philox4x64-10 (ctr = (i >> 2, 0, 0, 0), key = (seed,
+ * "awgnnois")), word i & 3, both branches of its Box-Muller pair. */
 void orc_awgn_normals(uint64_t noise_seed, uint64_t i, float *z_re, float *z_im) {
-    uint64_t ctr[4] = {i, 0, 0, 0}, key[2] = {noise_seed, LPK_AWGN_TAG}, w[4];
+    uint64_t ctr[4] = {i >> 2, 0, 0, 0}, key[2] = {noise_seed, LPK_AWGN_TAG}, w[4];
     orc_philox4x64(ctr, key, w);
-    float u1 = ((float)(w[0] >> 40) + 0.5f) * 0x1.0p-24f;
-    float u2 = (float)(w[1] >> 40) * 0x1.0p-24f;
-    float r = sqrtf(-2.0f * logf(u1));
-    float th = 6.28318530717958647692f * u2;
-    float s_th, c_th;
-    sincosf(th, &s_th, &c_th);
-    *z_re = r * c_th;
-    *z_im = r * s_th;
+    orc_box_muller(w[i & 3], z_re, z_im);
 }
 
 /* ---------------------------------------------------------------- the clock */

@@ -133,6 +133,7 @@ int orc_payload_qam16(const uint8_t *sync, uint64_t n_sync_bits, uint64_t n_payl
 
 /* ---- project RNG: Philox4x64-10 and the normals built on it ---- */
 void orc_philox4x64(const uint64_t ctr[4], const uint64_t key[2], uint64_t out[4]);
+void orc_box_muller(uint64_t w, float *zc, float *zs);
 double orc_clock_normal(uint64_t clock_seed, uint64_t k);
 void orc_awgn_normals(uint64_t noise_seed, uint64_t i, float *z_re, float *z_im);
 

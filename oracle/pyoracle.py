@@ -94,6 +94,7 @@ def load() -> C.CDLL:
         "orc_payload_ook": (C.c_int, [P(C.c_uint8), _u64, _u64, _u64, P(C.c_int8)]),
         "orc_payload_qam16": (C.c_int, [P(C.c_uint8), _u64, _u64, _u64, P(C.c_float)]),
         "orc_philox4x64": (None, [P(_u64), P(_u64), P(_u64)]),
+        "orc_box_muller": (None, [_u64, P(C.c_float), P(C.c_float)]),
         "orc_clock_normal": (_d, [_u64, _u64]),
         "orc_awgn_normals": (None, [_u64, _u64, P(C.c_float), P(C.c_float)]),
         "orc_thermal_noise_power_w": (_d, [_d, _d]),

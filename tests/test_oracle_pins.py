@@ -418,6 +418,40 @@ def test_philox_matches_numpy(orc):
         assert list(out) == [int(x) for x in ref]
 
 
+def test_box_muller_against_libm(orc):
+    """The project's libm-free Box-Muller pair against numpy's f64 log, sqrt,
+    cos and sin of the same uniforms, rounded to f32: within one f32 ulp
+    everywhere, at every quadrant and reflection edge of the angle and both
+    ends of u1; bits 15..0 of the word are ignored."""
+    rng = np.random.default_rng(3)
+    m1s = [0, 1, 2, (1 << 23) - 1, 1 << 23, (1 << 24) - 2, (1 << 24) - 1] + list(rng.integers(0, 1 << 24, 400))
+    edges = [0, 1, (1 << 21) - 1, 1 << 21, (1 << 21) + 1, (1 << 22) - 1, 1 << 22, (1 << 22) + 1,
+             3 << 21, (1 << 23) - 1, 1 << 23, 5 << 21, 3 << 22, 7 << 21, (1 << 24) - 1]
+    m2s = edges + list(rng.integers(0, 1 << 24, 400))
+    zc, zs = C.c_float(), C.c_float()
+    got, want = [], []
+    for a in m1s:
+        for b in m2s[:: 1 if a < 8 else 37]:
+            w = (int(a) << 40) | (int(b) << 16) | int(rng.integers(0, 1 << 16))
+            orc.orc_box_muller(w, C.byref(zc), C.byref(zs))
+            u1 = (int(a) + 0.5) * 2.0 ** -24
+            ang = 2.0 * math.pi * (int(b) * 2.0 ** -24)
+            r = math.sqrt(-2.0 * math.log(u1))
+            got.append((zc.value, zs.value))
+            want.append((r * math.cos(ang), r * math.sin(ang)))
+    got = np.array(got, dtype=np.float32)
+    want = np.array(want)
+    ulp = np.spacing(np.abs(want).astype(np.float32)).astype(np.float64)
+    err = np.abs(got.astype(np.float64) - want)
+    assert np.all(err <= ulp + 1e-12), np.max(err / ulp)
+    assert np.mean(got == want.astype(np.float32)) > 0.99
+    # exact values at the quadrant points: angle 0, pi/2, pi, 3 pi/2 with u1 = 2^-25 * (2 m1 + 1)
+    for b, (c, s) in {0: (1, 0), 1 << 22: (0, 1), 1 << 23: (-1, 0), 3 << 22: (0, -1)}.items():
+        orc.orc_box_muller((5 << 40) | (b << 16), C.byref(zc), C.byref(zs))
+        r = np.float32(math.sqrt(-2.0 * math.log(5.5 * 2.0 ** -24)))
+        assert (zc.value, zs.value) == (float(r) * c, float(r) * s)
+
+
 def test_normals_are_standard_normal(orc):
     from scipy import stats
     z = np.array([orc.orc_clock_normal(77, k) for k in range(40_000)])
