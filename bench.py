@@ -90,7 +90,7 @@ def gpu_leg(args, scene_n, forms, steps, warmup):
         ms_per_step = (t1 - t0) * 1e3 / steps
         kernel_ms = kern_ms.value / max(n_launch.value, 1)
         log(f"[bench] config {scene_n} form {name}: {steps} timed steps after {warmup} warmup: {ms_per_step:.3f} ms/step; "
-            f"render kernel {kernel_ms:.3f} ms/launch over {n_launch.value} launches")
+            f"render kernels {kernel_ms:.3f} ms per render call over {n_launch.value} calls")
         results[name] = (ms_per_step, kernel_ms)
     for d in bufs:
         be.check(lib.lpk_device_free(be.h, d))

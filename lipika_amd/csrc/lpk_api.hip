@@ -404,7 +404,7 @@ void render(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form,
         J.tile0 = uint32_t(tiles);
         J.n_tiles = B.clen ? uint32_t(((B.i0 + B.clen - 1) >> kTileShift) - (B.i0 >> kTileShift) + 1) : 0u;
         tiles += J.n_tiles;
-        if (tiles > 0x7FFFFFFFull) throw Fail{LPK_E_INVALID, "render call too large for one launch"};
+        if (tiles > 0xFFFFFFFFull) throw Fail{LPK_E_INVALID, "render call holds more than 2^32 tiles"};
         J.link0 = uint32_t(link_off);
         J.n_active = B.n_active;
         for (uint32_t s = 0; s < B.n_active; ++s) {

@@ -38,6 +38,10 @@ static_assert(kWalkThreads * kWalkPerThread == (1 << kWalkBlockShift), "walk blo
 // sit at absolute multiples of kTile in a band's sample index.
 constexpr int kTile = 256;
 constexpr int kTileShift = 8;
+// A dispatch packet counts its grid in work-items with 32 bits, so one
+// launch covers at most 2^32 / kTile - 1 tiles; a render call is sliced into
+// launches of kMaxLaunchTiles.
+constexpr uint32_t kMaxLaunchTiles = 1u << 23;
 
 struct TxDev {
     int32_t traj_kind;       // 0 stationary, 1 grid

@@ -406,12 +406,12 @@ template <int FORM>
 __global__ void __launch_bounds__(kTile) render_kernel(const JobDev *__restrict__ jobs, int n_jobs,
                                                        const LinkDev *__restrict__ links,
                                                        const TxDev *__restrict__ txs,
-                                                       uint32_t n_tiles, TapOut tap) {
+                                                       uint32_t tile_base, uint32_t n_tiles, TapOut tap) {
     __shared__ uint64_t s_noise[kTile];              // one Philox word per sample
     __shared__ double s_anc_t[3][3];                  // tu, t_local, delta_t at the anchors
     __shared__ Anchor s_anc[FORM ? kAnchorGroup : 1];
 
-    const uint32_t tile = xcd_tile(blockIdx.x, n_tiles);
+    const uint32_t tile = tile_base + xcd_tile(blockIdx.x, n_tiles);
     int lo = 0, hi = n_jobs - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -717,13 +717,18 @@ hipError_t launch_render(int form, const JobDev *jobs, int n_jobs, const LinkDev
                          double *tap_phase, double *tap_delta_t, uint32_t tap_slot,
                          hipStream_t stream) {
     TapOut tap{tap_t_emit, tap_range, tap_phase, tap_delta_t, tap_slot, tap_t_emit ? 1u : 0u};
-    if (form == 0)
-        hipLaunchKernelGGL(render_kernel<0>, dim3(n_tiles), dim3(kTile), 0, stream, jobs, n_jobs, links,
-                           txs, n_tiles, tap);
-    else
-        hipLaunchKernelGGL(render_kernel<1>, dim3(n_tiles), dim3(kTile), 0, stream, jobs, n_jobs, links,
-                           txs, n_tiles, tap);
-    return hipGetLastError();
+    for (uint32_t base = 0; base < n_tiles; base += kMaxLaunchTiles) {
+        const uint32_t n = n_tiles - base < kMaxLaunchTiles ? n_tiles - base : kMaxLaunchTiles;
+        if (form == 0)
+            hipLaunchKernelGGL(render_kernel<0>, dim3(n), dim3(kTile), 0, stream, jobs, n_jobs, links, txs,
+                               base, n, tap);
+        else
+            hipLaunchKernelGGL(render_kernel<1>, dim3(n), dim3(kTile), 0, stream, jobs, n_jobs, links, txs,
+                               base, n, tap);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_walk_gen(uint64_t seed, double sigma, int64_t first_block, int64_t n_blocks, float *local,

@@ -336,6 +336,50 @@ def test_full_config2_scale_properties(backend):
     assert hashes[0] == hashes[1] == hashes[2]
 
 
+@pytest.mark.parametrize("form", [_lib.FORM_PLAIN, _lib.FORM_PRECISION])
+def test_render_call_larger_than_one_launch(backend, form):
+    """One render call of 2.3e9 samples (8.98 M tiles, more than one dispatch
+    can hold): chunks at the start, across the first slice boundary and at the
+    very end equal the same chunks rendered on their own."""
+    import ctypes as C
+    from lipika_amd.config import ReceiverBandConfig, SceneConfig, SceneMeta, StationaryEcef, TransmitterConfig, Cw
+    seed = 4
+    tx = TransmitterConfig("cw", 1.0e9, Cw(1.25e6), StationaryEcef(6378137.0, 0.0, 300000.0), 30.0)
+    band = ReceiverBandConfig("wide", 1.0e9, 10.0e6, noise_power_w=0.0)
+    sc = prepare(SceneConfig(SceneMeta(util.EPOCH, 230.0), [tx], [util.lab_receiver([band])]))
+    n = sc.receivers[0].bands[0].n
+    assert n // 256 > 1 << 23
+    backend.upload(sc, seed)
+    lib = _lib.load()
+    chunk = DEFAULT_CHUNK_SAMPLES
+    dev = C.c_void_p()
+    backend.check(lib.lpk_device_alloc(backend.h, n * 8, C.byref(dev)))
+    try:
+        jobs, keep = [], []
+        for i0 in range(0, n, chunk):
+            j, k = backend.job(0, 0, 0, seed, i0, min(chunk, n - i0), out_dev=dev.value + i0 * 8)
+            jobs.append(j)
+            keep.append(k)
+        _say(f"one call: {len(jobs)} jobs, {n} samples, {(n + 255) // 256} tiles")
+        backend.render(jobs, form)
+        backend.sync()
+        boundary = ((1 << 23) * 256) // chunk
+        for q in (0, boundary - 1, boundary, boundary + 1, len(jobs) - 1):
+            i0 = q * chunk
+            clen = min(chunk, n - i0)
+            got = np.empty(clen, dtype=np.complex64)
+            backend.check(lib.lpk_copy_to_host(backend.h, got.ctypes.data, dev.value + i0 * 8, clen * 8))
+            want = np.empty(clen, dtype=np.complex64)
+            j, k = backend.job(0, 0, 0, seed, i0, clen, out_host=want)
+            backend.render([j], form)
+            backend.sync()
+            _say(f"chunk {q}: max|want| = {np.max(np.abs(want)):.3e}")
+            assert np.max(np.abs(want)) > 0.0
+            assert got.tobytes() == want.tobytes(), q
+    finally:
+        backend.check(lib.lpk_device_free(backend.h, dev))
+
+
 # ---------------------------------------------------------------- error paths (forced from the host)
 
 def test_errors_are_codes_not_crashes(backend):
