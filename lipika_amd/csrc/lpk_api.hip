@@ -18,10 +18,8 @@ hipError_t launch_render(int form, const JobDev *jobs, int n_jobs, const LinkDev
                          const TxDev *txs, uint32_t n_tiles, double *tap_t_emit, double *tap_range,
                          double *tap_phase, double *tap_delta_t, uint32_t tap_slot,
                          hipStream_t stream);
-hipError_t launch_walk_gen(uint64_t seed, double sigma, int64_t first_block, int64_t n_blocks, float *local,
-                           double *toff, double *blocksum, hipStream_t stream);
-hipError_t launch_walk_carry(double *base, int64_t first_block, int64_t n_blocks, const double *blocksum,
-                             hipStream_t stream);
+hipError_t launch_walk_gen(const WalkSegDev *segs, int n_segs, uint32_t n_blocks, hipStream_t stream);
+hipError_t launch_walk_carry(const CarrySegDev *segs, int n_segs, hipStream_t stream);
 }  // namespace lpk
 
 using namespace lpk;
@@ -99,7 +97,7 @@ struct lpk_ctx {
     bool have_scene = false;
     bool timed = false;
     // per-call buffers
-    DevBuf jobs, links, dt_stage, out_stage, codes_stage, blocksum;
+    DevBuf jobs, links, dt_stage, out_stage, codes_stage, blocksum, walk_tables;
     PinSlot pin[4];
     int pin_next = 0;
 };
@@ -262,53 +260,88 @@ void pinned_mark(lpk_ctx *ctx, int slot_used) {
     hip_check(hipEventRecord(ctx->pin[slot_used].done, ctx->stream), "event record");
 }
 
-// Bring the receiver's walk base table up to the block of k_hi and build the
-// walk window covering [k_lo, k_hi] (whole blocks).
-void prepare_walk(lpk_ctx *ctx, RxState &R, int64_t k_lo, int64_t k_hi) {
-    const int64_t b_hi = k_hi >> kWalkBlockShift;
-    if (b_hi + 1 > R.base_cap) {
-        int64_t cap = std::max<int64_t>(R.base_cap ? R.base_cap : 4096, 4096);
-        while (cap < b_hi + 2) cap *= 2;
-        double *nb = nullptr;
-        if (hipMalloc(&nb, size_t(cap) * sizeof(double)) != hipSuccess)
-            throw Fail{LPK_E_NOMEM, "clock walk table allocation failed"};
-        if (R.base) {
-            hip_check(hipMemcpyAsync(nb, R.base, size_t(R.nb_known + 1) * sizeof(double),
-                                     hipMemcpyDeviceToDevice, ctx->stream), "walk table copy");
-            hip_check(hipStreamSynchronize(ctx->stream), "walk table copy");
-            (void)hipFree(R.base);
-        } else {
-            const double zero = 0.0;
-            hip_check(hipMemcpy(nb, &zero, sizeof(double), hipMemcpyHostToDevice), "walk table init");
-            R.nb_known = 0;
+// Grow a receiver's walk base table to hold base[0 .. b_hi + 1].
+void ensure_base(lpk_ctx *ctx, RxState &R, int64_t b_hi) {
+    if (b_hi + 2 <= R.base_cap) return;
+    int64_t cap = std::max<int64_t>(R.base_cap ? R.base_cap : 4096, 4096);
+    while (cap < b_hi + 2) cap *= 2;
+    double *nb = nullptr;
+    if (hipMalloc(&nb, size_t(cap) * sizeof(double)) != hipSuccess)
+        throw Fail{LPK_E_NOMEM, "clock walk table allocation failed"};
+    if (R.base) {
+        hip_check(hipMemcpyAsync(nb, R.base, size_t(R.nb_known + 1) * sizeof(double), hipMemcpyDeviceToDevice,
+                                 ctx->stream),
+                  "walk table copy");
+        hip_check(hipStreamSynchronize(ctx->stream), "walk table copy");
+        (void)hipFree(R.base);
+    } else {
+        const double zero = 0.0;
+        hip_check(hipMemcpy(nb, &zero, sizeof(double), hipMemcpyHostToDevice), "walk table init");
+        R.nb_known = 0;
+    }
+    R.base = nb;
+    R.base_cap = cap;
+}
+
+// Build, for every receiver with a window [wlo[r], whi[r]] of master indices,
+// the walk blocks the render call needs, in two launches for all receivers:
+// the block generator (block sums for every block from the known frontier or
+// the window start, whichever is first, and the window's stored values) and
+// the carry that extends each base table from its frontier.
+void prepare_walks(lpk_ctx *ctx, const std::vector<int64_t> &wlo, const std::vector<int64_t> &whi) {
+    std::vector<WalkSegDev> segs;
+    std::vector<CarrySegDev> carries;
+    std::vector<int64_t> bs_off, car_off;  // block-sum offsets of each segment and carry
+    uint64_t blocks = 0;
+    for (size_t r = 0; r < ctx->rx.size(); ++r) {
+        if (whi[r] < 0) continue;
+        RxState &R = ctx->rx[r];
+        const int64_t b_lo = wlo[r] >> kWalkBlockShift, b_hi = whi[r] >> kWalkBlockShift;
+        ensure_base(ctx, R, b_hi);
+        const int64_t first = std::min(R.nb_known, b_lo);
+        const int64_t n = b_hi + 1 - first, n_win = b_hi + 1 - b_lo;
+        WalkSegDev S{};
+        S.seed = R.d.clock_seed;
+        S.sigma = R.sigma_step;
+        S.first_block = first;
+        S.win_block = b_lo;
+        S.block0 = uint32_t(blocks);
+        S.n_blocks = uint32_t(n);
+        S.local = static_cast<float *>(R.local.ensure(size_t(n_win) * size_t(kWalkBlock) * sizeof(float), ctx->stream));
+        S.toff = static_cast<double *>(
+            R.toff.ensure(size_t(n_win) * size_t(kWalkBlock >> kWalkSubShift) * sizeof(double), ctx->stream));
+        bs_off.push_back(int64_t(blocks));
+        if (R.nb_known <= b_hi) {
+            CarrySegDev Cs{};
+            Cs.base = R.base;
+            Cs.first = R.nb_known;
+            Cs.n = b_hi + 1 - R.nb_known;
+            carries.push_back(Cs);
+            car_off.push_back(int64_t(blocks) + (R.nb_known - first));
+            R.nb_known = b_hi + 1;
         }
-        R.base = nb;
-        R.base_cap = cap;
+        blocks += uint64_t(n);
+        if (blocks > 0xFFFFFFull) throw Fail{LPK_E_INVALID, "clock walk windows too large for one launch"};
+        segs.push_back(S);
     }
-    const int64_t b_lo = k_lo >> kWalkBlockShift;
-    const int64_t n_win = b_hi - b_lo + 1;
-    double *bs = static_cast<double *>(
-        ctx->blocksum.ensure(size_t(std::max<int64_t>(n_win, b_lo - R.nb_known)) * sizeof(double), ctx->stream));
-    // blocks between the frontier and the window: block sums only
-    if (R.nb_known < b_lo) {
-        const int64_t gap = b_lo - R.nb_known;
-        hip_check(launch_walk_gen(R.d.clock_seed, R.sigma_step, R.nb_known, gap, nullptr, nullptr, bs, ctx->stream),
-                  "clock walk block sums");
-        hip_check(launch_walk_carry(R.base, R.nb_known, gap, bs, ctx->stream), "clock walk carry");
-        R.nb_known = b_lo;
-    }
-    // the window: thread-local prefixes, thread offsets and block sums
-    float *loc = static_cast<float *>(R.local.ensure(size_t(n_win) * size_t(kWalkBlock) * sizeof(float), ctx->stream));
-    double *toff = static_cast<double *>(
-        R.toff.ensure(size_t(n_win) * size_t(kWalkBlock >> kWalkSubShift) * sizeof(double), ctx->stream));
-    hip_check(launch_walk_gen(R.d.clock_seed, R.sigma_step, b_lo, n_win, loc, toff, bs, ctx->stream),
-              "clock walk window");
-    if (R.nb_known <= b_hi) {
-        const int64_t first = R.nb_known;
-        hip_check(launch_walk_carry(R.base, first, b_hi + 1 - first, bs + (first - b_lo), ctx->stream),
-                  "clock walk carry");
-        R.nb_known = b_hi + 1;
-    }
+    if (segs.empty()) return;
+    double *bs = static_cast<double *>(ctx->blocksum.ensure(size_t(blocks) * sizeof(double), ctx->stream));
+    for (size_t q = 0; q < segs.size(); ++q) segs[q].blocksum = bs + bs_off[q];
+    for (size_t q = 0; q < carries.size(); ++q) carries[q].blocksum = bs + car_off[q];
+    const size_t seg_bytes = segs.size() * sizeof(WalkSegDev), car_bytes = carries.size() * sizeof(CarrySegDev);
+    const int slot = ctx->pin_next;
+    char *pin = static_cast<char *>(pinned_slot(ctx, seg_bytes + car_bytes));
+    std::memcpy(pin, segs.data(), seg_bytes);
+    if (car_bytes) std::memcpy(pin + seg_bytes, carries.data(), car_bytes);
+    char *dev = static_cast<char *>(ctx->walk_tables.ensure(seg_bytes + car_bytes, ctx->stream));
+    hip_check(hipMemcpyAsync(dev, pin, seg_bytes + car_bytes, hipMemcpyHostToDevice, ctx->stream), "walk table upload");
+    pinned_mark(ctx, slot);
+    hip_check(launch_walk_gen(reinterpret_cast<const WalkSegDev *>(dev), int(segs.size()), uint32_t(blocks),
+                              ctx->stream),
+              "clock walk blocks");
+    hip_check(launch_walk_carry(reinterpret_cast<const CarrySegDev *>(dev + seg_bytes), int(carries.size()),
+                                ctx->stream),
+              "clock walk carry");
 }
 
 struct TapReq {
@@ -375,8 +408,7 @@ void render(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form,
         wlo[B.rx] = std::min(wlo[B.rx], klo);
         whi[B.rx] = std::max(whi[B.rx], khi);
     }
-    for (size_t r = 0; r < ctx->rx.size(); ++r)
-        if (whi[r] >= 0) prepare_walk(ctx, ctx->rx[r], wlo[r], whi[r]);
+    prepare_walks(ctx, wlo, whi);
 
     // ---- staging buffers
     double *dt_dev = dt_total ? static_cast<double *>(ctx->dt_stage.ensure(dt_total * 8, ctx->stream)) : nullptr;

@@ -43,6 +43,27 @@ constexpr int kTileShift = 8;
 // launches of kMaxLaunchTiles.
 constexpr uint32_t kMaxLaunchTiles = 1u << 23;
 
+// One receiver's share of a batched clock-walk launch: walk blocks
+// [first_block, first_block + n_blocks); blocks from win_block on also store
+// the window the render reads (local per index, toff per 32 indices, both
+// indexed from win_block * kWalkBlock). blocksum is indexed from first_block.
+struct WalkSegDev {
+    uint64_t seed;
+    double sigma;
+    int64_t first_block, win_block;
+    uint32_t block0, n_blocks;  // the segment's first workgroup and its count
+    float *local;
+    double *toff;
+    double *blocksum;
+};
+
+// base[first + q + 1] = base[first + q] + blocksum[q] for q < n, in order.
+struct CarrySegDev {
+    double *base;
+    const double *blocksum;
+    int64_t first, n;
+};
+
 struct TxDev {
     int32_t traj_kind;       // 0 stationary, 1 grid
     int32_t signal_kind;     // lpk_signal_kind

@@ -19,7 +19,8 @@
 // order, through the range rates); a tile whose query times cross a
 // trajectory grid node takes the exact per-sample solve instead. The phase
 // is formed in f64 in the reference's operation order and reduced exactly;
-// sincos, mix and a compensated sum run in f32.
+// sine and cosine (hardware, in turns) and the mix run in f32, and the f32
+// contributions are summed exactly in f64 (the compensated sum).
 //
 // walk_gen_kernel: the receiver clock's white-FM walk (src/clock/mod.rs:
 // 299-307) on the master grid, as fixed 65 536-index blocks; values depend
@@ -197,6 +198,14 @@ __device__ __forceinline__ void solve_emission(const TxDev &T, const double *obs
 
 // ---------------------------------------------------------------- signals
 
+// Rust's saturating `floor(x) as i64` for an already floored x, with a
+// one-instruction path for every index a real scene produces (NaN fails the
+// range test and takes sat_i64, which maps it to 0).
+__device__ __forceinline__ int64_t floor_to_i64(double fl) {
+    if (fabs(fl) < 2147483648.0) return static_cast<int64_t>(static_cast<int32_t>(fl));
+    return sat_i64(fl);
+}
+
 __device__ __forceinline__ int64_t frame_index(int64_t raw, int64_t n, int repeating, int64_t k) {
     // src/signal/mod.rs:108-115; -1 means silent
     if (!repeating) return (raw >= 0 && raw < n) ? raw : -1;
@@ -231,46 +240,61 @@ __device__ __forceinline__ bool gate_on(const TxDev &T, double t) {
     return on;
 }
 
+// Value of symbol `u` (floor((t - epoch_rel) * rate), src/signal/mod.rs:108-115)
+// of a framed signal: BPSK (src/signal/bpsk.rs:58-71), OOK
+// (src/signal/ook.rs:60-74), QAM16 (src/signal/qam16.rs:96-109).
+__device__ __forceinline__ void framed_value(const TxDev &T, double u, float &re, float &im) {
+    re = 0.0f;
+    im = 0.0f;
+    const int64_t k = frame_index(floor_to_i64(u), T.n_symbols, T.repeating, T.repeat_from);
+    if (k < 0) return;
+    if (T.signal_kind == 3) {
+        const float2 s = reinterpret_cast<const float2 *>(T.symbols)[k];
+        re = s.x;
+        im = s.y;
+    } else {
+        re = static_cast<float>(T.bits[k]);
+    }
+}
+
+// NAV sign for bit index `b` (floor(dt * 50)), src/signal/ca_code.rs:94-108, :180-190.
+__device__ __forceinline__ int nav_pm(const TxDev &T, double b) {
+    if (T.nav_mode == 1) return nav_dummy_pm(T.nav_seed, floor_to_i64(b));
+    if (T.nav_mode == 2 && T.n_nav_bits > 0) {
+        const int64_t bi = floor_to_i64(b);
+        return T.nav_bits[bi < 0 ? 0 : (bi < T.n_nav_bits ? bi : T.n_nav_bits - 1)];
+    }
+    return 1;
+}
+
+// C/A chip, src/signal/ca_code.rs:167-178.
+__device__ __forceinline__ int ca_chip(const TxDev &T, double dt) {
+    return T.ca_code[rem_euclid(floor_to_i64(floor(dt * kCaChipRate)), kCaCodeLength)];
+}
+
+// CW phase as the reference forms it in f32, src/signal/cw.rs:41-52.
+__device__ __forceinline__ float cw_phase(const TxDev &T, double t) {
+    return T.cw_anchored ? static_cast<float>(rem_euclid_f64(T.omega * t + T.anchor_phase, kTau))
+                         : static_cast<float>(T.omega * t);
+}
+
 // SignalModel::evaluate_baseband_batch, one sample.
 __device__ __forceinline__ void baseband(const TxDev &T, double t, float &re, float &im) {
     re = 0.0f;
     im = 0.0f;
     if (T.gated && !gate_on(T, t)) return;
     switch (T.signal_kind) {
-    case 0: {  // CW, src/signal/cw.rs:41-52 (cos, sin of one f32 phase: one sincosf)
-        const float ph = T.cw_anchored ? static_cast<float>(rem_euclid_f64(T.omega * t + T.anchor_phase, kTau))
-                                       : static_cast<float>(T.omega * t);
-        sincosf(ph, &im, &re);
+    case 0:  // CW: cos, sin of one f32 phase (one sincosf)
+        sincosf(cw_phase(T, t), &im, &re);
         return;
-    }
-    case 1:    // BPSK, src/signal/bpsk.rs:58-71
-    case 2:    // OOK, src/signal/ook.rs:60-74
-    case 3: {  // QAM16, src/signal/qam16.rs:96-109
-        const int64_t raw = sat_i64(floor((t - T.epoch_rel) * T.rate));
-        const int64_t k = frame_index(raw, T.n_symbols, T.repeating, T.repeat_from);
-        if (k < 0) return;
-        if (T.signal_kind == 3) {
-            const float2 s = reinterpret_cast<const float2 *>(T.symbols)[k];
-            re = s.x;
-            im = s.y;
-        } else {
-            re = static_cast<float>(T.bits[k]);
-        }
+    case 1:
+    case 2:
+    case 3:
+        framed_value(T, floor((t - T.epoch_rel) * T.rate), re, im);
         return;
-    }
-    case 4: {  // C/A, src/signal/ca_code.rs:167-193
+    case 4: {
         const double dt = t - T.epoch_rel;
-        const int64_t idx = sat_i64(floor(dt * kCaChipRate));
-        int chip = T.ca_code[rem_euclid(idx, kCaCodeLength)];
-        if (T.nav_mode == 1) {
-            chip *= nav_dummy_pm(T.nav_seed, sat_i64(floor(dt * kNavBitRate)));
-        } else if (T.nav_mode == 2) {
-            const int64_t bi = sat_i64(floor(dt * kNavBitRate));
-            int pm = 1;
-            if (T.n_nav_bits > 0) pm = T.nav_bits[bi < 0 ? 0 : (bi < T.n_nav_bits ? bi : T.n_nav_bits - 1)];
-            chip *= pm;
-        }
-        re = static_cast<float>(chip);
+        re = static_cast<float>(ca_chip(T, dt) * nav_pm(T, floor(dt * kNavBitRate)));
         return;
     }
     default:
@@ -313,24 +337,35 @@ struct TapOut {
     uint32_t slot, on;
 };
 
-// Per (tile, transmitter) data of the precision form. The solve is fitted
-// in two parts: the smooth map (trajectory queried at unrounded times), a
-// quadratic in tu through three anchor samples, plus the first-order effect
+// Per (tile, transmitter) record of the precision form, in LDS. The solve is
+// fitted in two parts: the smooth map (trajectory queried at unrounded times),
+// a quadratic in tu through three anchor samples, plus the first-order effect
 // of the reference's rounding of the absolute query time epoch + t
-// (src/channel/mod.rs:79), which the lanes compute exactly per sample.
-struct Anchor {
-    double r0s[3], r1s[3];  // smooth ranges of iterations 1 and 2 at the anchors
-    double q0d1, q0c2;      // r0s(tu) = r0s_m + x (q0d1 + q0c2 y)
-    double q1d1, q1c2;      // r1s(tu) likewise
-    double rd0;             // d r0 / d (query time) at the middle anchor
-    double rd1q;            // d r1 / d (query time), position only
-    double k01;             // (d r1 / d t1, position and Sagnac) * rd0 / c
-    double r1_mid;          // range at the middle sample (model, or exact when not smooth)
-    double r1x_mid;         // exact (reference arithmetic) range at the middle sample
-    double inv_rm;          // 1 / r1_mid
-    int64_t cell[2][2];     // grid nodes of both iterations at the first and last anchors
-    float amp_m;            // (float)(amp_factor / r1_mid)
-    int smooth;             // the fit is valid across the tile
+// (src/channel/mod.rs:79), which the lanes compute exactly per sample. The
+// first part of the record holds everything the sample loop reads on its
+// fast path, so the loop touches no transmitter table in global memory.
+constexpr int32_t kLinkSmooth = 1, kLinkGated = 2, kLinkAnchoredCw = 4;
+struct alignas(16) Anchor {
+    // read by every lane of the sample loop
+    double r0m, q0d1, q0c2, rd0;   // r0(tu) = r0m + x (q0d1 + q0c2 x) + rd0 del0, x = tu - tu_mid
+    double r1m, q1d1, q1c2, rd1q;  // r1(tu) = r1m + x (q1d1 + q1c2 x) + rd1q del1 - k01 del0
+    double k01;                    // (d r1 / d t1, position and Sagnac) * rd0 / c
+    double r1_mid;                 // range at the middle sample (model, or exact when not smooth)
+    double neg_tau_src, tau_if;    // -TAU f_src, TAU (f_src - f_lo)
+    double epoch_rel, rate;        // signal epoch and symbol rate (framed kinds)
+    double hu;                     // symbol (framed) or NAV bit (C/A) index near the tile's first sample
+    double omega;                  // CW baseband angular offset
+    const int8_t *code;            // C/A chips
+    float amp_m;                   // (float)(amp_factor / r1_mid)
+    float inv_rm;                  // 1 / r1_mid
+    float h0re, h0im;              // value of index hu: symbol (framed) or NAV sign (C/A, h0re)
+    float h1re, h1im;              // value of index hu + 1
+    int32_t kind, flags;           // signal kind; kLink* bits
+    uint32_t tx, _pad;             // transmitter (slow paths read its table)
+    // anchor pass only
+    double r0s[3], r1s[3];         // smooth ranges of iterations 1 and 2 at the anchors
+    double r1x_mid;                // exact (reference arithmetic) range at the middle sample
+    int64_t cell[2][2];            // grid nodes of both iterations at the first and last anchors
 };
 constexpr int kAnchorGroup = 64;  // transmitters per anchor pass
 
@@ -403,7 +438,7 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n) {
 }
 
 template <int FORM>
-__global__ void __launch_bounds__(kTile) render_kernel(const JobDev *__restrict__ jobs, int n_jobs,
+__global__ void __launch_bounds__(kTile, FORM ? 5 : 6) render_kernel(const JobDev *__restrict__ jobs, int n_jobs,
                                                        const LinkDev *__restrict__ links,
                                                        const TxDev *__restrict__ txs,
                                                        uint32_t tile_base, uint32_t n_tiles, TapOut tap) {
@@ -443,7 +478,7 @@ __global__ void __launch_bounds__(kTile) render_kernel(const JobDev *__restrict_
     const double tu = t_local - dlt;
 
     float acc_re = 0.0f, acc_im = 0.0f;
-    float cmp_re = 0.0f, cmp_im = 0.0f;  // Neumaier compensation (precision form)
+    double sum_re = 0.0, sum_im = 0.0;  // exact sum of the f32 contributions (precision form)
 
     if (FORM == 0) {
         for (uint32_t s = 0; s < J.n_active; ++s) {
@@ -483,8 +518,7 @@ __global__ void __launch_bounds__(kTile) render_kernel(const JobDev *__restrict_
             }
         }
         __syncthreads();
-        const double tu_a = s_anc_t[0][0], tu_m = s_anc_t[1][0];
-        const double x = tu - tu_m, y = tu - tu_a;
+        const double x = tu - s_anc_t[1][0];
         // rounding of the iteration-1 query time, exact (Sterbenz) per sample
         const double t0q = tu - kEmissionInitialLag;
         const double del0 = ((J.epoch + t0q) - J.epoch) - t0q;
@@ -526,16 +560,20 @@ __global__ void __launch_bounds__(kTile) render_kernel(const JobDev *__restrict_
                 const LinkDev L = links[J.link0 + g0 + s];
                 const TxDev &T = txs[L.tx];
                 const bool same = A.cell[0][0] == A.cell[1][0] && A.cell[0][1] == A.cell[1][1];
-                A.smooth = fit_ok && (T.traj_kind == 0 || same);
+                const bool smooth = fit_ok && (T.traj_kind == 0 || same);
                 double r1m = A.r1x_mid;
-                if (A.smooth) {
+                A.r0m = A.r0s[1];
+                A.r1m = A.r1s[1];
+                if (smooth) {
                     const double xa = s_anc_t[0][0] - s_anc_t[1][0], xb = s_anc_t[2][0] - s_anc_t[1][0];
                     const double a0 = (A.r0s[1] - A.r0s[0]) / (-xa), b0 = (A.r0s[2] - A.r0s[1]) / xb;
                     const double a1 = (A.r1s[1] - A.r1s[0]) / (-xa), b1 = (A.r1s[2] - A.r1s[1]) / xb;
-                    A.q0d1 = a0;
+                    // Newton form r_m + x (d1 + c2 (x - xa)), with the -c2 xa
+                    // term folded into the linear coefficient
                     A.q0c2 = (b0 - a0) / (xb - xa);
-                    A.q1d1 = a1;
+                    A.q0d1 = fma(-A.q0c2, xa, a0);
                     A.q1c2 = (b1 - a1) / (xb - xa);
+                    A.q1d1 = fma(-A.q1c2, xa, a1);
                     // the model at the middle sample (x = 0)
                     const double t0m = s_anc_t[1][0] - kEmissionInitialLag;
                     const double d0m = ((J.epoch + t0m) - J.epoch) - t0m;
@@ -545,47 +583,98 @@ __global__ void __launch_bounds__(kTile) render_kernel(const JobDev *__restrict_
                     r1m = fma(A.rd1q, d1m, fma(-A.k01, d0m, A.r1s[1]));
                 }
                 A.r1_mid = r1m;
-                A.inv_rm = 1.0 / r1m;
+                A.inv_rm = static_cast<float>(1.0 / r1m);
                 A.amp_m = static_cast<float>(T.amp_factor / r1m);
+                A.neg_tau_src = T.neg_tau_src;
+                A.tau_if = L.tau_if;
+                A.epoch_rel = T.epoch_rel;
+                A.rate = T.rate;
+                A.omega = T.omega;
+                A.code = T.ca_code;
+                A.kind = T.signal_kind;
+                A.flags = (smooth ? kLinkSmooth : 0) | (T.gated ? kLinkGated : 0) |
+                          (T.signal_kind == 0 && T.cw_anchored ? kLinkAnchoredCw : 0);
+                A.tx = L.tx;
+                // Values of the symbol (or NAV bit) the tile starts in and of
+                // the next one; lanes take them when their own exact index
+                // matches and evaluate the signal otherwise.
+                const double te0 = s_anc_t[0][0] - A.r1s[0] * kInvC;
+                A.hu = 0.0;
+                A.h0re = A.h0im = A.h1re = A.h1im = 0.0f;
+                if (T.signal_kind >= 1 && T.signal_kind <= 3) {
+                    A.hu = floor((te0 - T.epoch_rel) * T.rate);
+                    framed_value(T, A.hu, A.h0re, A.h0im);
+                    framed_value(T, A.hu + 1.0, A.h1re, A.h1im);
+                } else if (T.signal_kind == 4) {
+                    A.hu = floor((te0 - T.epoch_rel) * kNavBitRate);
+                    A.h0re = static_cast<float>(nav_pm(T, A.hu));
+                    A.h1re = static_cast<float>(nav_pm(T, A.hu + 1.0));
+                }
             }
             __syncthreads();
             for (uint32_t s = 0; s < ng; ++s) {
-                const LinkDev L = links[J.link0 + g0 + s];
-                const TxDev &T = txs[L.tx];
                 const Anchor &A = s_anc[s];
+                const int32_t flags = A.flags, kind = A.kind;
                 double r;
-                if (A.smooth) {
-                    const double r0 = fma(A.rd0, del0, fma(x, fma(A.q0c2, y, A.q0d1), A.r0s[1]));
+                if (flags & kLinkSmooth) {
+                    const double r0 = fma(A.rd0, del0, fma(x, fma(A.q0c2, x, A.q0d1), A.r0m));
                     const double t1 = fma(-r0, kInvC, tu);
                     const double del1 = ((J.epoch + t1) - J.epoch) - t1;
-                    r = fma(A.rd1q, del1, fma(-A.k01, del0, fma(x, fma(A.q1c2, y, A.q1d1), A.r1s[1])));
+                    r = fma(A.rd1q, del1, fma(-A.k01, del0, fma(x, fma(A.q1c2, x, A.q1d1), A.r1m)));
                 } else {
                     double te;
                     int64_t c0, c1;
-                    solve_emission(T, J.obs, tu, J.epoch, te, r, c0, c1);
+                    solve_emission(txs[A.tx], J.obs, tu, J.epoch, te, r, c0, c1);
                 }
-                const double dr = r - A.r1_mid;
+                const float drf = static_cast<float>(r - A.r1_mid);
                 const double t_emit = fma(-r, kInvC, tu);
-                float bre, bim;
-                baseband(T, t_emit, bre, bim);
+                // s(t_emit): the exact index decides; the tile's hint supplies
+                // the value when the index is the one it holds. CW's own
+                // phase joins the carrier phase (one rotation instead of two).
+                float bre = 0.0f, bim = 0.0f;
+                double ph_cw = 0.0;
+                if (!(flags & kLinkGated) || gate_on(txs[A.tx], t_emit)) {
+                    const double dt = t_emit - A.epoch_rel;
+                    if (kind == 0) {
+                        ph_cw = (flags & kLinkAnchoredCw) ? static_cast<double>(cw_phase(txs[A.tx], t_emit))
+                                                          : static_cast<double>(static_cast<float>(A.omega * t_emit));
+                        bre = 1.0f;
+                    } else if (kind == 4) {
+                        const double b = floor(dt * kNavBitRate);
+                        const float pm =
+                            b == A.hu ? A.h0re : (b == A.hu + 1.0 ? A.h1re : float(nav_pm(txs[A.tx], b)));
+                        const int64_t ci = rem_euclid(floor_to_i64(floor(dt * kCaChipRate)), kCaCodeLength);
+                        bre = static_cast<float>(A.code[ci]) * pm;
+                    } else if (kind <= 3) {
+                        const double u = floor(dt * A.rate);
+                        if (u == A.hu) {
+                            bre = A.h0re;
+                            bim = A.h0im;
+                        } else if (u == A.hu + 1.0) {
+                            bre = A.h1re;
+                            bim = A.h1im;
+                        } else {
+                            framed_value(txs[A.tx], u, bre, bim);
+                        }
+                    }
+                }
                 // src/channel/mod.rs:256-258 in the reference's operation order,
                 // so its f64 rounding is followed; then an exact Cody-Waite
                 // reduction (24 + 24 + 53-bit split of 2 pi, exact for |k| < 2^29)
-                const double phase = T.neg_tau_src * (r * kInvC + dlt) + L.tau_if * t_local;
-                const double kk = rint(phase * 0.15915494309189535);
+                // and the hardware sine and cosine of the reduced turn fraction.
+                const double phase = A.neg_tau_src * (r * kInvC + dlt) + A.tau_if * t_local;
+                const double pt = phase + ph_cw;
+                const double kk = rint(pt * 0.15915494309189535);
                 const double rr = fma(-kk, -6.8604979977715316e-15,
-                                      fma(-kk, -1.7484555314695172e-07, fma(-kk, 6.2831854820251465, phase)));
+                                      fma(-kk, -1.7484555314695172e-07, fma(-kk, 6.2831854820251465, pt)));
                 const float fr = static_cast<float>(rr * 0.15915494309189535);
-                float spf, cpf;
-                sincospif(2.0f * fr, &spf, &cpf);
-                const float amplitude = A.amp_m * (1.0f - static_cast<float>(dr * A.inv_rm));
+                const float spf = __builtin_amdgcn_sinf(fr), cpf = __builtin_amdgcn_cosf(fr);
+                const float amplitude = A.amp_m * (1.0f - drf * A.inv_rm);
                 const float cre = amplitude * (bre * cpf - bim * spf);
                 const float cim = amplitude * (bre * spf + bim * cpf);
-                const float tr = acc_re + cre, ti = acc_im + cim;
-                cmp_re += fabsf(acc_re) >= fabsf(cre) ? (acc_re - tr) + cre : (cre - tr) + acc_re;
-                cmp_im += fabsf(acc_im) >= fabsf(cim) ? (acc_im - ti) + cim : (cim - ti) + acc_im;
-                acc_re = tr;
-                acc_im = ti;
+                // the f32 contributions summed exactly in f64
+                sum_re += static_cast<double>(cre);
+                sum_im += static_cast<double>(cim);
                 if (tap.on && g0 + s == tap.slot && valid) {
                     tap.t_emit[j] = t_emit;
                     tap.range[j] = r;
@@ -594,8 +683,8 @@ __global__ void __launch_bounds__(kTile) render_kernel(const JobDev *__restrict_
             }
             __syncthreads();
         }
-        acc_re = acc_re + cmp_re;
-        acc_im = acc_im + cmp_im;
+        acc_re = static_cast<float>(sum_re);
+        acc_im = static_cast<float>(sum_im);
     }
     if (tap.on && valid) tap.delta_t[j] = dlt;
 
@@ -636,44 +725,67 @@ __global__ void __launch_bounds__(kTile) render_kernel(const JobDev *__restrict_
 
 // ---------------------------------------------------------------- clock walk
 
-// One 65 536-index block of the walk per workgroup: each thread owns 256
-// consecutive increments (64 Philox blocks, 8 normals per block: master index
-// k uses block k >> 3, word (k >> 1) & 3, and the cos (even k) or sin (odd k)
-// branch of its Box-Muller pair) and sums them sequentially in f64. With
-// `local` set it stores, per index, the prefix since the start of its
-// 32-index sub-block as f32 (a value a few sigma*sqrt(32) in size), and per
-// sub-block the f64 offset of that start within the walk block (toff); a
-// fixed-tree scan of the 256 thread totals places each thread in the block,
-// and the block total goes to blocksum.
-__global__ void __launch_bounds__(kWalkThreads) walk_gen_kernel(uint64_t seed, double sigma,
-                                                                int64_t first_block, float *local,
-                                                                double *toff, double *blocksum) {
+// One 65 536-index block of the walk per workgroup, over the segments of
+// every receiver of a render call. Each thread owns 256 consecutive
+// increments (64 Philox blocks, 8 normals per block: master index k uses
+// block k >> 3, word (k >> 1) & 3, and the cos (even k) or sin (odd k) branch
+// of its Box-Muller pair) and sums them sequentially in f64. In window blocks
+// it stores, per index, the prefix since the start of its 32-index sub-block
+// as f32 (a value a few sigma*sqrt(32) in size; staged through LDS so each
+// wavefront writes whole 128-byte rows), and per sub-block the f64 offset of
+// that start within the walk block (toff); a fixed-tree scan of the 256
+// thread totals places each thread in the block, and the block total goes to
+// blocksum.
+constexpr int kWalkSub = 1 << kWalkSubShift;
+constexpr int kWalkStageRow = kWalkSub + 1;  // padded: conflict-free rows and columns
+
+__global__ void __launch_bounds__(kWalkThreads) walk_gen_kernel(const WalkSegDev *__restrict__ segs, int n_segs) {
     __shared__ double lds[kWalkThreads];
-    constexpr int kSub = 1 << kWalkSubShift;
-    constexpr int kSubsPerThread = kWalkPerThread / kSub;
+    __shared__ float stage[kWalkThreads * kWalkStageRow];
+    constexpr int kSubsPerThread = kWalkPerThread / kWalkSub;
+    int lo = 0, hi = n_segs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (segs[mid].block0 <= blockIdx.x) lo = mid;
+        else hi = mid - 1;
+    }
+    const WalkSegDev &S = segs[lo];
     const int t = threadIdx.x;
-    const int64_t b = first_block + blockIdx.x;
+    const int64_t rb = int64_t(blockIdx.x - S.block0);
+    const int64_t b = S.first_block + rb;
+    const bool win = b >= S.win_block;  // uniform over the workgroup
     const int64_t k0 = (b << kWalkBlockShift) + int64_t(t) * kWalkPerThread;
-    float *out = local ? local + (int64_t(blockIdx.x) << kWalkBlockShift) + int64_t(t) * kWalkPerThread
-                       : nullptr;
-    double s = 0.0, sub0 = 0.0;
+    float *out = win ? S.local + ((b - S.win_block) << kWalkBlockShift) : nullptr;
+    const uint64_t seed = S.seed;
+    const double sigma = S.sigma;
+    double s = 0.0;
     double subs[kSubsPerThread];
 #pragma unroll
     for (int m = 0; m < kSubsPerThread; ++m) {
         subs[m] = s;  // thread-local prefix before sub-block m
-        sub0 = s;
-        for (int q = m * kSub; q < (m + 1) * kSub; q += 8) {
-            uint64_t w[4] = {static_cast<uint64_t>((k0 + q) >> 3), 0, 0, 0};
+        const double sub0 = s;
+        for (int q = 0; q < kWalkSub; q += 8) {
+            uint64_t w[4] = {static_cast<uint64_t>((k0 + m * kWalkSub + q) >> 3), 0, 0, 0};
             philox4x64(w, seed, kClockTag);
 #pragma unroll
             for (int p = 0; p < 4; ++p) {
                 float zc, zs;
                 box_muller(w[p], zc, zs);
                 s = s + (0.0 + sigma * static_cast<double>(zc));
-                if (out) out[q + 2 * p] = static_cast<float>(s - sub0);
+                if (win) stage[t * kWalkStageRow + q + 2 * p] = static_cast<float>(s - sub0);
                 s = s + (0.0 + sigma * static_cast<double>(zs));
-                if (out) out[q + 2 * p + 1] = static_cast<float>(s - sub0);
+                if (win) stage[t * kWalkStageRow + q + 2 * p + 1] = static_cast<float>(s - sub0);
             }
+        }
+        if (win) {
+            // thread r's 32 values of sub-block m go to out[r * 256 + m * 32 + c]
+            __syncthreads();
+#pragma unroll 4
+            for (int e = t; e < kWalkThreads * kWalkSub; e += kWalkThreads) {
+                const int row = e >> kWalkSubShift, col = e & (kWalkSub - 1);
+                out[row * kWalkPerThread + m * kWalkSub + col] = stage[row * kWalkStageRow + col];
+            }
+            __syncthreads();
         }
     }
     // fixed-tree (Hillis-Steele) inclusive scan of the thread totals
@@ -685,23 +797,35 @@ __global__ void __launch_bounds__(kWalkThreads) walk_gen_kernel(uint64_t seed, d
         if (t >= off) lds[t] = lds[t] + add;
         __syncthreads();
     }
-    if (toff) {
+    if (win) {
         const double excl = t ? lds[t - 1] : 0.0;
-        double *dst = toff + (int64_t(blockIdx.x) << (kWalkBlockShift - kWalkSubShift)) + int64_t(t) * kSubsPerThread;
+        double *dst = S.toff + ((b - S.win_block) << (kWalkBlockShift - kWalkSubShift)) + int64_t(t) * kSubsPerThread;
 #pragma unroll
         for (int m = 0; m < kSubsPerThread; ++m) dst[m] = excl + subs[m];
     }
-    if (t == kWalkThreads - 1) blocksum[blockIdx.x] = lds[t];
+    if (t == kWalkThreads - 1) S.blocksum[rb] = lds[t];
 }
 
-// base[b + 1] = base[b] + blocksum[b], strictly left to right (one lane).
-__global__ void walk_carry_kernel(double *base, int64_t first_block, int64_t n_blocks,
-                                  const double *blocksum) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    double acc = base[first_block];
-    for (int64_t q = 0; q < n_blocks; ++q) {
-        acc = acc + blocksum[q];
-        base[first_block + q + 1] = acc;
+// One workgroup per receiver: base[first + q + 1] = base[first + q] +
+// blocksum[q], strictly left to right, 64 block sums staged at a time.
+__global__ void __launch_bounds__(64) walk_carry_kernel(const CarrySegDev *__restrict__ segs) {
+    __shared__ double buf[64];
+    const CarrySegDev &S = segs[blockIdx.x];
+    const int t = threadIdx.x;
+    double acc = S.base[S.first];
+    for (int64_t q0 = 0; q0 < S.n; q0 += 64) {
+        const int m = S.n - q0 < 64 ? static_cast<int>(S.n - q0) : 64;
+        if (t < m) buf[t] = S.blocksum[q0 + t];
+        __syncthreads();
+        if (t == 0) {
+            for (int i = 0; i < m; ++i) {
+                acc = acc + buf[i];
+                buf[i] = acc;
+            }
+        }
+        __syncthreads();
+        if (t < m) S.base[S.first + q0 + t + 1] = buf[t];
+        __syncthreads();
     }
 }
 
@@ -731,18 +855,15 @@ hipError_t launch_render(int form, const JobDev *jobs, int n_jobs, const LinkDev
     return hipSuccess;
 }
 
-hipError_t launch_walk_gen(uint64_t seed, double sigma, int64_t first_block, int64_t n_blocks, float *local,
-                           double *toff, double *blocksum, hipStream_t stream) {
-    if (n_blocks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(walk_gen_kernel, dim3(uint32_t(n_blocks)), dim3(kWalkThreads), 0, stream, seed, sigma,
-                       first_block, local, toff, blocksum);
+hipError_t launch_walk_gen(const WalkSegDev *segs, int n_segs, uint32_t n_blocks, hipStream_t stream) {
+    if (n_blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(walk_gen_kernel, dim3(n_blocks), dim3(kWalkThreads), 0, stream, segs, n_segs);
     return hipGetLastError();
 }
 
-hipError_t launch_walk_carry(double *base, int64_t first_block, int64_t n_blocks, const double *blocksum,
-                             hipStream_t stream) {
-    if (n_blocks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(walk_carry_kernel, dim3(1), dim3(64), 0, stream, base, first_block, n_blocks, blocksum);
+hipError_t launch_walk_carry(const CarrySegDev *segs, int n_segs, hipStream_t stream) {
+    if (n_segs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(walk_carry_kernel, dim3(uint32_t(n_segs)), dim3(64), 0, stream, segs);
     return hipGetLastError();
 }
 
