@@ -241,6 +241,16 @@ TxDev make_tx(lpk_ctx *ctx, const lpk_tx_desc &d, uint32_t idx) {
     return T;
 }
 
+// Precision-form group of a transmitter (kGroup*).
+int tx_group(const TxDev &T) {
+    if (T.gated) return kGroupGeneric;
+    if (T.signal_kind == LPK_SIG_CA) return kGroupCa;
+    if (T.signal_kind == LPK_SIG_BPSK || T.signal_kind == LPK_SIG_OOK || T.signal_kind == LPK_SIG_QAM16)
+        return kGroupFramed;
+    if (T.signal_kind == LPK_SIG_CW && !T.cw_anchored) return kGroupCw;
+    return kGroupGeneric;
+}
+
 void *pinned_slot(lpk_ctx *ctx, size_t bytes) {
     PinSlot &s = ctx->pin[ctx->pin_next];
     ctx->pin_next = (ctx->pin_next + 1) & 3;
@@ -426,6 +436,8 @@ void render(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form,
     LinkDev *ld = static_cast<LinkDev *>(ctx->links.ensure(link_bytes, ctx->stream));
 
     uint64_t tiles = 0, link_off = 0, dt_off = 0, st_off = 0, cd_off = 0;
+    uint32_t tap_slot = tap ? tap->slot : 0;
+    std::vector<uint32_t> perm;
     std::vector<float *> out_ptr(n_jobs, nullptr);
     std::vector<int16_t *> code_ptr(n_jobs, nullptr);
     for (uint32_t o = 0; o < n_jobs; ++o) {
@@ -439,9 +451,25 @@ void render(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form,
         if (tiles > 0xFFFFFFFFull) throw Fail{LPK_E_INVALID, "render call holds more than 2^32 tiles"};
         J.link0 = uint32_t(link_off);
         J.n_active = B.n_active;
+        // summation order: the reference's for the plain form; grouped by
+        // signal kind (stable within a group) for the precision form
+        perm.resize(B.n_active);
+        for (uint32_t s = 0; s < B.n_active; ++s) perm[s] = s;
+        if (form == LPK_FORM_PRECISION) {
+            auto group = [&](uint32_t s) { return tx_group(ctx->tx_host[B.active[s]]); };
+            std::stable_sort(perm.begin(), perm.end(), [&](uint32_t a, uint32_t b) { return group(a) < group(b); });
+            for (int g = 0; g < 3; ++g) {
+                uint32_t e = 0;
+                while (e < B.n_active && group(perm[e]) <= g) ++e;
+                J.group_end[g] = e;
+            }
+            if (tap)
+                for (uint32_t s = 0; s < B.n_active; ++s)
+                    if (perm[s] == tap->slot) tap_slot = s;
+        }
         for (uint32_t s = 0; s < B.n_active; ++s) {
             LinkDev L{};
-            L.tx = B.active[s];
+            L.tx = B.active[perm[s]];
             L.tau_if = kTau * (ctx->tx_carrier[L.tx] - B.lo_hz);  // src/channel/mod.rs:240
             lh[link_off + s] = L;
         }
@@ -521,8 +549,8 @@ void render(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form,
         hip_check(hipEventRecord(ctx->ev0, ctx->stream), "event record");
         hip_check(launch_render(form, jd, int(n_jobs), ld, ctx->tx_dev, uint32_t(tiles),
                                 tap ? tap->t_emit : nullptr, tap ? tap->range : nullptr,
-                                tap ? tap->phase : nullptr, tap ? tap->delta_t : nullptr,
-                                tap ? tap->slot : 0, ctx->stream),
+                                tap ? tap->phase : nullptr, tap ? tap->delta_t : nullptr, tap_slot,
+                                ctx->stream),
                   "render launch");
         hip_check(hipEventRecord(ctx->ev1, ctx->stream), "event record");
         ctx->timed = true;

@@ -344,7 +344,7 @@ struct TapOut {
 // (src/channel/mod.rs:79), which the lanes compute exactly per sample. The
 // first part of the record holds everything the sample loop reads on its
 // fast path, so the loop touches no transmitter table in global memory.
-constexpr int32_t kLinkSmooth = 1, kLinkGated = 2, kLinkAnchoredCw = 4;
+constexpr int32_t kLinkSmooth = 1;
 struct alignas(16) Anchor {
     // read by every lane of the sample loop
     double r0m, q0d1, q0c2, rd0;   // r0(tu) = r0m + x (q0d1 + q0c2 x) + rd0 del0, x = tu - tu_mid
@@ -352,19 +352,17 @@ struct alignas(16) Anchor {
     double k01;                    // (d r1 / d t1, position and Sagnac) * rd0 / c
     double r1_mid;                 // range at the middle sample (model, or exact when not smooth)
     double neg_tau_src, tau_if;    // -TAU f_src, TAU (f_src - f_lo)
-    double epoch_rel, rate;        // signal epoch and symbol rate (framed kinds)
+    double epoch_rel, rate;        // signal epoch; symbol rate (framed) or omega (CW)
     double hu;                     // symbol (framed) or NAV bit (C/A) index near the tile's first sample
-    double omega;                  // CW baseband angular offset
     const int8_t *code;            // C/A chips
     float amp_m;                   // (float)(amp_factor / r1_mid)
     float inv_rm;                  // 1 / r1_mid
     float h0re, h0im;              // value of index hu: symbol (framed) or NAV sign (C/A, h0re)
     float h1re, h1im;              // value of index hu + 1
-    int32_t kind, flags;           // signal kind; kLink* bits
+    int32_t flags, _pad0;          // kLink* bits
     uint32_t tx, _pad;             // transmitter (slow paths read its table)
     // anchor pass only
     double r0s[3], r1s[3];         // smooth ranges of iterations 1 and 2 at the anchors
-    double r1x_mid;                // exact (reference arithmetic) range at the middle sample
     int64_t cell[2][2];            // grid nodes of both iterations at the first and last anchors
 };
 constexpr int kAnchorGroup = 64;  // transmitters per anchor pass
@@ -428,6 +426,95 @@ __device__ __forceinline__ void solve_smooth(const TxDev &T, const double *obs, 
     rd0 = rq[0];
     rd1q = rq[1];
     rd1t = rt[1];
+}
+
+// One (sample, transmitter) contribution of the precision form, for the
+// transmitter group G (kGroup*): the fitted range with the per-sample rounding
+// corrections, s(t_emit), the carrier phase in the reference's operation
+// order reduced exactly, the hardware sine and cosine, the f32 mix, and the
+// exact f64 sum. Slow paths (a tile crossing a grid node, a symbol or NAV bit
+// other than the tile's hints, gates) read the transmitter's table.
+template <int G>
+__device__ __forceinline__ void precision_link(const Anchor &A, const TxDev *__restrict__ txs, const double *obs,
+                                               double epoch, double tu, double t_local, double dlt, double x,
+                                               double del0, double &sum_re, double &sum_im, bool tap_here,
+                                               double &tap_te, double &tap_r, double &tap_ph) {
+    double r;
+    if (A.flags & kLinkSmooth) {
+        const double r0 = fma(A.rd0, del0, fma(x, fma(A.q0c2, x, A.q0d1), A.r0m));
+        const double t1 = fma(-r0, kInvC, tu);
+        const double del1 = ((epoch + t1) - epoch) - t1;
+        r = fma(A.rd1q, del1, fma(-A.k01, del0, fma(x, fma(A.q1c2, x, A.q1d1), A.r1m)));
+    } else {
+        double te;
+        int64_t c0, c1;
+        solve_emission(txs[A.tx], obs, tu, epoch, te, r, c0, c1);
+    }
+    const double t_emit = fma(-r, kInvC, tu);
+    // s(t_emit): the exact index decides; the tile's hints supply the value
+    // when the index is one they hold. CW's own phase joins the carrier phase.
+    float bre = 0.0f, bim = 0.0f;
+    double ph_cw = 0.0;
+    if (G == kGroupCa) {
+        const double dt = t_emit - A.epoch_rel;
+        const double b = floor(dt * kNavBitRate);
+        const float pm = b == A.hu ? A.h0re : (b == A.hu + 1.0 ? A.h1re : float(nav_pm(txs[A.tx], b)));
+        const int64_t ci = rem_euclid(floor_to_i64(floor(dt * kCaChipRate)), kCaCodeLength);
+        bre = static_cast<float>(A.code[ci]) * pm;
+    } else if (G == kGroupFramed) {
+        const double u = floor((t_emit - A.epoch_rel) * A.rate);
+        if (u == A.hu) {
+            bre = A.h0re;
+            bim = A.h0im;
+        } else if (u == A.hu + 1.0) {
+            bre = A.h1re;
+            bim = A.h1im;
+        } else {
+            framed_value(txs[A.tx], u, bre, bim);
+        }
+    } else if (G == kGroupCw) {
+        ph_cw = static_cast<double>(static_cast<float>(A.rate * t_emit));  // rate holds omega
+        bre = 1.0f;
+    } else {
+        const TxDev &T = txs[A.tx];
+        if (!T.gated || gate_on(T, t_emit)) {
+            if (T.signal_kind == 0) {
+                ph_cw = static_cast<double>(cw_phase(T, t_emit));
+                bre = 1.0f;
+            } else {
+                baseband(T, t_emit, bre, bim);
+            }
+        }
+    }
+    // src/channel/mod.rs:256-258 in the reference's operation order, so its
+    // f64 rounding is followed; then an exact Cody-Waite reduction (24 + 24 +
+    // 53-bit split of 2 pi, exact for |k| < 2^29) and the hardware sine and
+    // cosine of the reduced turn fraction.
+    const double phase = A.neg_tau_src * (r * kInvC + dlt) + A.tau_if * t_local;
+    const double pt = phase + ph_cw;
+    const double kk = rint(pt * 0.15915494309189535);
+    const double rr =
+        fma(-kk, -6.8604979977715316e-15, fma(-kk, -1.7484555314695172e-07, fma(-kk, 6.2831854820251465, pt)));
+    const float fr = static_cast<float>(rr * 0.15915494309189535);
+    const float spf = __builtin_amdgcn_sinf(fr), cpf = __builtin_amdgcn_cosf(fr);
+    const float amplitude = A.amp_m * (1.0f - static_cast<float>(r - A.r1_mid) * A.inv_rm);
+    float cre, cim;
+    if (G == kGroupCa || G == kGroupCw) {
+        const float ab = amplitude * bre;
+        cre = ab * cpf;
+        cim = ab * spf;
+    } else {
+        cre = amplitude * (bre * cpf - bim * spf);
+        cim = amplitude * (bre * spf + bim * cpf);
+    }
+    // the f32 contributions summed exactly in f64
+    sum_re += static_cast<double>(cre);
+    sum_im += static_cast<double>(cim);
+    if (tap_here) {
+        tap_te = t_emit;
+        tap_r = r;
+        tap_ph = phase;
+    }
 }
 
 // Workgroups are dealt round-robin to the 8 XCDs; give each XCD a contiguous
@@ -525,33 +612,25 @@ __global__ void __launch_bounds__(kTile, FORM ? 5 : 6) render_kernel(const JobDe
         const bool fit_ok = a_last - a_first >= 2;
         for (uint32_t g0 = 0; g0 < J.n_active; g0 += kAnchorGroup) {
             const uint32_t ng = J.n_active - g0 < kAnchorGroup ? J.n_active - g0 : kAnchorGroup;
-            // smooth solves at the three anchors of every transmitter of the
-            // group, and the exact solve at the middle one
-            for (uint32_t q = threadIdx.x; q < 4 * ng; q += kTile) {
-                const uint32_t s = q >> 2, a = q & 3;
+            // smooth solves at the three anchors of every transmitter of the group
+            for (uint32_t q = threadIdx.x; q < 3 * ng; q += kTile) {
+                const uint32_t s = q / 3, a = q - 3 * s;
                 const LinkDev L = links[J.link0 + g0 + s];
                 const TxDev &T = txs[L.tx];
                 Anchor &A = s_anc[s];
-                if (a == 3) {
-                    double te, r;
-                    int64_t c0, c1;
-                    solve_emission(T, J.obs, s_anc_t[1][0], J.epoch, te, r, c0, c1);
-                    A.r1x_mid = r;
+                const double eoff = T.traj_kind ? J.epoch - T.grid_t0 : 0.0;
+                double r0, r1, rd0, rd1q, rd1t;
+                int64_t c0, c1;
+                solve_smooth(T, J.obs, s_anc_t[a][0], eoff, r0, r1, c0, c1, rd0, rd1q, rd1t);
+                A.r0s[a] = r0;
+                A.r1s[a] = r1;
+                if (a != 1) {
+                    A.cell[a >> 1][0] = c0;
+                    A.cell[a >> 1][1] = c1;
                 } else {
-                    const double eoff = T.traj_kind ? J.epoch - T.grid_t0 : 0.0;
-                    double r0, r1, rd0, rd1q, rd1t;
-                    int64_t c0, c1;
-                    solve_smooth(T, J.obs, s_anc_t[a][0], eoff, r0, r1, c0, c1, rd0, rd1q, rd1t);
-                    A.r0s[a] = r0;
-                    A.r1s[a] = r1;
-                    if (a != 1) {
-                        A.cell[a >> 1][0] = c0;
-                        A.cell[a >> 1][1] = c1;
-                    } else {
-                        A.rd0 = rd0;
-                        A.rd1q = rd1q;
-                        A.k01 = rd1t * rd0 * kInvC;
-                    }
+                    A.rd0 = rd0;
+                    A.rd1q = rd1q;
+                    A.k01 = rd1t * rd0 * kInvC;
                 }
             }
             __syncthreads();
@@ -561,7 +640,7 @@ __global__ void __launch_bounds__(kTile, FORM ? 5 : 6) render_kernel(const JobDe
                 const TxDev &T = txs[L.tx];
                 const bool same = A.cell[0][0] == A.cell[1][0] && A.cell[0][1] == A.cell[1][1];
                 const bool smooth = fit_ok && (T.traj_kind == 0 || same);
-                double r1m = A.r1x_mid;
+                double r1m;
                 A.r0m = A.r0s[1];
                 A.r1m = A.r1s[1];
                 if (smooth) {
@@ -581,6 +660,11 @@ __global__ void __launch_bounds__(kTile, FORM ? 5 : 6) render_kernel(const JobDe
                     const double t1m = fma(-r0m, kInvC, s_anc_t[1][0]);
                     const double d1m = ((J.epoch + t1m) - J.epoch) - t1m;
                     r1m = fma(A.rd1q, d1m, fma(-A.k01, d0m, A.r1s[1]));
+                } else {
+                    // the exact solve at the middle sample (the lanes solve exactly too)
+                    double te;
+                    int64_t c0, c1;
+                    solve_emission(T, J.obs, s_anc_t[1][0], J.epoch, te, r1m, c0, c1);
                 }
                 A.r1_mid = r1m;
                 A.inv_rm = static_cast<float>(1.0 / r1m);
@@ -588,12 +672,9 @@ __global__ void __launch_bounds__(kTile, FORM ? 5 : 6) render_kernel(const JobDe
                 A.neg_tau_src = T.neg_tau_src;
                 A.tau_if = L.tau_if;
                 A.epoch_rel = T.epoch_rel;
-                A.rate = T.rate;
-                A.omega = T.omega;
+                A.rate = T.signal_kind == 0 ? T.omega : T.rate;
                 A.code = T.ca_code;
-                A.kind = T.signal_kind;
-                A.flags = (smooth ? kLinkSmooth : 0) | (T.gated ? kLinkGated : 0) |
-                          (T.signal_kind == 0 && T.cw_anchored ? kLinkAnchoredCw : 0);
+                A.flags = smooth ? kLinkSmooth : 0;
                 A.tx = L.tx;
                 // Values of the symbol (or NAV bit) the tile starts in and of
                 // the next one; lanes take them when their own exact index
@@ -612,74 +693,29 @@ __global__ void __launch_bounds__(kTile, FORM ? 5 : 6) render_kernel(const JobDe
                 }
             }
             __syncthreads();
-            for (uint32_t s = 0; s < ng; ++s) {
-                const Anchor &A = s_anc[s];
-                const int32_t flags = A.flags, kind = A.kind;
-                double r;
-                if (flags & kLinkSmooth) {
-                    const double r0 = fma(A.rd0, del0, fma(x, fma(A.q0c2, x, A.q0d1), A.r0m));
-                    const double t1 = fma(-r0, kInvC, tu);
-                    const double del1 = ((J.epoch + t1) - J.epoch) - t1;
-                    r = fma(A.rd1q, del1, fma(-A.k01, del0, fma(x, fma(A.q1c2, x, A.q1d1), A.r1m)));
-                } else {
-                    double te;
-                    int64_t c0, c1;
-                    solve_emission(txs[A.tx], J.obs, tu, J.epoch, te, r, c0, c1);
-                }
-                const float drf = static_cast<float>(r - A.r1_mid);
-                const double t_emit = fma(-r, kInvC, tu);
-                // s(t_emit): the exact index decides; the tile's hint supplies
-                // the value when the index is the one it holds. CW's own
-                // phase joins the carrier phase (one rotation instead of two).
-                float bre = 0.0f, bim = 0.0f;
-                double ph_cw = 0.0;
-                if (!(flags & kLinkGated) || gate_on(txs[A.tx], t_emit)) {
-                    const double dt = t_emit - A.epoch_rel;
-                    if (kind == 0) {
-                        ph_cw = (flags & kLinkAnchoredCw) ? static_cast<double>(cw_phase(txs[A.tx], t_emit))
-                                                          : static_cast<double>(static_cast<float>(A.omega * t_emit));
-                        bre = 1.0f;
-                    } else if (kind == 4) {
-                        const double b = floor(dt * kNavBitRate);
-                        const float pm =
-                            b == A.hu ? A.h0re : (b == A.hu + 1.0 ? A.h1re : float(nav_pm(txs[A.tx], b)));
-                        const int64_t ci = rem_euclid(floor_to_i64(floor(dt * kCaChipRate)), kCaCodeLength);
-                        bre = static_cast<float>(A.code[ci]) * pm;
-                    } else if (kind <= 3) {
-                        const double u = floor(dt * A.rate);
-                        if (u == A.hu) {
-                            bre = A.h0re;
-                            bim = A.h0im;
-                        } else if (u == A.hu + 1.0) {
-                            bre = A.h1re;
-                            bim = A.h1im;
-                        } else {
-                            framed_value(txs[A.tx], u, bre, bim);
-                        }
-                    }
-                }
-                // src/channel/mod.rs:256-258 in the reference's operation order,
-                // so its f64 rounding is followed; then an exact Cody-Waite
-                // reduction (24 + 24 + 53-bit split of 2 pi, exact for |k| < 2^29)
-                // and the hardware sine and cosine of the reduced turn fraction.
-                const double phase = A.neg_tau_src * (r * kInvC + dlt) + A.tau_if * t_local;
-                const double pt = phase + ph_cw;
-                const double kk = rint(pt * 0.15915494309189535);
-                const double rr = fma(-kk, -6.8604979977715316e-15,
-                                      fma(-kk, -1.7484555314695172e-07, fma(-kk, 6.2831854820251465, pt)));
-                const float fr = static_cast<float>(rr * 0.15915494309189535);
-                const float spf = __builtin_amdgcn_sinf(fr), cpf = __builtin_amdgcn_cosf(fr);
-                const float amplitude = A.amp_m * (1.0f - drf * A.inv_rm);
-                const float cre = amplitude * (bre * cpf - bim * spf);
-                const float cim = amplitude * (bre * spf + bim * cpf);
-                // the f32 contributions summed exactly in f64
-                sum_re += static_cast<double>(cre);
-                sum_im += static_cast<double>(cim);
-                if (tap.on && g0 + s == tap.slot && valid) {
-                    tap.t_emit[j] = t_emit;
-                    tap.range[j] = r;
-                    tap.phase[j] = phase;
-                }
+            // the sample loops, one per transmitter group
+            const uint32_t e_ca = J.group_end[kGroupCa] > g0 ? min(J.group_end[kGroupCa] - g0, ng) : 0u;
+            const uint32_t e_fr = J.group_end[kGroupFramed] > g0 ? min(J.group_end[kGroupFramed] - g0, ng) : 0u;
+            const uint32_t e_cw = J.group_end[kGroupCw] > g0 ? min(J.group_end[kGroupCw] - g0, ng) : 0u;
+            const bool tapping = tap.on && valid && tap.slot >= g0 && tap.slot < g0 + ng;
+            double tv0 = 0.0, tv1 = 0.0, tv2 = 0.0;
+            uint32_t s = 0;
+            for (; s < e_ca; ++s)
+                precision_link<kGroupCa>(s_anc[s], txs, J.obs, J.epoch, tu, t_local, dlt, x, del0, sum_re, sum_im,
+                                         tapping && g0 + s == tap.slot, tv0, tv1, tv2);
+            for (; s < e_fr; ++s)
+                precision_link<kGroupFramed>(s_anc[s], txs, J.obs, J.epoch, tu, t_local, dlt, x, del0, sum_re,
+                                             sum_im, tapping && g0 + s == tap.slot, tv0, tv1, tv2);
+            for (; s < e_cw; ++s)
+                precision_link<kGroupCw>(s_anc[s], txs, J.obs, J.epoch, tu, t_local, dlt, x, del0, sum_re, sum_im,
+                                         tapping && g0 + s == tap.slot, tv0, tv1, tv2);
+            for (; s < ng; ++s)
+                precision_link<kGroupGeneric>(s_anc[s], txs, J.obs, J.epoch, tu, t_local, dlt, x, del0, sum_re,
+                                              sum_im, tapping && g0 + s == tap.slot, tv0, tv1, tv2);
+            if (tapping) {
+                tap.t_emit[j] = tv0;
+                tap.range[j] = tv1;
+                tap.phase[j] = tv2;
             }
             __syncthreads();
         }
