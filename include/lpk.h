@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define LPK_ABI_VERSION 1
+#define LPK_ABI_VERSION 2
 
 typedef enum {
     LPK_OK = 0,
@@ -149,6 +149,16 @@ typedef struct {
     void *out_dev;               /* device complex64 [clen]; NULL = internal staging */
     void *out_host;              /* optional host complex64 [clen], filled by the render */
     int16_t *adc_codes_host;     /* optional host int16 [2*clen]: ADC integer levels */
+    /* Receiver antenna pattern (src/rx/mod.rs:23-58), applied per source in the
+     * channel contribution (src/channel/mod.rs:190-207, :253-255): sqrt(G) at the
+     * arrival elevation, 0 below the cutoff (the sample's contribution is then
+     * skipped). has_antenna = 0: no antenna, gain 1, nothing skipped. */
+    int32_t has_antenna, has_cutoff;
+    double antenna_peak_dbi, antenna_horizon_dbi, antenna_fb_db, antenna_cutoff_deg;
+    uint8_t *unoccluded;         /* optional host uint8 [n_active], in `active` order: 1 where
+                                  * the transmitter's gain was non-zero for some sample of the
+                                  * chunk (src/channel/mod.rs:208-211); the render call then
+                                  * returns only once the flags are written */
 } lpk_band_chunk;
 
 typedef struct lpk_ctx lpk_ctx;

@@ -12,6 +12,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblpk.so")
 
+ABI_VERSION = 2  # LPK_ABI_VERSION of the header these structures mirror
 LPK_OK = 0
 STATUS = {
     0: "LPK_OK", -1: "LPK_E_INVALID", -2: "LPK_E_NOMEM", -3: "LPK_E_HIP",
@@ -76,6 +77,10 @@ class BandChunk(C.Structure):
         ("post", PostOp * 4),
         ("out_dev", C.c_void_p), ("out_host", C.c_void_p),
         ("adc_codes_host", C.POINTER(C.c_int16)),
+        ("has_antenna", C.c_int32), ("has_cutoff", C.c_int32),
+        ("antenna_peak_dbi", _d), ("antenna_horizon_dbi", _d), ("antenna_fb_db", _d),
+        ("antenna_cutoff_deg", _d),
+        ("unoccluded", C.POINTER(C.c_uint8)),
     ]
 
 

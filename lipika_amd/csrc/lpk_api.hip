@@ -14,7 +14,7 @@
 #include "lpk_device.hpp"
 
 namespace lpk {
-hipError_t launch_render(int form, const JobDev *jobs, int n_jobs, const LinkDev *links,
+hipError_t launch_render(int form, bool antenna, const JobDev *jobs, int n_jobs, const LinkDev *links,
                          const TxDev *txs, uint32_t n_tiles, double *tap_t_emit, double *tap_range,
                          double *tap_phase, double *tap_delta_t, uint32_t tap_slot,
                          hipStream_t stream);
@@ -365,6 +365,7 @@ void render(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form,
     if (n_jobs && !jobs) throw Fail{LPK_E_INVALID, "jobs is NULL"};
     // ---- validation (src/pipeline.rs:759-767 and the descriptor contracts)
     uint64_t total_links = 0, dt_total = 0, stage_total = 0, code_total = 0;
+    bool want_flags = false, any_antenna = false;
     for (uint32_t q = 0; q < n_jobs; ++q) {
         const lpk_band_chunk &B = jobs[q];
         auto bad = [&](int code, const std::string &why) {
@@ -388,6 +389,12 @@ void render(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form,
         if (B.delta_t)
             for (uint64_t j = 0; j < B.clen; ++j)
                 if (!is_fin(B.delta_t[j])) throw bad(LPK_E_NONFINITE, "delta_t holds a non-finite value");
+        if (B.has_antenna &&
+            (!is_fin(B.antenna_peak_dbi) || !is_fin(B.antenna_horizon_dbi) || !is_fin(B.antenna_fb_db) ||
+             (B.has_cutoff && !is_fin(B.antenna_cutoff_deg))))
+            throw bad(LPK_E_NONFINITE, "antenna parameters");
+        if (B.unoccluded && B.n_active) want_flags = true;
+        if (B.has_antenna) any_antenna = true;
         total_links += B.n_active;
         if (B.delta_t) dt_total += B.clen;
         if (!B.out_dev) stage_total += B.clen;
@@ -428,12 +435,20 @@ void render(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form,
     // ---- job and link tables
     const size_t job_bytes = size_t(std::max<uint32_t>(n_jobs, 1)) * sizeof(JobDev);
     const size_t link_bytes = size_t(std::max<uint64_t>(total_links, 1)) * sizeof(LinkDev);
+    const size_t flag_bytes = size_t(std::max<uint64_t>(total_links, 1)) * sizeof(uint32_t);
     const int slot = ctx->pin_next;
-    char *pin = static_cast<char *>(pinned_slot(ctx, job_bytes + link_bytes));
+    char *pin = static_cast<char *>(pinned_slot(ctx, job_bytes + link_bytes + flag_bytes));
     JobDev *jh = reinterpret_cast<JobDev *>(pin);
     LinkDev *lh = reinterpret_cast<LinkDev *>(pin + job_bytes);
+    std::memset(pin + job_bytes + link_bytes, 0, flag_bytes);
     JobDev *jd = static_cast<JobDev *>(ctx->jobs.ensure(job_bytes, ctx->stream));
-    LinkDev *ld = static_cast<LinkDev *>(ctx->links.ensure(link_bytes, ctx->stream));
+    LinkDev *ld = static_cast<LinkDev *>(ctx->links.ensure(link_bytes + flag_bytes, ctx->stream));
+    uint32_t *fd = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(ld) + link_bytes);
+    struct FlagJob {
+        uint32_t q, link0;
+        std::vector<uint32_t> perm;
+    };
+    std::vector<FlagJob> flag_jobs;
 
     uint64_t tiles = 0, link_off = 0, dt_off = 0, st_off = 0, cd_off = 0;
     uint32_t tap_slot = tap ? tap->slot : 0;
@@ -473,6 +488,16 @@ void render(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form,
             L.tau_if = kTau * (ctx->tx_carrier[L.tx] - B.lo_hz);  // src/channel/mod.rs:240
             lh[link_off + s] = L;
         }
+        if (B.has_antenna) {
+            J.ant_on = 1;
+            J.ant_cut = B.has_cutoff ? 1 : 0;
+            J.ant_peak = B.antenna_peak_dbi;
+            J.ant_horizon = B.antenna_horizon_dbi;
+            J.ant_fb = B.antenna_fb_db;
+            J.ant_cutoff = B.antenna_cutoff_deg;
+            J.unocc = fd;
+        }
+        if (B.unoccluded && B.n_active) flag_jobs.push_back(FlagJob{q, uint32_t(link_off), perm});
         link_off += B.n_active;
         J.fs = B.fs_hz;
         J.lo = B.lo_hz;
@@ -531,7 +556,8 @@ void render(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form,
         jh[o] = J;
     }
     hip_check(hipMemcpyAsync(jd, jh, job_bytes, hipMemcpyHostToDevice, ctx->stream), "job table upload");
-    hip_check(hipMemcpyAsync(ld, lh, link_bytes, hipMemcpyHostToDevice, ctx->stream), "link table upload");
+    hip_check(hipMemcpyAsync(ld, lh, link_bytes + flag_bytes, hipMemcpyHostToDevice, ctx->stream),
+              "link table upload");
     pinned_mark(ctx, slot);
 
     if (tiles) {
@@ -547,13 +573,25 @@ void render(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form,
             ++ctx->tused;
         }
         hip_check(hipEventRecord(ctx->ev0, ctx->stream), "event record");
-        hip_check(launch_render(form, jd, int(n_jobs), ld, ctx->tx_dev, uint32_t(tiles),
+        hip_check(launch_render(form, any_antenna, jd, int(n_jobs), ld, ctx->tx_dev, uint32_t(tiles),
                                 tap ? tap->t_emit : nullptr, tap ? tap->range : nullptr,
                                 tap ? tap->phase : nullptr, tap ? tap->delta_t : nullptr, tap_slot,
                                 ctx->stream),
                   "render launch");
         hip_check(hipEventRecord(ctx->ev1, ctx->stream), "event record");
         ctx->timed = true;
+    }
+    if (want_flags) {
+        // src/channel/mod.rs:208-211: with no antenna, every sample counts
+        std::vector<uint32_t> fl(total_links);
+        hip_check(hipMemcpyAsync(fl.data(), fd, total_links * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream),
+                  "occlusion flag download");
+        hip_check(hipStreamSynchronize(ctx->stream), "occlusion flag download");
+        for (const FlagJob &F : flag_jobs) {
+            const lpk_band_chunk &B = jobs[F.q];
+            for (uint32_t s = 0; s < B.n_active; ++s)
+                B.unoccluded[F.perm[s]] = B.has_antenna ? (fl[F.link0 + s] ? 1 : 0) : (B.clen > 0 ? 1 : 0);
+        }
     }
     for (uint32_t q = 0; q < n_jobs; ++q) {
         const lpk_band_chunk &B = jobs[q];

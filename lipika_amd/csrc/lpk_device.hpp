@@ -130,7 +130,7 @@ struct JobDev {
     // clock
     const double *delta_t;   // host-provided offsets (device copy) or null
     const double *walk_base; // per 65 536-index block, absolute block index
-    const double *walk_toff; // per 256 indices of the window
+    const double *walk_toff; // per 32 indices of the window
     const float *walk_local; // per index of the window
     int64_t eps_k0;          // first master index of the window (block aligned)
     double c0, y0, master;
@@ -143,6 +143,10 @@ struct JobDev {
     PostDev post[4];
     float *out;              // interleaved complex64
     int16_t *codes;          // optional ADC codes (2 per sample)
+    // receiver antenna (src/rx/mod.rs:23-58): gain sqrt(G) at the arrival elevation
+    int32_t ant_on, ant_cut;
+    double ant_peak, ant_horizon, ant_fb, ant_cutoff;
+    uint32_t *unocc;         // per link of the job: set to 1 where some sample's gain is non-zero
 };
 
 }  // namespace lpk

@@ -1,6 +1,6 @@
 // lpk_kernels.hip -- gfx950 kernels of the IQ-generation backend.
 //
-// render_kernel<FORM>: one workgroup renders one tile of kTile consecutive
+// render_kernel<FORM, ANT>: one workgroup renders one tile of kTile consecutive
 // output samples (tiles sit at absolute multiples of kTile in the band's
 // sample index) of one (receiver, band, chunk) job. Every tile of every job of
 // a render call is one launch. Each lane owns one sample and walks the job's
@@ -194,6 +194,27 @@ __device__ __forceinline__ void solve_emission(const TxDev &T, const double *obs
         t = tu - r * kInvC;
     }
     t_emit = t;
+}
+
+// Antenna::amplitude_at (src/rx/mod.rs:48-58) at the arrival elevation of
+// the source at the solved emission time (src/channel/mod.rs:190-207): the
+// position is queried at epoch + t_emit (no Sagnac rotation), sin(el) =
+// LOS . up with up = O / |O|, clamped, then asin and x * (180 / pi). Only the
+// render kernels instantiated for antenna scenes contain it.
+__device__ __forceinline__ float antenna_gain(const JobDev &J, const TxDev &T, double t_emit) {
+    double sx, sy, sz;
+    int64_t cell;
+    tx_position(T, J.epoch + t_emit, sx, sy, sz, cell);
+    const double ox = J.obs[0], oy = J.obs[1], oz = J.obs[2];
+    const double on = sqrt(ox * ox + oy * oy + oz * oz);
+    const double dx = sx - ox, dy = sy - oy, dz = sz - oz;
+    const double dn = sqrt(dx * dx + dy * dy + dz * dz);
+    const double sin_el = (dx * ox + dy * oy + dz * oz) / (dn * on);
+    const double c = sin_el < -1.0 ? -1.0 : (sin_el > 1.0 ? 1.0 : sin_el);
+    const double el = asin(c) * (180.0 / 3.141592653589793);
+    if (J.ant_cut && el < J.ant_cutoff) return 0.0f;
+    const double dbi = el >= 0.0 ? J.ant_horizon + (J.ant_peak - J.ant_horizon) * (el / 90.0) : J.ant_horizon - J.ant_fb;
+    return static_cast<float>(pow(10.0, dbi / 20.0));
 }
 
 // ---------------------------------------------------------------- signals
@@ -434,11 +455,12 @@ __device__ __forceinline__ void solve_smooth(const TxDev &T, const double *obs, 
 // order reduced exactly, the hardware sine and cosine, the f32 mix, and the
 // exact f64 sum. Slow paths (a tile crossing a grid node, a symbol or NAV bit
 // other than the tile's hints, gates) read the transmitter's table.
-template <int G>
-__device__ __forceinline__ void precision_link(const Anchor &A, const TxDev *__restrict__ txs, const double *obs,
-                                               double epoch, double tu, double t_local, double dlt, double x,
-                                               double del0, double &sum_re, double &sum_im, bool tap_here,
-                                               double &tap_te, double &tap_r, double &tap_ph) {
+template <int G, bool ANT>
+__device__ __forceinline__ void precision_link(const Anchor &A, const TxDev *__restrict__ txs, const JobDev &J,
+                                               uint32_t link, const double *obs, double epoch, double tu,
+                                               double t_local, double dlt, double x, double del0, double &sum_re,
+                                               double &sum_im, bool tap_here, double &tap_te, double &tap_r,
+                                               double &tap_ph) {
     double r;
     if (A.flags & kLinkSmooth) {
         const double r0 = fma(A.rd0, del0, fma(x, fma(A.q0c2, x, A.q0d1), A.r0m));
@@ -451,6 +473,21 @@ __device__ __forceinline__ void precision_link(const Anchor &A, const TxDev *__r
         solve_emission(txs[A.tx], obs, tu, epoch, te, r, c0, c1);
     }
     const double t_emit = fma(-r, kInvC, tu);
+    // src/channel/mod.rs:256-258 in the reference's operation order, so its
+    // f64 rounding is followed
+    const double phase = A.neg_tau_src * (r * kInvC + dlt) + A.tau_if * t_local;
+    if (tap_here) {
+        tap_te = t_emit;
+        tap_r = r;
+        tap_ph = phase;
+    }
+    // antenna gain; an occluded sample adds nothing (src/channel/mod.rs:253-255)
+    float g = 1.0f;
+    if (ANT && J.ant_on) {
+        g = antenna_gain(J, txs[A.tx], t_emit);
+        if (g != 0.0f && J.unocc) J.unocc[J.link0 + link] = 1u;
+        if (g == 0.0f) return;
+    }
     // s(t_emit): the exact index decides; the tile's hints supply the value
     // when the index is one they hold. CW's own phase joins the carrier phase.
     float bre = 0.0f, bim = 0.0f;
@@ -486,18 +523,16 @@ __device__ __forceinline__ void precision_link(const Anchor &A, const TxDev *__r
             }
         }
     }
-    // src/channel/mod.rs:256-258 in the reference's operation order, so its
-    // f64 rounding is followed; then an exact Cody-Waite reduction (24 + 24 +
-    // 53-bit split of 2 pi, exact for |k| < 2^29) and the hardware sine and
-    // cosine of the reduced turn fraction.
-    const double phase = A.neg_tau_src * (r * kInvC + dlt) + A.tau_if * t_local;
+    // an exact Cody-Waite reduction of the phase (24 + 24 + 53-bit split of
+    // 2 pi, exact for |k| < 2^29) and the hardware sine and cosine of the
+    // reduced turn fraction
     const double pt = phase + ph_cw;
     const double kk = rint(pt * 0.15915494309189535);
     const double rr =
         fma(-kk, -6.8604979977715316e-15, fma(-kk, -1.7484555314695172e-07, fma(-kk, 6.2831854820251465, pt)));
     const float fr = static_cast<float>(rr * 0.15915494309189535);
     const float spf = __builtin_amdgcn_sinf(fr), cpf = __builtin_amdgcn_cosf(fr);
-    const float amplitude = A.amp_m * (1.0f - static_cast<float>(r - A.r1_mid) * A.inv_rm);
+    const float amplitude = A.amp_m * (1.0f - static_cast<float>(r - A.r1_mid) * A.inv_rm) * g;
     float cre, cim;
     if (G == kGroupCa || G == kGroupCw) {
         const float ab = amplitude * bre;
@@ -510,11 +545,6 @@ __device__ __forceinline__ void precision_link(const Anchor &A, const TxDev *__r
     // the f32 contributions summed exactly in f64
     sum_re += static_cast<double>(cre);
     sum_im += static_cast<double>(cim);
-    if (tap_here) {
-        tap_te = t_emit;
-        tap_r = r;
-        tap_ph = phase;
-    }
 }
 
 // Workgroups are dealt round-robin to the 8 XCDs; give each XCD a contiguous
@@ -524,8 +554,8 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n) {
     return x * per + (x < rem ? x : rem) + (b >> 3);
 }
 
-template <int FORM>
-__global__ void __launch_bounds__(kTile, FORM ? 5 : 6) render_kernel(const JobDev *__restrict__ jobs, int n_jobs,
+template <int FORM, bool ANT>
+__global__ void __launch_bounds__(kTile, ANT ? 4 : (FORM ? 5 : 6)) render_kernel(const JobDev *__restrict__ jobs, int n_jobs,
                                                        const LinkDev *__restrict__ links,
                                                        const TxDev *__restrict__ txs,
                                                        uint32_t tile_base, uint32_t n_tiles, TapOut tap) {
@@ -574,22 +604,29 @@ __global__ void __launch_bounds__(kTile, FORM ? 5 : 6) render_kernel(const JobDe
             double t_emit, r;
             int64_t c0, c1;
             solve_emission(T, J.obs, tu, J.epoch, t_emit, r, c0, c1);
-            float bre, bim;
-            baseband(T, t_emit, bre, bim);
             // src/channel/mod.rs:255-267
             const double delay = r * kInvC;
             const double phase = T.neg_tau_src * (delay + dlt) + L.tau_if * t_local;
-            double sp, cp;
-            sincos(phase, &sp, &cp);
-            const float cpf = static_cast<float>(cp), spf = static_cast<float>(sp);
-            const float amplitude = static_cast<float>(T.amp_factor / r);
-            acc_re = acc_re + amplitude * (bre * cpf - bim * spf);
-            acc_im = acc_im + amplitude * (bre * spf + bim * cpf);
             if (tap.on && s == tap.slot && valid) {
                 tap.t_emit[j] = t_emit;
                 tap.range[j] = r;
                 tap.phase[j] = phase;
             }
+            // antenna gain; an occluded sample adds nothing (src/channel/mod.rs:253-255)
+            float g = 1.0f;
+            if (ANT && J.ant_on) {
+                g = antenna_gain(J, T, t_emit);
+                if (g != 0.0f && J.unocc) J.unocc[J.link0 + s] = 1u;
+                if (g == 0.0f) continue;
+            }
+            float bre, bim;
+            baseband(T, t_emit, bre, bim);
+            double sp, cp;
+            sincos(phase, &sp, &cp);
+            const float cpf = static_cast<float>(cp), spf = static_cast<float>(sp);
+            const float amplitude = static_cast<float>(T.amp_factor / r) * g;
+            acc_re = acc_re + amplitude * (bre * cpf - bim * spf);
+            acc_im = acc_im + amplitude * (bre * spf + bim * cpf);
         }
     } else {
         const int a_first = static_cast<int>(first - tile_start);
@@ -701,16 +738,16 @@ __global__ void __launch_bounds__(kTile, FORM ? 5 : 6) render_kernel(const JobDe
             double tv0 = 0.0, tv1 = 0.0, tv2 = 0.0;
             uint32_t s = 0;
             for (; s < e_ca; ++s)
-                precision_link<kGroupCa>(s_anc[s], txs, J.obs, J.epoch, tu, t_local, dlt, x, del0, sum_re, sum_im,
+                precision_link<kGroupCa, ANT>(s_anc[s], txs, J, g0 + s, J.obs, J.epoch, tu, t_local, dlt, x, del0, sum_re, sum_im,
                                          tapping && g0 + s == tap.slot, tv0, tv1, tv2);
             for (; s < e_fr; ++s)
-                precision_link<kGroupFramed>(s_anc[s], txs, J.obs, J.epoch, tu, t_local, dlt, x, del0, sum_re,
+                precision_link<kGroupFramed, ANT>(s_anc[s], txs, J, g0 + s, J.obs, J.epoch, tu, t_local, dlt, x, del0, sum_re,
                                              sum_im, tapping && g0 + s == tap.slot, tv0, tv1, tv2);
             for (; s < e_cw; ++s)
-                precision_link<kGroupCw>(s_anc[s], txs, J.obs, J.epoch, tu, t_local, dlt, x, del0, sum_re, sum_im,
+                precision_link<kGroupCw, ANT>(s_anc[s], txs, J, g0 + s, J.obs, J.epoch, tu, t_local, dlt, x, del0, sum_re, sum_im,
                                          tapping && g0 + s == tap.slot, tv0, tv1, tv2);
             for (; s < ng; ++s)
-                precision_link<kGroupGeneric>(s_anc[s], txs, J.obs, J.epoch, tu, t_local, dlt, x, del0, sum_re,
+                precision_link<kGroupGeneric, ANT>(s_anc[s], txs, J, g0 + s, J.obs, J.epoch, tu, t_local, dlt, x, del0, sum_re,
                                               sum_im, tapping && g0 + s == tap.slot, tv0, tv1, tv2);
             if (tapping) {
                 tap.t_emit[j] = tv0;
@@ -872,19 +909,16 @@ __global__ void __launch_bounds__(64) walk_carry_kernel(const CarrySegDev *__res
 
 namespace lpk {
 
-hipError_t launch_render(int form, const JobDev *jobs, int n_jobs, const LinkDev *links,
+hipError_t launch_render(int form, bool antenna, const JobDev *jobs, int n_jobs, const LinkDev *links,
                          const TxDev *txs, uint32_t n_tiles, double *tap_t_emit, double *tap_range,
                          double *tap_phase, double *tap_delta_t, uint32_t tap_slot,
                          hipStream_t stream) {
     TapOut tap{tap_t_emit, tap_range, tap_phase, tap_delta_t, tap_slot, tap_t_emit ? 1u : 0u};
+    auto kernel = form == 0 ? (antenna ? render_kernel<0, true> : render_kernel<0, false>)
+                            : (antenna ? render_kernel<1, true> : render_kernel<1, false>);
     for (uint32_t base = 0; base < n_tiles; base += kMaxLaunchTiles) {
         const uint32_t n = n_tiles - base < kMaxLaunchTiles ? n_tiles - base : kMaxLaunchTiles;
-        if (form == 0)
-            hipLaunchKernelGGL(render_kernel<0>, dim3(n), dim3(kTile), 0, stream, jobs, n_jobs, links, txs,
-                               base, n, tap);
-        else
-            hipLaunchKernelGGL(render_kernel<1>, dim3(n), dim3(kTile), 0, stream, jobs, n_jobs, links, txs,
-                               base, n, tap);
+        hipLaunchKernelGGL(kernel, dim3(n), dim3(kTile), 0, stream, jobs, n_jobs, links, txs, base, n, tap);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

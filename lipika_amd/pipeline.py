@@ -104,6 +104,7 @@ class PreparedBand:
     active: np.ndarray            # uint32 transmitter indices, summation order
     noise_power_w: float
     post: list                    # [(kind, bits, value)]
+    antenna: Optional[tuple] = None  # (peak_dbi, horizon_dbi, fb_db, cutoff_deg or None)
 
 
 @dataclass
@@ -265,8 +266,6 @@ def _post_ops(band: ReceiverBandConfig) -> list:
     fe = band.frontend
     if fe is None:
         return ops
-    if fe.antenna is not None:
-        raise NotImplementedError("antenna patterns (and the horizon gate) are not on the GPU path")
     if fe.gain_db is not None:
         ops.append((_lib.POST_GAIN, 0, float(fe.gain_db)))
     if fe.nonlinearity is not None:
@@ -274,6 +273,19 @@ def _post_ops(band: ReceiverBandConfig) -> list:
     if fe.adc is not None:
         ops.append((_lib.POST_ADC, int(fe.adc.bits), float(fe.adc.full_scale_v)))
     return ops
+
+
+def _antenna(band: ReceiverBandConfig) -> Optional[tuple]:
+    """The band's antenna pattern (src/rx/mod.rs:117-125), or None."""
+    fe = band.frontend
+    if fe is None or fe.antenna is None:
+        return None
+    a = fe.antenna
+    vals = [float(a["peak_dbi"]), float(a["horizon_dbi"]), float(a["fb_db"])]
+    cutoff = a.get("cutoff_deg")
+    if not all(math.isfinite(v) for v in vals) or (cutoff is not None and not math.isfinite(float(cutoff))):
+        raise ValueError(f"band '{band.name}': antenna parameters must be finite")
+    return (*vals, None if cutoff is None else float(cutoff))
 
 
 def prepare(scene: SceneConfig, time_base: TimeBase = TimeBase.SCENE) -> PreparedScene:
@@ -299,7 +311,8 @@ def prepare(scene: SceneConfig, time_base: TimeBase = TimeBase.SCENE) -> Prepare
                       if lib.lpk_bands_overlap(t.carrier_hz, t.bandwidth_hz, b.lo_carrier_hz, bw)]
             p = b.noise_power_w if b.noise_power_w is not None else \
                 float(lib.lpk_thermal_noise_power_w(b.noise_temperature_k, b.sample_rate_hz))
-            bands.append(PreparedBand(b, n, np.asarray(active, dtype=np.uint32), float(p), _post_ops(b)))
+            bands.append(PreparedBand(b, n, np.asarray(active, dtype=np.uint32), float(p), _post_ops(b),
+                                      _antenna(b)))
         rxs.append(PreparedRx(r.name, pos, r.clock, master, bands))
     return PreparedScene(meta, txs, rxs)
 
@@ -387,9 +400,11 @@ class Backend:
 
     def job(self, r_idx: int, b_idx: int, work_idx: int, rng_seed_base: int, i0: int, clen: int,
             out_host: Optional[np.ndarray] = None, out_dev: Optional[int] = None,
-            codes: Optional[np.ndarray] = None, delta_t: Optional[np.ndarray] = None) -> tuple:
+            codes: Optional[np.ndarray] = None, delta_t: Optional[np.ndarray] = None,
+            unoccluded: Optional[np.ndarray] = None) -> tuple:
         """One lpk_band_chunk for samples [i0, i0+clen) of band b_idx of receiver r_idx.
-        Returns (descriptor, keep-alive list)."""
+        Returns (descriptor, keep-alive list). `unoccluded` (uint8 [n_active])
+        receives the per-transmitter flags of the chunk."""
         sc = self.scene
         pb = sc.receivers[r_idx].bands[b_idx]
         j = _lib.BandChunk()
@@ -414,6 +429,14 @@ class Backend:
         if codes is not None:
             assert codes.dtype == np.int16 and codes.flags.c_contiguous and codes.size >= 2 * clen
             j.adc_codes_host = _ptr(codes, C.c_int16)
+        if pb.antenna is not None:
+            peak, horizon, fb, cutoff = pb.antenna
+            j.has_antenna, j.antenna_peak_dbi, j.antenna_horizon_dbi, j.antenna_fb_db = 1, peak, horizon, fb
+            if cutoff is not None:
+                j.has_cutoff, j.antenna_cutoff_deg = 1, cutoff
+        if unoccluded is not None:
+            assert unoccluded.dtype == np.uint8 and unoccluded.flags.c_contiguous and len(unoccluded) >= len(pb.active)
+            j.unoccluded = _ptr(unoccluded, C.c_uint8)
         return j, keep
 
     def render(self, jobs: Sequence, form: int = _lib.FORM_PLAIN) -> None:

@@ -641,13 +641,37 @@ void orc_post_ops(const orc_post_op *ops, uint32_t n_ops, float *iq, uint64_t n,
 
 /* src/channel/mod.rs:164-271 with no antenna: solve, baseband, mix into `out`
  * (which the caller zeroed). Optional taps for one transmitter. */
-static void tx_contribution(const orc_tx *tx, const double obs[3], const double *t_local,
-                            const double *t_utc, const double *delta_t, uint64_t n, double lo,
-                            double epoch, double *t_emit, double *ranges, float *bb, float *out,
-                            double *tap_t_emit, double *tap_range, double *tap_phase) {
+/* Antenna::amplitude_at, src/rx/mod.rs:48-58: sqrt(G) at elevation elev_deg. */
+float orc_antenna_amplitude(const orc_band *band, double elev_deg) {
+    if (band->ant_has_cutoff && elev_deg < band->ant_cutoff_deg) return 0.0f;
+    double dbi = elev_deg >= 0.0
+        ? band->ant_horizon_dbi + (band->ant_peak_dbi - band->ant_horizon_dbi) * (elev_deg / 90.0)
+        : band->ant_horizon_dbi - band->ant_fb_db;
+    return (float)pow(10.0, dbi / 20.0);
+}
+
+/* Arrival elevation of a source at `src` seen from `obs` against obs's
+ * geocentric up, in degrees, src/channel/mod.rs:193-201 (clamp, asin,
+ * to_degrees = x * (180 / pi)). */
+double orc_arrival_elevation_deg(const double obs[3], const double src[3]) {
+    double ox = obs[0], oy = obs[1], oz = obs[2];
+    double on = sqrt(ox * ox + oy * oy + oz * oz);
+    double dx = src[0] - ox, dy = src[1] - oy, dz = src[2] - oz;
+    double dn = sqrt(dx * dx + dy * dy + dz * dz);
+    double sin_el = (dx * ox + dy * oy + dz * oz) / (dn * on);
+    double c = sin_el < -1.0 ? -1.0 : (sin_el > 1.0 ? 1.0 : sin_el);
+    return asin(c) * (180.0 / PI);
+}
+
+static void tx_contribution(const orc_tx *tx, const orc_band *band, const double obs[3],
+                            const double *t_local, const double *t_utc, const double *delta_t,
+                            uint64_t n, double lo, double epoch, double *t_emit, double *ranges,
+                            float *bb, float *out, uint8_t *unoccluded, double *tap_t_emit,
+                            double *tap_range, double *tap_phase) {
     for (uint64_t i = 0; i < n; ++i)
         orc_solve_emission(tx, obs, t_utc[i], epoch, NEWTON_ITERATIONS, &t_emit[i], &ranges[i]);
     orc_baseband(tx, t_emit, n, bb);
+    int any = 0;
     double src_carrier = tx->carrier_hz;
     double wavelength_m = SPEED_OF_LIGHT_M_S / src_carrier;
     double tx_amplitude_v = sqrt(tx->tx_power_w);
@@ -659,19 +683,30 @@ static void tx_contribution(const orc_tx *tx, const double obs[3], const double 
         double r = ranges[i];
         double delay = r * inv_c;
         double phase = neg_tau_src * (delay + delta_t[i]) + tau_if * t_local[i];
-        double sp, cp;
-        sincos(phase, &sp, &cp);
-        float amplitude = (float)(amp_factor / r);
-        float bre = bb[2 * i], bim = bb[2 * i + 1];
-        float cpf = (float)cp, spf = (float)sp;
-        out[2 * i] = out[2 * i] + amplitude * (bre * cpf - bim * spf);
-        out[2 * i + 1] = out[2 * i + 1] + amplitude * (bre * spf + bim * cpf);
         if (tap_t_emit) {
             tap_t_emit[i] = t_emit[i];
             tap_range[i] = r;
             tap_phase[i] = phase;
         }
+        /* antenna gain at the solved emission time, src/channel/mod.rs:190-207;
+         * an occluded sample leaves the contribution at zero (:253-255) */
+        float g = 1.0f;
+        if (band->ant_on) {
+            double src[3];
+            tx_position(tx, epoch + t_emit[i], src);
+            g = orc_antenna_amplitude(band, orc_arrival_elevation_deg(obs, src));
+            if (g == 0.0f) continue;
+        }
+        any = 1;
+        double sp, cp;
+        sincos(phase, &sp, &cp);
+        float amplitude = (float)(amp_factor / r) * g;
+        float bre = bb[2 * i], bim = bb[2 * i + 1];
+        float cpf = (float)cp, spf = (float)sp;
+        out[2 * i] = out[2 * i] + amplitude * (bre * cpf - bim * spf);
+        out[2 * i + 1] = out[2 * i + 1] + amplitude * (bre * spf + bim * cpf);
     }
+    if (unoccluded && any) *unoccluded = 1;
 }
 
 typedef struct {
@@ -717,19 +752,20 @@ static void render_chunk(const orc_tx *txs, const orc_band *band, const double o
         for (uint32_t k = 0; k < na; ++k) {
             float *c = s->contrib + (uint64_t)k * 2 * clen;
             memset(c, 0, 2 * clen * sizeof(float));
-            tx_contribution(&txs[band->active[k]], obs, s->t_local, s->t_utc, s->delta_t, clen,
+            tx_contribution(&txs[band->active[k]], band, obs, s->t_local, s->t_utc, s->delta_t, clen,
                             band->lo_hz, band->epoch_unix, s->t_emit + (uint64_t)k * clen,
                             s->ranges + (uint64_t)k * clen, s->bb + (uint64_t)k * 2 * clen, c,
-                            NULL, NULL, NULL);
+                            band->unoccluded ? band->unoccluded + k : NULL, NULL, NULL, NULL);
         }
     } else {
         for (uint32_t k = 0; k < na; ++k) {
             float *c = s->contrib + (uint64_t)k * 2 * clen;
             memset(c, 0, 2 * clen * sizeof(float));
             int tap = tap_t_emit && k == tap_slot;
-            tx_contribution(&txs[band->active[k]], obs, s->t_local, s->t_utc, s->delta_t, clen,
+            tx_contribution(&txs[band->active[k]], band, obs, s->t_local, s->t_utc, s->delta_t, clen,
                             band->lo_hz, band->epoch_unix, s->t_emit + (uint64_t)k * clen,
                             s->ranges + (uint64_t)k * clen, s->bb + (uint64_t)k * 2 * clen, c,
+                            band->unoccluded ? band->unoccluded + k : NULL,
                             tap ? tap_t_emit + i0 : NULL, tap ? tap_range + i0 : NULL,
                             tap ? tap_phase + i0 : NULL);
         }

@@ -96,7 +96,15 @@ typedef struct {
     uint64_t noise_seed;
     uint32_t n_post, _p;
     orc_post_op post[4];
+    /* receiver antenna, src/rx/mod.rs:23-58 (ant_on = 0: none) */
+    int32_t ant_on, ant_has_cutoff;
+    double ant_peak_dbi, ant_horizon_dbi, ant_fb_db, ant_cutoff_deg;
+    uint8_t *unoccluded;         /* optional output [n_active]: ORed over the rendered chunks */
 } orc_band;
+
+/* ---- antenna (src/rx/mod.rs:48-58, src/channel/mod.rs:190-207) ---- */
+float orc_antenna_amplitude(const orc_band *band, double elev_deg);
+double orc_arrival_elevation_deg(const double obs[3], const double src[3]);
 
 /* ---- geometry ---- */
 void orc_geodetic_to_ecef(double lat_deg, double lon_deg, double h_m, double out[3]);

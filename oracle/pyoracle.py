@@ -49,7 +49,10 @@ class OrcBand(C.Structure):
     _fields_ = [("rx", _u32), ("n_active", _u32), ("active", P(_u32)),
                 ("fs_hz", _d), ("lo_hz", _d), ("epoch_unix", _d), ("duration_s", _d),
                 ("noise_power_w", _d), ("noise_seed", _u64), ("n_post", _u32), ("_p", _u32),
-                ("post", OrcPost * 4)]
+                ("post", OrcPost * 4),
+                ("ant_on", C.c_int32), ("ant_has_cutoff", C.c_int32),
+                ("ant_peak_dbi", _d), ("ant_horizon_dbi", _d), ("ant_fb_db", _d), ("ant_cutoff_deg", _d),
+                ("unoccluded", P(C.c_uint8))]
 
 
 class OrcClock(C.Structure):
@@ -96,6 +99,8 @@ def load() -> C.CDLL:
         "orc_philox4x64": (None, [P(_u64), P(_u64), P(_u64)]),
         "orc_box_muller": (None, [_u64, P(C.c_float), P(C.c_float)]),
         "orc_clock_normal": (_d, [_u64, _u64]),
+        "orc_antenna_amplitude": (C.c_float, [P(OrcBand), _d]),
+        "orc_arrival_elevation_deg": (_d, [P(_d), P(_d)]),
         "orc_awgn_normals": (None, [_u64, _u64, P(C.c_float), P(C.c_float)]),
         "orc_thermal_noise_power_w": (_d, [_d, _d]),
         "orc_clock_init": (None, [P(OrcClock), P(OrcRx)]),
@@ -178,12 +183,20 @@ class OracleScene:
             b.n_post = len(pb.post)
             for k, (kind, bits, value) in enumerate(pb.post):
                 b.post[k].kind, b.post[k].bits, b.post[k].value = kind, bits, value
+            if pb.antenna is not None:
+                peak, horizon, fb, cutoff = pb.antenna
+                b.ant_on, b.ant_peak_dbi, b.ant_horizon_dbi, b.ant_fb_db = 1, peak, horizon, fb
+                if cutoff is not None:
+                    b.ant_has_cutoff, b.ant_cutoff_deg = 1, cutoff
 
     def render_band(self, work_idx: int, n: int | None = None, chunk_samples: int = 1 << 20,
-                    tap_slot: int | None = None, codes: bool = False):
-        """Renders samples [0, n) of unit `work_idx`; returns (iq, codes, taps)."""
+                    tap_slot: int | None = None, codes: bool = False, unoccluded: np.ndarray | None = None):
+        """Renders samples [0, n) of unit `work_idx`; returns (iq, codes, taps).
+        `unoccluded` (uint8 [n_active], zeroed by the caller) receives the
+        per-transmitter flags."""
         lib = load()
         r_idx, pb = self.units[work_idx]
+        self.bands[work_idx].unoccluded = _ptr(unoccluded, C.c_uint8)
         n = pb.n if n is None else n
         out = np.empty(n, dtype=np.complex64)
         cd = np.empty(2 * n, dtype=np.int16) if codes else None

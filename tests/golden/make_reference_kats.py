@@ -152,6 +152,21 @@ KATS = {
                   [1576444001.0, 0.0, 1575420000.0, 2048000.0, 0],
                   [1580420000.0, 20000000.0, 1575420000.0, 2048000.0, 1]],
     },
+    "antenna_pattern": {
+        "source": "src/rx/mod.rs:196-209",
+        "antenna": {"peak_dbi": 3.9, "horizon_dbi": -3.0, "fb_db": 8.5},
+        "cases_dbi": [[90.0, 3.9], [0.0, -3.0], [-10.0, -11.5]],
+        "tolerance_db": 1e-3,
+        "monotone": [0.0, 45.0, 90.0],
+        "cutoff_deg": 0.0, "occluded_at": -0.1, "present_at": 0.0, "unchanged_at": 45.0,
+    },
+    "antenna_occlusion": {
+        "source": "src/channel/mod.rs:384-420",
+        "obs": [6378137.0, 0.0, 0.0], "above": [6.9e6, 0.0, 0.0], "below": [-7.0e6, 0.0, 0.0],
+        "carrier_hz": 1575420000.0, "prn": 1, "tx_power_w": 20.0, "epoch": 1717260840.0,
+        "unit_antenna": {"peak_dbi": 0.0, "horizon_dbi": 0.0, "fb_db": 0.0},
+        "n": 8,
+    },
     "xoshiro256pp_reference": {
         "source": "xoshiro256++ reference vector (Blackman & Vigna), state {1,2,3,4}",
         "state": [1, 2, 3, 4],

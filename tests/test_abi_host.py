@@ -35,7 +35,37 @@ def test_library_exports_every_declared_symbol(lpk):
     assert sorted(_lib.EXPORTED) == names
     for n in names:
         assert hasattr(lpk, n), n
-    assert lpk.lpk_abi_version() == 1
+    assert lpk.lpk_abi_version() == _lib.ABI_VERSION
+
+
+def test_ctypes_layout_matches_header(tmp_path):
+    """Every ctypes structure of the binding has the size and field offsets the
+    C header gives (compiled here with gcc against include/lpk.h)."""
+    import shutil
+    import subprocess
+    if not shutil.which("gcc"):
+        pytest.skip("gcc is not available")
+    structs = {"lpk_tx_desc": _lib.TxDesc, "lpk_rx_desc": _lib.RxDesc, "lpk_post_op": _lib.PostOp,
+               "lpk_band_chunk": _lib.BandChunk}
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "lpk.h"', "int main(void) {"]
+    for cname, ct in structs.items():
+        lines.append(f'    printf("{cname} size %zu\\n", sizeof({cname}));')
+        for field, _ in ct._fields_:
+            lines.append(f'    printf("{cname} {field} %zu\\n", offsetof({cname}, {field}));')
+    lines += ["    return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    subprocess.run(["gcc", "-std=c11", "-I", inc, str(src), "-o", str(exe)], check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines():
+        a, b, v = line.split()
+        got[(a, b)] = int(v)
+    for cname, ct in structs.items():
+        assert got[(cname, "size")] == C.sizeof(ct), cname
+        for field, _ in ct._fields_:
+            assert got[(cname, field)] == getattr(ct, field).offset, (cname, field)
 
 
 def test_context_creation_fails_cleanly_without_gpu(lpk):

@@ -12,6 +12,7 @@ the GPU path. Tolerances:
     so bit-identical given the same clock offsets; statistics against the
     reference's bounds.
 """
+import ctypes as C
 import math
 
 import numpy as np
@@ -235,6 +236,76 @@ def test_precision_form_error_vs_fp64_oracle(backend, name):
                 compare_taps(backend, orc, sc, w, r, b, seed, slot, n, form=_lib.FORM_PRECISION,
                              label=f"precision {name} band {b}")
             w += 1
+
+
+def _antenna_scene():
+    """A Glasgow band with an antenna whose cutoff a LEO crosses mid-window,
+    a source 500 km overhead and one on the far side of the Earth."""
+    from lipika_amd.config import (Bpsk, CircularKeplerian, Cw, FrontendConfig, ReceiverBandConfig, SceneConfig,
+                                   SceneMeta, StationaryGeodetic, TransmitterConfig)
+    from oracle import pyoracle
+    orc = pyoracle.load()
+    obs = (C.c_double * 3)()
+    orc.orc_geodetic_to_ecef(55.86, -4.25, 10.0, obs)
+
+    def elevation(orbit, t):
+        p = (C.c_double * 3)()
+        orc.orc_kepler_position(*orbit, util.EPOCH, t, p)
+        return orc.orc_arrival_elevation_deg(obs, p)
+
+    best = None
+    for raan in range(0, 360, 15):
+        for nu in range(0, 360, 3):
+            orbit = (6.9e6, 60.0, float(raan), float(nu))
+            e0, e1 = elevation(orbit, util.EPOCH), elevation(orbit, util.EPOCH + 0.2)
+            if 10.0 < e0 < 60.0 and abs(e1 - e0) > 0.05 and (best is None or abs(e1 - e0) > best[1]):
+                best = (orbit, abs(e1 - e0))
+    orbit = best[0]
+    cutoff = elevation(orbit, util.EPOCH + 0.1)
+    e0, e1 = elevation(orbit, util.EPOCH), elevation(orbit, util.EPOCH + 0.2)
+    assert min(e0, e1) < cutoff < max(e0, e1)  # the LEO crosses the cutoff inside the window
+    sync = [-1, 1, 1, -1, 1, -1, -1, 1]
+    txs = [TransmitterConfig("leo", 437e6, Bpsk(9600.0, 2000, 3, sync), CircularKeplerian(*orbit, util.EPOCH), 0.0),
+           TransmitterConfig("overhead", 437.001e6, Cw(0.0), StationaryGeodetic(55.86, -4.25, 5.0e5), -10.0),
+           TransmitterConfig("antipode", 436.999e6, Cw(0.0), StationaryGeodetic(-55.86, 175.75, 5.0e5), 30.0)]
+    fe = FrontendConfig(antenna={"peak_dbi": 3.9, "horizon_dbi": -3.0, "fb_db": 8.5, "cutoff_deg": cutoff})
+    band = ReceiverBandConfig("uhf", 437e6, 250e3, 290.0, noise_power_w=0.0, frontend=fe)
+    return SceneConfig(SceneMeta(util.EPOCH, 0.2), txs, [util.lab_receiver([band])])
+
+
+@pytest.mark.parametrize("form", [_lib.FORM_PLAIN, _lib.FORM_PRECISION])
+def test_antenna_gain_and_occlusion(backend, form):
+    """src/channel/mod.rs:190-211 on the GPU: the gain at each sample's arrival
+    elevation, samples below the cutoff dropped, and the per-transmitter
+    unoccluded flags, against the oracle (src/rx/mod.rs:48-58)."""
+    seed = 12
+    sc = prepare(_antenna_scene())
+    backend.upload(sc, seed)
+    orc = OracleScene(sc, seed)
+    want_flags = np.zeros(3, dtype=np.uint8)
+    o, _, _ = orc.render_band(0, unoccluded=want_flags)
+    assert list(want_flags) == [1, 1, 0]
+    n = sc.receivers[0].bands[0].n
+    g = np.empty(n, dtype=np.complex64)
+    flags = np.zeros(3, dtype=np.uint8)
+    chunk = 20_000
+    for i0 in range(0, n, chunk):
+        f = np.zeros(3, dtype=np.uint8)
+        j, keep = backend.job(0, 0, 0, seed, i0, min(chunk, n - i0), out_host=g[i0:i0 + chunk], unoccluded=f)
+        backend.render([j], form)
+        flags |= f
+    backend.sync()
+    assert list(flags) == [1, 1, 0]
+    label = f"antenna, {'plain' if form == _lib.FORM_PLAIN else 'precision'}"
+    if form == _lib.FORM_PLAIN:
+        compare_iq(g, o, label)
+    else:
+        peak = float(np.max(np.abs(o)))
+        d = np.abs(g.astype(np.complex128) - o.astype(np.complex128))
+        _say(f"{label}: n={n} max|d|/peak={d.max() / peak:.3e}")
+        assert d.max() <= 2e-6 * peak
+    for slot in range(3):
+        compare_taps(backend, orc, sc, 0, 0, 0, seed, slot, 20_000, form=form, label=label)
 
 
 def test_plain_form_large_phase_scene(backend):

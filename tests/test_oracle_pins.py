@@ -467,6 +467,83 @@ def test_normals_are_standard_normal(orc):
     assert abs(np.corrcoef(pr[:, 0], pr[:, 1])[0, 1]) < 0.03
 
 
+# ---------------------------------------------------------------- antenna
+
+def _band_with_antenna(peak, horizon, fb, cutoff=None):
+    from oracle.pyoracle import OrcBand
+    b = OrcBand()
+    b.ant_on, b.ant_peak_dbi, b.ant_horizon_dbi, b.ant_fb_db = 1, peak, horizon, fb
+    if cutoff is not None:
+        b.ant_has_cutoff, b.ant_cutoff_deg = 1, cutoff
+    return b
+
+
+def test_antenna_pattern_known_answers(orc, kats):
+    k = kats["antenna_pattern"]
+    a = k["antenna"]
+    b = _band_with_antenna(a["peak_dbi"], a["horizon_dbi"], a["fb_db"])
+    dbi = lambda e: 20.0 * math.log10(orc.orc_antenna_amplitude(C.byref(b), e))
+    for e, want in k["cases_dbi"]:
+        assert abs(dbi(e) - want) < k["tolerance_db"], e
+    lo, mid, hi = k["monotone"]
+    assert dbi(lo) < dbi(mid) < dbi(hi)
+    # linear in dBi above the horizon: the midpoint gain is the mean of the ends
+    assert abs(dbi(mid) - 0.5 * (dbi(lo) + dbi(hi))) < 1e-5
+    bc = _band_with_antenna(a["peak_dbi"], a["horizon_dbi"], a["fb_db"], k["cutoff_deg"])
+    assert orc.orc_antenna_amplitude(C.byref(bc), k["occluded_at"]) == 0.0
+    assert orc.orc_antenna_amplitude(C.byref(bc), k["present_at"]) > 0.0
+    assert orc.orc_antenna_amplitude(C.byref(bc), k["unchanged_at"]) == orc.orc_antenna_amplitude(
+        C.byref(b), k["unchanged_at"])
+
+
+def test_arrival_elevation_closed_form(orc):
+    """Elevation against geocentric up: overhead 90, tangent 0, far side
+    negative, and atan2(radial, tangential) for sources in the obs meridian plane."""
+    R = 6378137.0
+    obs = (C.c_double * 3)(R, 0.0, 0.0)
+    el = lambda p: orc.orc_arrival_elevation_deg(obs, (C.c_double * 3)(*p))
+    assert el((R + 5e5, 0.0, 0.0)) == 90.0
+    assert el((R, 1e6, 0.0)) == 0.0
+    assert el((-7e6, 0.0, 0.0)) == -90.0
+    rng = np.random.default_rng(9)
+    for _ in range(200):
+        a, t = rng.uniform(-3e6, 3e6), rng.uniform(1e3, 3e6)
+        ang = rng.uniform(0, 2 * math.pi)
+        want = math.degrees(math.atan2(a, t))
+        assert abs(el((R + a, t * math.cos(ang), t * math.sin(ang))) - want) < 1e-9
+
+
+def test_antenna_cutoff_occludes_below_horizon(kats):
+    """src/channel/mod.rs:384-420 through the oracle's band render: a 0 deg
+    cutoff drops a source on the far side of the Earth and keeps an overhead
+    one; an open antenna (and no antenna) lets the far source through."""
+    from lipika_amd.config import (CaCode, FrontendConfig, ReceiverBandConfig, ReceiverConfig, SceneConfig,
+                                   SceneMeta, StationaryEcef, TransmitterConfig)
+    from lipika_amd.pipeline import prepare
+    from oracle.pyoracle import OracleScene
+    from lipika_amd.config import NodeClockSpec
+    k = kats["antenna_occlusion"]
+    ua = k["unit_antenna"]
+
+    def power(src, cutoff):
+        tx = TransmitterConfig("s", k["carrier_hz"], CaCode(k["prn"], k["epoch"]), StationaryEcef(*src),
+                               10.0 * math.log10(k["tx_power_w"]))
+        fe = None if cutoff == "none" else FrontendConfig(antenna=dict(ua, **({} if cutoff is None else
+                                                                             {"cutoff_deg": cutoff})))
+        band = ReceiverBandConfig("l1", k["carrier_hz"], 2.048e6, 290.0, noise_power_w=0.0, frontend=fe)
+        rx = ReceiverConfig("eq", StationaryEcef(*k["obs"]), NodeClockSpec(), [band])
+        sc = prepare(SceneConfig(SceneMeta(k["epoch"], k["n"] / 2.048e6), [tx], [rx]))
+        flags = np.zeros(1, dtype=np.uint8)
+        iq, _, _ = OracleScene(sc, 1).render_band(0, unoccluded=flags)
+        return float(np.sum(np.abs(iq.astype(np.complex128)) ** 2)), int(flags[0])
+
+    assert power(k["below"], "none")[0] > 0.0
+    assert power(k["below"], None)[0] > 0.0
+    assert power(k["below"], 0.0) == (0.0, 0)
+    p, f = power(k["above"], 0.0)
+    assert p > 0.0 and f == 1
+
+
 # ---------------------------------------------------------------- receiver ops
 
 def test_ktb(orc, kats):
