@@ -31,6 +31,9 @@ METRIC = "Msamples/s (rx-band×TX summed) at ≤1e-6 rad phase err; plain vs cho
 FLOPS_PER_CONTRIBUTION = 75        # FP64 work per contribution, counted from the reference source
 PEAK_FP64_TFLOPS = 78.6            # MI355X published FP64 peak
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per configuration-2 render dispatch (reads + writes), from the rocprofv3 FETCH_SIZE and
+# WRITE_SIZE passes scaled by a copy kernel of known size: profiles/render_config2_hbm_traffic.md
+TRAFFIC_CONFIG2 = {"plain": 1.930e9, "precision": 1.969e9}
 WORKLOADS = {
     2: "BPSK LEO 437 MHz + 6 C/A MEO 1575.42 MHz; bands 48 kHz and 2.048 MHz; clock walk and kTB noise on; 60 s",
     4: "40 transmitters (CW, OOK, BPSK, 16-QAM, C/A) x 64 receivers with 130 bands at 48 kHz to 8.184 MHz; "
@@ -182,8 +185,12 @@ def main():
                    "output_msamples_per_s": round(out_samples / (ms / 1e3) / 1e6, 3),
                    "forms": summary(results, contributions)},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
-                     "kernel": f"render_kernel<{0 if args.form == 'plain' else 1}>", "kernel_ms": round(kernel_ms, 4),
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
+                     "traffic": TRAFFIC_CONFIG2[args.form] if args.config == 2 else None,
+                     "traffic_source": "profiles/render_config2_hbm_traffic.md",
+                     "compute_unit": "FP64 vector ALU (the path issues no matrix ops)",
+                     "kernel": f"render_kernel<{0 if args.form == 'plain' else 1}, false>",
+                     "kernel_ms": round(kernel_ms, 4),
                      "flops_per_contribution": FLOPS_PER_CONTRIBUTION,
                      "output_gb_s": round(out_samples * 8 / (kernel_ms / 1e3) / 1e9, 2)},
         "cpu_baseline": None,
