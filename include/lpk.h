@@ -196,6 +196,43 @@ int lpk_clock_rewind(lpk_ctx *ctx);
 int lpk_render_taps(lpk_ctx *ctx, const lpk_band_chunk *job, int form, uint32_t slot,
                     double *t_emit, double *range, double *phase, double *delta_t);
 
+/* ---- tuning of the render launch (tile size and work split). The reference
+ *      has no counterpart: its chunk body runs on the CPU thread pool
+ *      (src/pipeline.rs:943-964). The plain form's samples do not depend on
+ *      the tuning; the precision form fits its model per tile, so its samples
+ *      move within its error bound when the tile size changes. ---- */
+typedef struct {
+    int32_t tile_shift;          /* log2 of the samples per workgroup tile: 7, 8 (default) or 9 */
+    int32_t xcd_map;             /* 1 (default): deal contiguous tile runs to each XCD; 0: round-robin */
+} lpk_tuning;
+int lpk_set_tuning(lpk_ctx *ctx, const lpk_tuning *tuning);
+int lpk_get_tuning(const lpk_ctx *ctx, lpk_tuning *tuning);
+/* Renders `jobs` (which must not ask for host outputs or flags) once per
+ * candidate tuning after a warm-up, keeps the fastest on the context and
+ * returns it with its render-kernel time. Synchronous. */
+int lpk_autotune(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form,
+                 lpk_tuning *best, double *best_ms);
+
+/* ---- streaming plans: generate_band's chunk loop (src/pipeline.rs:905-990)
+ *      for many bands at once. Step k renders chunk k (samples
+ *      [k*chunk_samples, (k+1)*chunk_samples)) of every band: the job, clock
+ *      walk and carry tables of the step go to HBM and the walk, carry and
+ *      render kernels run, all as one captured HIP graph replayed per step.
+ *      Each band descriptor covers the whole band (i0 = 0, clen = its
+ *      sample count) with the device clock (delta_t NULL) and no host
+ *      outputs or flags; out_dev, if set, holds the whole band and step k
+ *      writes its chunk there, otherwise the plan keeps one chunk buffer per
+ *      band (lpk_plan_output). Steps run in order from 0; step 0 restarts
+ *      the plan's receivers' clock walks. The tuning is the context's at
+ *      creation. ---- */
+typedef struct lpk_plan lpk_plan;
+int lpk_plan_create(lpk_ctx *ctx, const lpk_band_chunk *bands, uint32_t n_bands, uint64_t chunk_samples,
+                    int form, lpk_plan **out);
+int lpk_plan_step(lpk_plan *plan, uint64_t chunk);     /* asynchronous, like lpk_render_chunk */
+int lpk_plan_info(const lpk_plan *plan, uint64_t *n_chunks, uint32_t *max_tiles);
+int lpk_plan_output(const lpk_plan *plan, uint32_t band, void **chunk_dev);
+void lpk_plan_destroy(lpk_plan *plan);
+
 /* Time of the last lpk_render_chunk kernel on the context's stream, in ms,
  * measured with HIP events around the launch (valid after lpk_sync). */
 int lpk_last_kernel_ms(lpk_ctx *ctx, double *ms);

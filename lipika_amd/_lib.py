@@ -84,6 +84,10 @@ class BandChunk(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("tile_shift", C.c_int32), ("xcd_map", C.c_int32)]
+
+
 class LpkError(RuntimeError):
     """A non-zero lpk_status, with the library's error text."""
 
@@ -115,6 +119,14 @@ def load() -> C.CDLL:
         "lpk_clock_rewind": (_i32, [C.c_void_p]),
         "lpk_render_taps": (_i32, [C.c_void_p, P(BandChunk), C.c_int, _u32, P(_d), P(_d), P(_d), P(_d)]),
         "lpk_last_kernel_ms": (_i32, [C.c_void_p, P(_d)]),
+        "lpk_set_tuning": (_i32, [C.c_void_p, P(Tuning)]),
+        "lpk_get_tuning": (_i32, [C.c_void_p, P(Tuning)]),
+        "lpk_autotune": (_i32, [C.c_void_p, P(BandChunk), _u32, C.c_int, P(Tuning), P(_d)]),
+        "lpk_plan_create": (_i32, [C.c_void_p, P(BandChunk), _u32, _u64, C.c_int, P(C.c_void_p)]),
+        "lpk_plan_step": (_i32, [C.c_void_p, _u64]),
+        "lpk_plan_info": (_i32, [C.c_void_p, P(_u64), P(_u32)]),
+        "lpk_plan_output": (_i32, [C.c_void_p, _u32, P(C.c_void_p)]),
+        "lpk_plan_destroy": (None, [C.c_void_p]),
         "lpk_timing_reset": (_i32, [C.c_void_p]),
         "lpk_timing_read": (_i32, [C.c_void_p, P(_d), P(_u64)]),
         "lpk_device_alloc": (_i32, [C.c_void_p, C.c_size_t, P(C.c_void_p)]),
@@ -152,6 +164,8 @@ EXPORTED = (
     "lpk_ca_code", "lpk_nav_fold_seed", "lpk_payload_bpsk", "lpk_payload_ook",
     "lpk_payload_qam16", "lpk_elapsed_periods", "lpk_frame_anchor", "lpk_cw_anchor_phase",
     "lpk_bands_overlap", "lpk_thermal_noise_power_w",
+    "lpk_set_tuning", "lpk_get_tuning", "lpk_autotune",
+    "lpk_plan_create", "lpk_plan_step", "lpk_plan_info", "lpk_plan_output", "lpk_plan_destroy",
 )
 
 

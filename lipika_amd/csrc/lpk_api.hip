@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <string>
 #include <vector>
@@ -14,11 +15,12 @@
 #include "lpk_device.hpp"
 
 namespace lpk {
-hipError_t launch_render(int form, bool antenna, const JobDev *jobs, int n_jobs, const LinkDev *links,
-                         const TxDev *txs, uint32_t n_tiles, double *tap_t_emit, double *tap_range,
-                         double *tap_phase, double *tap_delta_t, uint32_t tap_slot,
-                         hipStream_t stream);
-hipError_t launch_walk_gen(const WalkSegDev *segs, int n_segs, uint32_t n_blocks, hipStream_t stream);
+hipError_t launch_render(int form, bool antenna, int tile_shift, bool xcd_map, const JobDev *jobs, int n_jobs,
+                         const LinkDev *links, const TxDev *txs, uint32_t n_tiles, const uint32_t *n_live,
+                         double *tap_t_emit, double *tap_range, double *tap_phase, double *tap_delta_t,
+                         uint32_t tap_slot, hipStream_t stream);
+hipError_t launch_walk_gen(const WalkSegDev *segs, int n_segs, uint32_t n_blocks, const uint32_t *n_live,
+                           hipStream_t stream);
 hipError_t launch_walk_carry(const CarrySegDev *segs, int n_segs, hipStream_t stream);
 }  // namespace lpk
 
@@ -96,10 +98,43 @@ struct lpk_ctx {
     std::vector<RxState> rx;
     bool have_scene = false;
     bool timed = false;
+    // tuning of the render launch
+    int tile_shift = kDefaultTileShift;
+    int xcd_map = 1;
     // per-call buffers
     DevBuf jobs, links, dt_stage, out_stage, codes_stage, blocksum, walk_tables;
     PinSlot pin[4];
     int pin_next = 0;
+};
+
+// A streaming plan (lpk_plan_*): chunk k of every band per step, as one
+// captured graph of [table copy, clock walk, carry, render] replayed per step.
+// Two graphs alternate so the host can fill one pinned table while the other
+// is in flight.
+struct lpk_plan {
+    lpk_ctx *ctx = nullptr;
+    int form = 0, tile_shift = kDefaultTileShift, xcd_map = 1;
+    bool antenna = false;
+    uint64_t chunk = 0, n_chunks = 0, next = 0;
+    std::vector<lpk_band_chunk> bands;
+    std::vector<std::vector<uint32_t>> active;
+    std::vector<uint32_t> order;          // bands in job-table order (heaviest first)
+    std::vector<uint64_t> link0;          // per band: its first link
+    uint64_t total_links = 0;
+    std::vector<uint32_t> walk_rx;        // receivers whose device clock the plan walks
+    std::vector<int> walk_slot;           // per receiver: its index in walk_rx, or -1
+    std::vector<DevBuf> local, toff;      // per walked receiver: the step's window
+    std::vector<int64_t> win_cap;         // per walked receiver: window capacity in blocks
+    DevBuf blocksum;
+    uint32_t max_blocks = 0, max_tiles = 0;
+    std::vector<DevBuf> outbuf;           // per band: the chunk buffer when out_dev is NULL
+    size_t off_jobs = 0, off_links = 0, off_flags = 0, off_segs = 0, off_carry = 0, bytes = 0;
+    DevBuf tables;
+    void *pin[2] = {nullptr, nullptr};
+    hipEvent_t done[2] = {nullptr, nullptr};
+    bool used[2] = {false, false};
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
 };
 
 namespace {
@@ -346,7 +381,7 @@ void prepare_walks(lpk_ctx *ctx, const std::vector<int64_t> &wlo, const std::vec
     char *dev = static_cast<char *>(ctx->walk_tables.ensure(seg_bytes + car_bytes, ctx->stream));
     hip_check(hipMemcpyAsync(dev, pin, seg_bytes + car_bytes, hipMemcpyHostToDevice, ctx->stream), "walk table upload");
     pinned_mark(ctx, slot);
-    hip_check(launch_walk_gen(reinterpret_cast<const WalkSegDev *>(dev), int(segs.size()), uint32_t(blocks),
+    hip_check(launch_walk_gen(reinterpret_cast<const WalkSegDev *>(dev), int(segs.size()), uint32_t(blocks), nullptr,
                               ctx->stream),
               "clock walk blocks");
     hip_check(launch_walk_carry(reinterpret_cast<const CarrySegDev *>(dev + seg_bytes), int(carries.size()),
@@ -354,45 +389,155 @@ void prepare_walks(lpk_ctx *ctx, const std::vector<int64_t> &wlo, const std::vec
               "clock walk carry");
 }
 
+// Master-index range [klo, khi] of the receiver clock that a job's samples
+// interpolate (src/clock/mod.rs:329-346); false when the job uses no device walk.
+bool job_window(const lpk_ctx *ctx, const lpk_band_chunk &B, uint64_t i0, uint64_t clen, int64_t &klo, int64_t &khi) {
+    const RxState &R = ctx->rx[B.rx];
+    if (!R.walk || B.delta_t || clen == 0) return false;
+    const double M = R.d.master_rate_hz;
+    const double xa = (double(i0) / B.fs_hz) * M;
+    const double xb = (double(i0 + clen - 1) / B.fs_hz) * M;
+    if (!(xb < 4.0e15)) throw Fail{LPK_E_INVALID, "clock index range too large"};
+    klo = xa <= 0.0 ? 0 : int64_t(std::floor(xa));
+    khi = xb <= 0.0 ? 0 : int64_t(std::floor(xb)) + 1;
+    return true;
+}
+
+// Checks one band-chunk descriptor against the scene and the descriptor
+// contracts (src/pipeline.rs:759-767).
+void validate_job(const lpk_ctx *ctx, const lpk_band_chunk &B, uint32_t q) {
+    auto bad = [&](int code, const std::string &why) { return Fail{code, "job " + std::to_string(q) + ": " + why}; };
+    if (B.rx >= ctx->rx.size()) throw bad(LPK_E_INVALID, "receiver index out of range");
+    if (!(B.fs_hz > 0.0) || !is_fin(B.fs_hz)) throw bad(LPK_E_INVALID, "fs_hz must be positive and finite");
+    if (!is_fin(B.lo_hz) || !is_fin(B.epoch_unix)) throw bad(LPK_E_NONFINITE, "lo_hz / epoch_unix");
+    if (!is_fin(B.noise_power_w) || B.noise_power_w < 0.0) throw bad(LPK_E_INVALID, "noise_power_w");
+    if (B.n_active && !B.active) throw bad(LPK_E_INVALID, "active list is NULL");
+    if (B.clen > (uint64_t(1) << 40) || B.i0 > (uint64_t(1) << 52)) throw bad(LPK_E_INVALID, "chunk too large");
+    for (uint32_t s = 0; s < B.n_active; ++s)
+        if (B.active[s] >= ctx->n_tx) throw bad(LPK_E_INVALID, "active transmitter index out of range");
+    if (B.n_post > 4) throw bad(LPK_E_INVALID, "at most 4 post-sum ops");
+    for (uint32_t p = 0; p < B.n_post; ++p) {
+        const lpk_post_op &P = B.post[p];
+        if (P.kind < LPK_POST_GAIN || P.kind > LPK_POST_ADC) throw bad(LPK_E_INVALID, "unknown post op");
+        if (P.kind == LPK_POST_ADC && (P.bits < 0 || P.bits > 16)) throw bad(LPK_E_INVALID, "ADC bits must be 0..16");
+        if (!is_fin(P.value)) throw bad(LPK_E_NONFINITE, "post op parameter");
+    }
+    if (B.delta_t)
+        for (uint64_t j = 0; j < B.clen; ++j)
+            if (!is_fin(B.delta_t[j])) throw bad(LPK_E_NONFINITE, "delta_t holds a non-finite value");
+    if (B.has_antenna &&
+        (!is_fin(B.antenna_peak_dbi) || !is_fin(B.antenna_horizon_dbi) || !is_fin(B.antenna_fb_db) ||
+         (B.has_cutoff && !is_fin(B.antenna_cutoff_deg))))
+        throw bad(LPK_E_NONFINITE, "antenna parameters");
+}
+
+// The job-table record of samples [i0, i0 + clen) of a band and its links
+// (written at `lh`), apart from the tile range, link offset, output, clock
+// window and flag pointers, which depend on the call. `perm` receives the
+// order of the links: the reference's for the plain form, grouped by signal
+// kind (stable within a group) for the precision form.
+JobDev fill_job(const lpk_ctx *ctx, const lpk_band_chunk &B, uint64_t i0, uint64_t clen, int form, LinkDev *lh,
+                std::vector<uint32_t> &perm) {
+    const RxState &R = ctx->rx[B.rx];
+    JobDev J{};
+    J.n_active = B.n_active;
+    perm.resize(B.n_active);
+    for (uint32_t s = 0; s < B.n_active; ++s) perm[s] = s;
+    if (form == LPK_FORM_PRECISION) {
+        auto group = [&](uint32_t s) { return tx_group(ctx->tx_host[B.active[s]]); };
+        std::stable_sort(perm.begin(), perm.end(), [&](uint32_t a, uint32_t b) { return group(a) < group(b); });
+        for (int g = 0; g < 3; ++g) {
+            uint32_t e = 0;
+            while (e < B.n_active && group(perm[e]) <= g) ++e;
+            J.group_end[g] = e;
+        }
+    }
+    for (uint32_t s = 0; s < B.n_active; ++s) {
+        LinkDev L{};
+        L.tx = B.active[perm[s]];
+        L.tau_if = kTau * (ctx->tx_carrier[L.tx] - B.lo_hz);  // src/channel/mod.rs:240
+        lh[s] = L;
+    }
+    if (B.has_antenna) {
+        J.ant_on = 1;
+        J.ant_cut = B.has_cutoff ? 1 : 0;
+        J.ant_peak = B.antenna_peak_dbi;
+        J.ant_horizon = B.antenna_horizon_dbi;
+        J.ant_fb = B.antenna_fb_db;
+        J.ant_cutoff = B.antenna_cutoff_deg;
+    }
+    J.fs = B.fs_hz;
+    J.lo = B.lo_hz;
+    J.epoch = B.epoch_unix;
+    J.i0 = i0;
+    J.clen = clen;
+    for (int k = 0; k < 3; ++k) J.obs[k] = R.d.pos_ecef[k];
+    J.c0 = R.d.initial_time_offset_s;
+    J.y0 = R.d.frequency_offset_y0;
+    J.master = R.d.master_rate_hz;
+    J.walk = (R.walk && !B.delta_t) ? 1 : 0;
+    J.sigma = float(std::sqrt(B.noise_power_w / 2.0));  // src/channel/mod.rs:328
+    J.noise_seed = B.noise_seed;
+    J.n_post = int32_t(B.n_post);
+    for (uint32_t p = 0; p < B.n_post; ++p) {
+        const lpk_post_op &P = B.post[p];
+        PostDev D{};
+        D.kind = P.kind;
+        if (P.kind == LPK_POST_GAIN) {
+            D.a = std::pow(10.0f, float(P.value / 20.0));  // src/rx/mod.rs:86
+        } else if (P.kind == LPK_POST_NONLINEARITY) {
+            D.a = float(4.0 / 3.0 / P.value);  // src/rx/mod.rs:90
+        } else {
+            const float fsv = float(P.value);  // src/rx/mod.rs:94-96
+            const int bits = P.bits < 2 ? 2 : P.bits;
+            const float levels = float((1ULL << (bits - 1)) - 1);
+            D.a = fsv;
+            D.b = fsv / levels;
+        }
+        J.post[p] = D;
+    }
+    return J;
+}
+
+uint32_t tiles_of(uint64_t i0, uint64_t clen, int tile_shift) {
+    return clen ? uint32_t(((i0 + clen - 1) >> tile_shift) - (i0 >> tile_shift) + 1) : 0u;
+}
+
 struct TapReq {
     uint32_t slot;
     double *t_emit, *range, *phase, *delta_t;
 };
 
+// Next timing event pair of the context (bench accounting, lpk_timing_*).
+void timing_begin(lpk_ctx *ctx) {
+    if (ctx->tused == ctx->tpool.size() && ctx->tused < 65536) {
+        std::pair<hipEvent_t, hipEvent_t> pr{nullptr, nullptr};
+        hip_check(hipEventCreate(&pr.first), "event create");
+        hip_check(hipEventCreate(&pr.second), "event create");
+        ctx->tpool.push_back(pr);
+    }
+    if (ctx->tused < ctx->tpool.size()) {
+        ctx->ev0 = ctx->tpool[ctx->tused].first;
+        ctx->ev1 = ctx->tpool[ctx->tused].second;
+        ++ctx->tused;
+    }
+    hip_check(hipEventRecord(ctx->ev0, ctx->stream), "event record");
+}
+void timing_end(lpk_ctx *ctx) {
+    hip_check(hipEventRecord(ctx->ev1, ctx->stream), "event record");
+    ctx->timed = true;
+}
+
 void render(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form, const TapReq *tap) {
     if (!ctx->have_scene) throw Fail{LPK_E_STATE, "no scene uploaded"};
     if (form != LPK_FORM_PLAIN && form != LPK_FORM_PRECISION) throw Fail{LPK_E_INVALID, "unknown form"};
     if (n_jobs && !jobs) throw Fail{LPK_E_INVALID, "jobs is NULL"};
-    // ---- validation (src/pipeline.rs:759-767 and the descriptor contracts)
+    // ---- validation
     uint64_t total_links = 0, dt_total = 0, stage_total = 0, code_total = 0;
     bool want_flags = false, any_antenna = false;
     for (uint32_t q = 0; q < n_jobs; ++q) {
         const lpk_band_chunk &B = jobs[q];
-        auto bad = [&](int code, const std::string &why) {
-            return Fail{code, "job " + std::to_string(q) + ": " + why};
-        };
-        if (B.rx >= ctx->rx.size()) throw bad(LPK_E_INVALID, "receiver index out of range");
-        if (!(B.fs_hz > 0.0) || !is_fin(B.fs_hz)) throw bad(LPK_E_INVALID, "fs_hz must be positive and finite");
-        if (!is_fin(B.lo_hz) || !is_fin(B.epoch_unix)) throw bad(LPK_E_NONFINITE, "lo_hz / epoch_unix");
-        if (!is_fin(B.noise_power_w) || B.noise_power_w < 0.0) throw bad(LPK_E_INVALID, "noise_power_w");
-        if (B.n_active && !B.active) throw bad(LPK_E_INVALID, "active list is NULL");
-        if (B.clen > (uint64_t(1) << 40) || B.i0 > (uint64_t(1) << 52)) throw bad(LPK_E_INVALID, "chunk too large");
-        for (uint32_t s = 0; s < B.n_active; ++s)
-            if (B.active[s] >= ctx->n_tx) throw bad(LPK_E_INVALID, "active transmitter index out of range");
-        if (B.n_post > 4) throw bad(LPK_E_INVALID, "at most 4 post-sum ops");
-        for (uint32_t p = 0; p < B.n_post; ++p) {
-            const lpk_post_op &P = B.post[p];
-            if (P.kind < LPK_POST_GAIN || P.kind > LPK_POST_ADC) throw bad(LPK_E_INVALID, "unknown post op");
-            if (P.kind == LPK_POST_ADC && (P.bits < 0 || P.bits > 16)) throw bad(LPK_E_INVALID, "ADC bits must be 0..16");
-            if (!is_fin(P.value)) throw bad(LPK_E_NONFINITE, "post op parameter");
-        }
-        if (B.delta_t)
-            for (uint64_t j = 0; j < B.clen; ++j)
-                if (!is_fin(B.delta_t[j])) throw bad(LPK_E_NONFINITE, "delta_t holds a non-finite value");
-        if (B.has_antenna &&
-            (!is_fin(B.antenna_peak_dbi) || !is_fin(B.antenna_horizon_dbi) || !is_fin(B.antenna_fb_db) ||
-             (B.has_cutoff && !is_fin(B.antenna_cutoff_deg))))
-            throw bad(LPK_E_NONFINITE, "antenna parameters");
+        validate_job(ctx, B, q);
         if (B.unoccluded && B.n_active) want_flags = true;
         if (B.has_antenna) any_antenna = true;
         total_links += B.n_active;
@@ -402,6 +547,7 @@ void render(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form,
     }
     if (tap && n_jobs != 1) throw Fail{LPK_E_INVALID, "taps need exactly one job"};
     if (tap && tap->slot >= jobs[0].n_active) throw Fail{LPK_E_INVALID, "tap slot out of range"};
+    const int ts = ctx->tile_shift;
 
     // ---- order: heavy jobs first so their tiles are dispatched first
     std::vector<uint32_t> order(n_jobs);
@@ -414,14 +560,8 @@ void render(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form,
     std::vector<int64_t> wlo(ctx->rx.size(), INT64_MAX), whi(ctx->rx.size(), -1);
     for (uint32_t q = 0; q < n_jobs; ++q) {
         const lpk_band_chunk &B = jobs[q];
-        RxState &R = ctx->rx[B.rx];
-        if (!R.walk || B.delta_t || B.clen == 0) continue;
-        const double M = R.d.master_rate_hz;
-        const double xa = (double(B.i0) / B.fs_hz) * M;
-        const double xb = (double(B.i0 + B.clen - 1) / B.fs_hz) * M;
-        if (!(xb < 4.0e15)) throw Fail{LPK_E_INVALID, "job " + std::to_string(q) + ": clock index range too large"};
-        const int64_t klo = xa <= 0.0 ? 0 : int64_t(std::floor(xa));
-        const int64_t khi = xb <= 0.0 ? 0 : int64_t(std::floor(xb)) + 1;
+        int64_t klo, khi;
+        if (!job_window(ctx, B, B.i0, B.clen, klo, khi)) continue;
         wlo[B.rx] = std::min(wlo[B.rx], klo);
         whi[B.rx] = std::max(whi[B.rx], khi);
     }
@@ -459,87 +599,29 @@ void render(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form,
         const uint32_t q = order[o];
         const lpk_band_chunk &B = jobs[q];
         const RxState &R = ctx->rx[B.rx];
-        JobDev J{};
+        JobDev J = fill_job(ctx, B, B.i0, B.clen, form, lh + link_off, perm);
         J.tile0 = uint32_t(tiles);
-        J.n_tiles = B.clen ? uint32_t(((B.i0 + B.clen - 1) >> kTileShift) - (B.i0 >> kTileShift) + 1) : 0u;
+        J.n_tiles = tiles_of(B.i0, B.clen, ts);
         tiles += J.n_tiles;
         if (tiles > 0xFFFFFFFFull) throw Fail{LPK_E_INVALID, "render call holds more than 2^32 tiles"};
         J.link0 = uint32_t(link_off);
-        J.n_active = B.n_active;
-        // summation order: the reference's for the plain form; grouped by
-        // signal kind (stable within a group) for the precision form
-        perm.resize(B.n_active);
-        for (uint32_t s = 0; s < B.n_active; ++s) perm[s] = s;
-        if (form == LPK_FORM_PRECISION) {
-            auto group = [&](uint32_t s) { return tx_group(ctx->tx_host[B.active[s]]); };
-            std::stable_sort(perm.begin(), perm.end(), [&](uint32_t a, uint32_t b) { return group(a) < group(b); });
-            for (int g = 0; g < 3; ++g) {
-                uint32_t e = 0;
-                while (e < B.n_active && group(perm[e]) <= g) ++e;
-                J.group_end[g] = e;
-            }
-            if (tap)
-                for (uint32_t s = 0; s < B.n_active; ++s)
-                    if (perm[s] == tap->slot) tap_slot = s;
-        }
-        for (uint32_t s = 0; s < B.n_active; ++s) {
-            LinkDev L{};
-            L.tx = B.active[perm[s]];
-            L.tau_if = kTau * (ctx->tx_carrier[L.tx] - B.lo_hz);  // src/channel/mod.rs:240
-            lh[link_off + s] = L;
-        }
-        if (B.has_antenna) {
-            J.ant_on = 1;
-            J.ant_cut = B.has_cutoff ? 1 : 0;
-            J.ant_peak = B.antenna_peak_dbi;
-            J.ant_horizon = B.antenna_horizon_dbi;
-            J.ant_fb = B.antenna_fb_db;
-            J.ant_cutoff = B.antenna_cutoff_deg;
-            J.unocc = fd;
-        }
+        if (tap)
+            for (uint32_t s = 0; s < B.n_active; ++s)
+                if (perm[s] == tap->slot) tap_slot = s;
+        if (B.has_antenna) J.unocc = fd;
         if (B.unoccluded && B.n_active) flag_jobs.push_back(FlagJob{q, uint32_t(link_off), perm});
         link_off += B.n_active;
-        J.fs = B.fs_hz;
-        J.lo = B.lo_hz;
-        J.epoch = B.epoch_unix;
-        J.i0 = B.i0;
-        J.clen = B.clen;
-        for (int k = 0; k < 3; ++k) J.obs[k] = R.d.pos_ecef[k];
         if (B.delta_t) {
             hip_check(hipMemcpyAsync(dt_dev + dt_off, B.delta_t, B.clen * 8, hipMemcpyHostToDevice, ctx->stream),
                       "delta_t upload");
             J.delta_t = dt_dev + dt_off;
             dt_off += B.clen;
         }
-        J.c0 = R.d.initial_time_offset_s;
-        J.y0 = R.d.frequency_offset_y0;
-        J.master = R.d.master_rate_hz;
-        J.walk = (R.walk && !B.delta_t) ? 1 : 0;
         if (J.walk) {
             J.walk_base = R.base;
             J.walk_toff = static_cast<const double *>(R.toff.p);
             J.walk_local = static_cast<const float *>(R.local.p);
             J.eps_k0 = (wlo[B.rx] >> kWalkBlockShift) << kWalkBlockShift;
-        }
-        J.sigma = float(std::sqrt(B.noise_power_w / 2.0));  // src/channel/mod.rs:328
-        J.noise_seed = B.noise_seed;
-        J.n_post = int32_t(B.n_post);
-        for (uint32_t p = 0; p < B.n_post; ++p) {
-            const lpk_post_op &P = B.post[p];
-            PostDev D{};
-            D.kind = P.kind;
-            if (P.kind == LPK_POST_GAIN) {
-                D.a = std::pow(10.0f, float(P.value / 20.0));  // src/rx/mod.rs:86
-            } else if (P.kind == LPK_POST_NONLINEARITY) {
-                D.a = float(4.0 / 3.0 / P.value);  // src/rx/mod.rs:90
-            } else {
-                const float fsv = float(P.value);  // src/rx/mod.rs:94-96
-                const int bits = P.bits < 2 ? 2 : P.bits;
-                const float levels = float((1ULL << (bits - 1)) - 1);
-                D.a = fsv;
-                D.b = fsv / levels;
-            }
-            J.post[p] = D;
         }
         if (B.out_dev) {
             J.out = static_cast<float *>(B.out_dev);
@@ -561,25 +643,12 @@ void render(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form,
     pinned_mark(ctx, slot);
 
     if (tiles) {
-        if (ctx->tused == ctx->tpool.size() && ctx->tused < 65536) {
-            std::pair<hipEvent_t, hipEvent_t> pr{nullptr, nullptr};
-            hip_check(hipEventCreate(&pr.first), "event create");
-            hip_check(hipEventCreate(&pr.second), "event create");
-            ctx->tpool.push_back(pr);
-        }
-        if (ctx->tused < ctx->tpool.size()) {
-            ctx->ev0 = ctx->tpool[ctx->tused].first;
-            ctx->ev1 = ctx->tpool[ctx->tused].second;
-            ++ctx->tused;
-        }
-        hip_check(hipEventRecord(ctx->ev0, ctx->stream), "event record");
-        hip_check(launch_render(form, any_antenna, jd, int(n_jobs), ld, ctx->tx_dev, uint32_t(tiles),
-                                tap ? tap->t_emit : nullptr, tap ? tap->range : nullptr,
-                                tap ? tap->phase : nullptr, tap ? tap->delta_t : nullptr, tap_slot,
-                                ctx->stream),
+        timing_begin(ctx);
+        hip_check(launch_render(form, any_antenna, ts, ctx->xcd_map != 0, jd, int(n_jobs), ld, ctx->tx_dev,
+                                uint32_t(tiles), nullptr, tap ? tap->t_emit : nullptr, tap ? tap->range : nullptr,
+                                tap ? tap->phase : nullptr, tap ? tap->delta_t : nullptr, tap_slot, ctx->stream),
                   "render launch");
-        hip_check(hipEventRecord(ctx->ev1, ctx->stream), "event record");
-        ctx->timed = true;
+        timing_end(ctx);
     }
     if (want_flags) {
         // src/channel/mod.rs:208-211: with no antenna, every sample counts
@@ -602,6 +671,262 @@ void render(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form,
             hip_check(hipMemcpyAsync(B.adc_codes_host, code_ptr[q], B.clen * 4, hipMemcpyDeviceToHost, ctx->stream),
                       "ADC code download");
     }
+}
+
+// --------------------------------------------------------------- plans
+
+// Samples of band b in chunk k.
+void plan_chunk(const lpk_plan *P, uint32_t b, uint64_t k, uint64_t &i0, uint64_t &clen) {
+    const uint64_t n = P->bands[b].clen;
+    i0 = k * P->chunk;
+    clen = i0 < n ? std::min<uint64_t>(P->chunk, n - i0) : 0;
+}
+
+// Union of the master-index windows of a receiver's bands in chunk k.
+bool plan_window(const lpk_plan *P, uint32_t r, uint64_t k, int64_t &lo, int64_t &hi) {
+    bool any = false;
+    lo = INT64_MAX;
+    hi = -1;
+    for (uint32_t b = 0; b < P->bands.size(); ++b) {
+        if (P->bands[b].rx != r) continue;
+        uint64_t i0, clen;
+        plan_chunk(P, b, k, i0, clen);
+        int64_t klo, khi;
+        if (!job_window(P->ctx, P->bands[b], i0, clen, klo, khi)) continue;
+        lo = std::min(lo, klo);
+        hi = std::max(hi, khi);
+        any = true;
+    }
+    return any;
+}
+
+// Fills the pinned table of step k: header (live tiles, live walk blocks),
+// jobs, links, zeroed flags, walk segments and carries.
+void plan_fill(lpk_plan *P, uint64_t k, char *dst) {
+    lpk_ctx *ctx = P->ctx;
+    char *dev = static_cast<char *>(P->tables.p);
+    uint32_t *hdr = reinterpret_cast<uint32_t *>(dst);
+    JobDev *jh = reinterpret_cast<JobDev *>(dst + P->off_jobs);
+    LinkDev *lh = reinterpret_cast<LinkDev *>(dst + P->off_links);
+    std::memset(dst + P->off_flags, 0, P->total_links * sizeof(uint32_t));
+    uint32_t *flags_dev = reinterpret_cast<uint32_t *>(dev + P->off_flags);
+    WalkSegDev *sh = reinterpret_cast<WalkSegDev *>(dst + P->off_segs);
+    CarrySegDev *ch = reinterpret_cast<CarrySegDev *>(dst + P->off_carry);
+    double *bs = static_cast<double *>(P->blocksum.p);
+    // clock walks
+    std::vector<int64_t> k0(P->walk_rx.size(), 0);
+    uint64_t blocks = 0;
+    for (size_t w = 0; w < P->walk_rx.size(); ++w) {
+        RxState &R = ctx->rx[P->walk_rx[w]];
+        WalkSegDev S{};
+        S.seed = R.d.clock_seed;
+        S.sigma = R.sigma_step;
+        S.block0 = uint32_t(blocks);
+        S.local = static_cast<float *>(P->local[w].p);
+        S.toff = static_cast<double *>(P->toff[w].p);
+        S.blocksum = bs + blocks;
+        CarrySegDev C{};
+        C.base = R.base;
+        C.blocksum = bs;
+        int64_t lo, hi;
+        if (plan_window(P, P->walk_rx[w], k, lo, hi)) {
+            const int64_t b_lo = lo >> kWalkBlockShift, b_hi = hi >> kWalkBlockShift;
+            const int64_t first = std::min(R.nb_known, b_lo);
+            if (b_hi + 1 - b_lo > P->win_cap[w] || b_hi + 2 > R.base_cap)
+                throw Fail{LPK_E_STATE, "clock window beyond the plan's capacity"};
+            S.first_block = first;
+            S.win_block = b_lo;
+            S.n_blocks = uint32_t(b_hi + 1 - first);
+            k0[w] = b_lo << kWalkBlockShift;
+            if (R.nb_known <= b_hi) {
+                C.first = R.nb_known;
+                C.n = b_hi + 1 - R.nb_known;
+                C.blocksum = bs + blocks + (R.nb_known - first);
+                R.nb_known = b_hi + 1;
+            }
+            blocks += S.n_blocks;
+        }
+        sh[w] = S;
+        ch[w] = C;
+    }
+    if (blocks > P->max_blocks)
+        throw Fail{LPK_E_STATE, "clock walk of this step beyond the plan's capacity (clock rewound during the plan?)"};
+    // jobs and links
+    uint64_t tiles = 0;
+    std::vector<uint32_t> perm;
+    for (uint32_t b : P->order) {
+        const lpk_band_chunk &B = P->bands[b];
+        uint64_t i0, clen;
+        plan_chunk(P, b, k, i0, clen);
+        JobDev J = fill_job(ctx, B, i0, clen, P->form, lh + P->link0[b], perm);
+        J.tile0 = uint32_t(tiles);
+        J.n_tiles = tiles_of(i0, clen, P->tile_shift);
+        tiles += J.n_tiles;
+        J.link0 = uint32_t(P->link0[b]);
+        if (B.has_antenna) J.unocc = flags_dev;
+        if (J.walk) {
+            const int w = P->walk_slot[B.rx];
+            J.walk_base = ctx->rx[B.rx].base;
+            J.walk_toff = static_cast<const double *>(P->toff[w].p);
+            J.walk_local = static_cast<const float *>(P->local[w].p);
+            J.eps_k0 = k0[w];
+        }
+        J.out = B.out_dev ? static_cast<float *>(B.out_dev) + 2 * i0 : static_cast<float *>(P->outbuf[b].p);
+        *jh++ = J;
+    }
+    if (tiles > P->max_tiles) throw Fail{LPK_E_STATE, "step beyond the plan's tile capacity"};
+    hdr[0] = uint32_t(tiles);
+    hdr[1] = uint32_t(blocks);
+}
+
+void plan_free(lpk_plan *P) {
+    if (!P) return;
+    if (P->ctx) (void)hipStreamSynchronize(P->ctx->stream);
+    for (int i = 0; i < 2; ++i) {
+        if (P->exec[i]) (void)hipGraphExecDestroy(P->exec[i]);
+        if (P->graph[i]) (void)hipGraphDestroy(P->graph[i]);
+        if (P->done[i]) (void)hipEventDestroy(P->done[i]);
+        if (P->pin[i]) (void)hipHostFree(P->pin[i]);
+    }
+    for (auto &d : P->local) d.release();
+    for (auto &d : P->toff) d.release();
+    for (auto &d : P->outbuf) d.release();
+    P->blocksum.release();
+    P->tables.release();
+    delete P;
+}
+
+lpk_plan *plan_create(lpk_ctx *ctx, const lpk_band_chunk *bands, uint32_t n_bands, uint64_t chunk, int form) {
+    if (!ctx->have_scene) throw Fail{LPK_E_STATE, "no scene uploaded"};
+    if (form != LPK_FORM_PLAIN && form != LPK_FORM_PRECISION) throw Fail{LPK_E_INVALID, "unknown form"};
+    if (!n_bands || !bands) throw Fail{LPK_E_INVALID, "a plan needs bands"};
+    if (chunk == 0 || chunk > (uint64_t(1) << 32)) throw Fail{LPK_E_INVALID, "chunk_samples out of range"};
+    std::unique_ptr<lpk_plan, void (*)(lpk_plan *)> P(new lpk_plan, plan_free);
+    P->ctx = ctx;
+    P->form = form;
+    P->tile_shift = ctx->tile_shift;
+    P->xcd_map = ctx->xcd_map;
+    P->chunk = chunk;
+    P->bands.assign(bands, bands + n_bands);
+    P->active.resize(n_bands);
+    for (uint32_t b = 0; b < n_bands; ++b) {
+        lpk_band_chunk &B = P->bands[b];
+        validate_job(ctx, B, b);
+        if (B.i0 != 0 || B.delta_t || B.out_host || B.adc_codes_host || B.unoccluded)
+            throw Fail{LPK_E_INVALID, "band " + std::to_string(b) +
+                                          ": a plan band starts at sample 0, uses the device clock and has "
+                                          "device outputs only"};
+        P->active[b].assign(B.active, B.active + B.n_active);
+        B.active = P->active[b].data();
+        P->antenna = P->antenna || B.has_antenna;
+        P->n_chunks = std::max(P->n_chunks, (B.clen + chunk - 1) / chunk);
+    }
+    P->order.resize(n_bands);
+    for (uint32_t b = 0; b < n_bands; ++b) P->order[b] = b;
+    std::stable_sort(P->order.begin(), P->order.end(),
+                     [&](uint32_t a, uint32_t b) { return P->bands[a].n_active > P->bands[b].n_active; });
+    P->link0.assign(n_bands, 0);
+    for (uint32_t b : P->order) {
+        P->link0[b] = P->total_links;
+        P->total_links += P->bands[b].n_active;
+    }
+    // the receivers the plan walks, and a dry run of every step for capacities
+    P->walk_slot.assign(ctx->rx.size(), -1);
+    for (const auto &B : P->bands)
+        if (ctx->rx[B.rx].walk && P->walk_slot[B.rx] < 0) {
+            P->walk_slot[B.rx] = int(P->walk_rx.size());
+            P->walk_rx.push_back(B.rx);
+        }
+    const size_t nw = P->walk_rx.size();
+    std::vector<int64_t> nb(nw, 0), max_bhi(nw, -1);
+    P->win_cap.assign(nw, 0);
+    for (uint64_t k = 0; k < P->n_chunks; ++k) {
+        uint64_t blocks = 0, tiles = 0;
+        for (size_t w = 0; w < nw; ++w) {
+            int64_t lo, hi;
+            if (!plan_window(P.get(), P->walk_rx[w], k, lo, hi)) continue;
+            const int64_t b_lo = lo >> kWalkBlockShift, b_hi = hi >> kWalkBlockShift;
+            blocks += uint64_t(b_hi + 1 - std::min(nb[w], b_lo));
+            P->win_cap[w] = std::max(P->win_cap[w], b_hi + 1 - b_lo);
+            max_bhi[w] = std::max(max_bhi[w], b_hi);
+            if (nb[w] <= b_hi) nb[w] = b_hi + 1;
+        }
+        for (uint32_t b = 0; b < n_bands; ++b) {
+            uint64_t i0, clen;
+            plan_chunk(P.get(), b, k, i0, clen);
+            tiles += tiles_of(i0, clen, P->tile_shift);
+        }
+        if (blocks > 0xFFFFFFull || tiles > 0xFFFFFFFFull) throw Fail{LPK_E_INVALID, "plan step too large"};
+        P->max_blocks = std::max<uint32_t>(P->max_blocks, uint32_t(blocks));
+        P->max_tiles = std::max<uint32_t>(P->max_tiles, uint32_t(tiles));
+    }
+    // device buffers
+    P->local.resize(nw);
+    P->toff.resize(nw);
+    for (size_t w = 0; w < nw; ++w) {
+        RxState &R = ctx->rx[P->walk_rx[w]];
+        ensure_base(ctx, R, max_bhi[w]);
+        const size_t cap = size_t(std::max<int64_t>(P->win_cap[w], 1));
+        P->local[w].ensure(cap * size_t(kWalkBlock) * sizeof(float), ctx->stream);
+        P->toff[w].ensure(cap * size_t(kWalkBlock >> kWalkSubShift) * sizeof(double), ctx->stream);
+    }
+    P->blocksum.ensure(size_t(std::max<uint32_t>(P->max_blocks, 1)) * sizeof(double), ctx->stream);
+    P->outbuf.resize(n_bands);
+    for (uint32_t b = 0; b < n_bands; ++b)
+        if (!P->bands[b].out_dev) P->outbuf[b].ensure(size_t(chunk) * 8, ctx->stream);
+    auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
+    P->off_jobs = 256;
+    P->off_links = align(P->off_jobs + n_bands * sizeof(JobDev));
+    P->off_flags = align(P->off_links + std::max<uint64_t>(P->total_links, 1) * sizeof(LinkDev));
+    P->off_segs = align(P->off_flags + std::max<uint64_t>(P->total_links, 1) * sizeof(uint32_t));
+    P->off_carry = align(P->off_segs + std::max<size_t>(nw, 1) * sizeof(WalkSegDev));
+    P->bytes = align(P->off_carry + std::max<size_t>(nw, 1) * sizeof(CarrySegDev));
+    char *dev = static_cast<char *>(P->tables.ensure(P->bytes, ctx->stream));
+    hip_check(hipStreamSynchronize(ctx->stream), "plan set-up");
+    const uint32_t *live = reinterpret_cast<const uint32_t *>(dev);
+    for (int i = 0; i < 2; ++i) {
+        if (hipHostMalloc(&P->pin[i], P->bytes, 0) != hipSuccess) throw Fail{LPK_E_NOMEM, "plan staging allocation"};
+        std::memset(P->pin[i], 0, P->bytes);
+        hip_check(hipEventCreateWithFlags(&P->done[i], hipEventDisableTiming), "event create");
+        // the step's graph: table copy, clock walk, carry, render
+        hip_check(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "graph capture");
+        hipError_t e = hipMemcpyAsync(dev, P->pin[i], P->bytes, hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess && nw)
+            e = launch_walk_gen(reinterpret_cast<const WalkSegDev *>(dev + P->off_segs), int(nw), P->max_blocks,
+                                live + 1, ctx->stream);
+        if (e == hipSuccess && nw)
+            e = launch_walk_carry(reinterpret_cast<const CarrySegDev *>(dev + P->off_carry), int(nw), ctx->stream);
+        if (e == hipSuccess && P->max_tiles)
+            e = launch_render(form, P->antenna, P->tile_shift, P->xcd_map != 0,
+                              reinterpret_cast<const JobDev *>(dev + P->off_jobs), int(n_bands),
+                              reinterpret_cast<const LinkDev *>(dev + P->off_links), ctx->tx_dev, P->max_tiles, live,
+                              nullptr, nullptr, nullptr, nullptr, 0, ctx->stream);
+        hipGraph_t g = nullptr;
+        const hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
+        P->graph[i] = g;
+        hip_check(e, "graph capture");
+        hip_check(e2, "graph capture");
+        hip_check(hipGraphInstantiate(&P->exec[i], g, nullptr, nullptr, 0), "graph instantiate");
+    }
+    return P.release();
+}
+
+void plan_step(lpk_plan *P, uint64_t k) {
+    if (k >= P->n_chunks) throw Fail{LPK_E_INVALID, "chunk beyond the plan"};
+    if (k != 0 && k != P->next) throw Fail{LPK_E_STATE, "plan steps run in order; expected chunk " +
+                                                            std::to_string(P->next) + " or 0"};
+    lpk_ctx *ctx = P->ctx;
+    if (k == 0)
+        for (uint32_t r : P->walk_rx) ctx->rx[r].nb_known = 0;  // a fresh ClockStream per receiver
+    const int slot = int(k & 1);
+    if (P->used[slot]) hip_check(hipEventSynchronize(P->done[slot]), "plan staging reuse");
+    plan_fill(P, k, static_cast<char *>(P->pin[slot]));
+    timing_begin(ctx);
+    hip_check(hipGraphLaunch(P->exec[slot], ctx->stream), "graph launch");
+    timing_end(ctx);
+    hip_check(hipEventRecord(P->done[slot], ctx->stream), "event record");
+    P->used[slot] = true;
+    P->next = k + 1;
 }
 
 }  // namespace
@@ -798,6 +1123,80 @@ int lpk_copy_to_host(lpk_ctx *ctx, void *host, const void *dev, size_t bytes) {
         hip_check(hipMemcpy(host, dev, bytes, hipMemcpyDeviceToHost), "copy to host");
     });
 }
+
+int lpk_set_tuning(lpk_ctx *ctx, const lpk_tuning *tuning) {
+    if (!ctx || !tuning) return LPK_E_INVALID;
+    if (tuning->tile_shift < 7 || tuning->tile_shift > 9 || (tuning->xcd_map != 0 && tuning->xcd_map != 1))
+        return fail(ctx, Fail{LPK_E_INVALID, "tile_shift must be 7, 8 or 9 and xcd_map 0 or 1"});
+    ctx->tile_shift = tuning->tile_shift;
+    ctx->xcd_map = tuning->xcd_map;
+    return LPK_OK;
+}
+
+int lpk_get_tuning(const lpk_ctx *ctx, lpk_tuning *tuning) {
+    if (!ctx || !tuning) return LPK_E_INVALID;
+    tuning->tile_shift = ctx->tile_shift;
+    tuning->xcd_map = ctx->xcd_map;
+    return LPK_OK;
+}
+
+int lpk_autotune(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form, lpk_tuning *best,
+                 double *best_ms) {
+    if (!ctx || !best || !best_ms) return LPK_E_INVALID;
+    return guarded(ctx, [&] {
+        for (uint32_t q = 0; q < n_jobs; ++q)
+            if (jobs && (jobs[q].out_host || jobs[q].adc_codes_host || jobs[q].unoccluded))
+                throw Fail{LPK_E_INVALID, "autotune jobs keep their output in device memory"};
+        const int shifts[3] = {7, 8, 9};
+        double best_t = 1e300;
+        lpk_tuning pick{ctx->tile_shift, ctx->xcd_map};
+        for (int xcd = 1; xcd >= 0; --xcd)
+            for (int ts : shifts) {
+                ctx->tile_shift = ts;
+                ctx->xcd_map = xcd;
+                render(ctx, jobs, n_jobs, form, nullptr);  // warm-up
+                render(ctx, jobs, n_jobs, form, nullptr);
+                hip_check(hipStreamSynchronize(ctx->stream), "autotune");
+                float ms = 0.0f;
+                hip_check(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "event elapsed");
+                if (ms < best_t) {
+                    best_t = ms;
+                    pick = lpk_tuning{ts, xcd};
+                }
+            }
+        ctx->tile_shift = pick.tile_shift;
+        ctx->xcd_map = pick.xcd_map;
+        *best = pick;
+        *best_ms = best_t;
+    });
+}
+
+int lpk_plan_create(lpk_ctx *ctx, const lpk_band_chunk *bands, uint32_t n_bands, uint64_t chunk_samples, int form,
+                    lpk_plan **out) {
+    if (!ctx || !out) return LPK_E_INVALID;
+    *out = nullptr;
+    return guarded(ctx, [&] { *out = plan_create(ctx, bands, n_bands, chunk_samples, form); });
+}
+
+int lpk_plan_step(lpk_plan *plan, uint64_t chunk) {
+    if (!plan) return LPK_E_INVALID;
+    return guarded(plan->ctx, [&] { plan_step(plan, chunk); });
+}
+
+int lpk_plan_info(const lpk_plan *plan, uint64_t *n_chunks, uint32_t *max_tiles) {
+    if (!plan) return LPK_E_INVALID;
+    if (n_chunks) *n_chunks = plan->n_chunks;
+    if (max_tiles) *max_tiles = plan->max_tiles;
+    return LPK_OK;
+}
+
+int lpk_plan_output(const lpk_plan *plan, uint32_t band, void **chunk_dev) {
+    if (!plan || !chunk_dev || band >= plan->bands.size()) return LPK_E_INVALID;
+    *chunk_dev = plan->bands[band].out_dev ? plan->bands[band].out_dev : plan->outbuf[band].p;
+    return LPK_OK;
+}
+
+void lpk_plan_destroy(lpk_plan *plan) { plan_free(plan); }
 
 int lpk_device_fnv1a(lpk_ctx *ctx, const void *dev, size_t bytes, uint64_t *hash) {
     if (!ctx || !hash || (bytes && !dev)) return LPK_E_INVALID;

@@ -34,14 +34,14 @@ constexpr int kWalkBlockShift = 16;
 constexpr int64_t kWalkBlock = int64_t(1) << kWalkBlockShift;
 static_assert(kWalkThreads * kWalkPerThread == (1 << kWalkBlockShift), "walk block shape");
 
-// Render tile: one workgroup of kTile lanes, one output sample per lane; tiles
-// sit at absolute multiples of kTile in a band's sample index.
-constexpr int kTile = 256;
-constexpr int kTileShift = 8;
+// Render tile: one workgroup of 2^tile_shift lanes, one output sample per
+// lane; tiles sit at absolute multiples of the tile size in a band's sample
+// index. The tile size is a tuning choice (128, 256 or 512 samples).
+constexpr int kDefaultTileShift = 8;
 // A dispatch packet counts its grid in work-items with 32 bits, so one
-// launch covers at most 2^32 / kTile - 1 tiles; a render call is sliced into
-// launches of kMaxLaunchTiles.
-constexpr uint32_t kMaxLaunchTiles = 1u << 23;
+// launch covers at most 2^32 / 512 - 1 tiles of the largest size; a render
+// call is sliced into launches of kMaxLaunchTiles.
+constexpr uint32_t kMaxLaunchTiles = 1u << 22;
 
 // One receiver's share of a batched clock-walk launch: walk blocks
 // [first_block, first_block + n_blocks); blocks from win_block on also store

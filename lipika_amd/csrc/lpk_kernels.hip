@@ -1,7 +1,7 @@
 // lpk_kernels.hip -- gfx950 kernels of the IQ-generation backend.
 //
-// render_kernel<FORM, ANT>: one workgroup renders one tile of kTile consecutive
-// output samples (tiles sit at absolute multiples of kTile in the band's
+// render_kernel<FORM, ANT, TS>: one workgroup renders one tile of 2^TS
+// consecutive output samples (tiles sit at absolute multiples of 2^TS in the band's
 // sample index) of one (receiver, band, chunk) job. Every tile of every job of
 // a render call is one launch. Each lane owns one sample and walks the job's
 // in-band transmitters in summation order:
@@ -554,16 +554,24 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n) {
     return x * per + (x < rem ? x : rem) + (b >> 3);
 }
 
-template <int FORM, bool ANT>
-__global__ void __launch_bounds__(kTile, ANT ? 4 : (FORM ? 5 : 6)) render_kernel(const JobDev *__restrict__ jobs, int n_jobs,
-                                                       const LinkDev *__restrict__ links,
-                                                       const TxDev *__restrict__ txs,
-                                                       uint32_t tile_base, uint32_t n_tiles, TapOut tap) {
+// Launch-wide choices of a render dispatch.
+struct RenderLaunch {
+    uint32_t tile_base, n_tiles;  // this dispatch's slice of the call's tiles
+    const uint32_t *n_live;       // if set: tiles from *n_live on have no work (streaming plans)
+    uint32_t xcd_map;             // deal contiguous tile runs to each XCD
+};
+
+template <int FORM, bool ANT, int TS>
+__global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM ? 5 : 6))
+    render_kernel(const JobDev *__restrict__ jobs, int n_jobs, const LinkDev *__restrict__ links,
+                  const TxDev *__restrict__ txs, RenderLaunch lx, TapOut tap) {
+    constexpr int kTile = 1 << TS;
     __shared__ uint64_t s_noise[kTile];              // one Philox word per sample
     __shared__ double s_anc_t[3][3];                  // tu, t_local, delta_t at the anchors
     __shared__ Anchor s_anc[FORM ? kAnchorGroup : 1];
 
-    const uint32_t tile = tile_base + xcd_tile(blockIdx.x, n_tiles);
+    const uint32_t tile = lx.tile_base + (lx.xcd_map ? xcd_tile(blockIdx.x, lx.n_tiles) : blockIdx.x);
+    if (lx.n_live && tile >= *lx.n_live) return;
     int lo = 0, hi = n_jobs - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -571,7 +579,7 @@ __global__ void __launch_bounds__(kTile, ANT ? 4 : (FORM ? 5 : 6)) render_kernel
         else hi = mid - 1;
     }
     const JobDev &J = jobs[lo];
-    const uint64_t tile_start = ((J.i0 >> kTileShift) + (tile - J.tile0)) << kTileShift;
+    const uint64_t tile_start = ((J.i0 >> TS) + (tile - J.tile0)) << TS;
     const uint64_t i = tile_start + threadIdx.x;
     const bool valid = i >= J.i0 && i < J.i0 + J.clen;
     const uint64_t j = i - J.i0;
@@ -589,6 +597,7 @@ __global__ void __launch_bounds__(kTile, ANT ? 4 : (FORM ? 5 : 6)) render_kernel
     // src/pipeline.rs:918-926 (clamped to a valid sample on the tile's edge lanes)
     const uint64_t first = J.i0 > tile_start ? J.i0 : tile_start;
     const uint64_t end = (J.i0 + J.clen) < tile_start + kTile ? (J.i0 + J.clen) : tile_start + kTile;
+    if (first >= end) return;  // a tile with no sample of its job (uniform over the workgroup)
     const uint64_t ic = valid ? i : first;
     const double t_local = static_cast<double>(ic) / J.fs;
     const double dlt = J.delta_t ? J.delta_t[ic - J.i0] : clock_delta(J, t_local);
@@ -812,10 +821,12 @@ __global__ void __launch_bounds__(kTile, ANT ? 4 : (FORM ? 5 : 6)) render_kernel
 constexpr int kWalkSub = 1 << kWalkSubShift;
 constexpr int kWalkStageRow = kWalkSub + 1;  // padded: conflict-free rows and columns
 
-__global__ void __launch_bounds__(kWalkThreads) walk_gen_kernel(const WalkSegDev *__restrict__ segs, int n_segs) {
+__global__ void __launch_bounds__(kWalkThreads) walk_gen_kernel(const WalkSegDev *__restrict__ segs, int n_segs,
+                                                                const uint32_t *n_live) {
     __shared__ double lds[kWalkThreads];
     __shared__ float stage[kWalkThreads * kWalkStageRow];
     constexpr int kSubsPerThread = kWalkPerThread / kWalkSub;
+    if (n_live && blockIdx.x >= *n_live) return;  // streaming plans launch the largest grid
     int lo = 0, hi = n_segs - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -909,25 +920,38 @@ __global__ void __launch_bounds__(64) walk_carry_kernel(const CarrySegDev *__res
 
 namespace lpk {
 
-hipError_t launch_render(int form, bool antenna, const JobDev *jobs, int n_jobs, const LinkDev *links,
-                         const TxDev *txs, uint32_t n_tiles, double *tap_t_emit, double *tap_range,
-                         double *tap_phase, double *tap_delta_t, uint32_t tap_slot,
-                         hipStream_t stream) {
+template <int FORM, bool ANT>
+using RenderFn = void (*)(const JobDev *, int, const LinkDev *, const TxDev *, RenderLaunch, TapOut);
+
+template <int FORM, bool ANT>
+RenderFn<FORM, ANT> render_fn(int tile_shift) {
+    return tile_shift == 7 ? render_kernel<FORM, ANT, 7>
+                           : (tile_shift == 9 ? render_kernel<FORM, ANT, 9> : render_kernel<FORM, ANT, 8>);
+}
+
+hipError_t launch_render(int form, bool antenna, int tile_shift, bool xcd_map, const JobDev *jobs, int n_jobs,
+                         const LinkDev *links, const TxDev *txs, uint32_t n_tiles, const uint32_t *n_live,
+                         double *tap_t_emit, double *tap_range, double *tap_phase, double *tap_delta_t,
+                         uint32_t tap_slot, hipStream_t stream) {
     TapOut tap{tap_t_emit, tap_range, tap_phase, tap_delta_t, tap_slot, tap_t_emit ? 1u : 0u};
-    auto kernel = form == 0 ? (antenna ? render_kernel<0, true> : render_kernel<0, false>)
-                            : (antenna ? render_kernel<1, true> : render_kernel<1, false>);
+    if (tile_shift < 7 || tile_shift > 9) return hipErrorInvalidValue;
+    void (*kernel)(const JobDev *, int, const LinkDev *, const TxDev *, RenderLaunch, TapOut) =
+        form == 0 ? (antenna ? render_fn<0, true>(tile_shift) : render_fn<0, false>(tile_shift))
+                  : (antenna ? render_fn<1, true>(tile_shift) : render_fn<1, false>(tile_shift));
     for (uint32_t base = 0; base < n_tiles; base += kMaxLaunchTiles) {
         const uint32_t n = n_tiles - base < kMaxLaunchTiles ? n_tiles - base : kMaxLaunchTiles;
-        hipLaunchKernelGGL(kernel, dim3(n), dim3(kTile), 0, stream, jobs, n_jobs, links, txs, base, n, tap);
+        const RenderLaunch lx{base, n, n_live, xcd_map ? 1u : 0u};
+        hipLaunchKernelGGL(kernel, dim3(n), dim3(1u << tile_shift), 0, stream, jobs, n_jobs, links, txs, lx, tap);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
 }
 
-hipError_t launch_walk_gen(const WalkSegDev *segs, int n_segs, uint32_t n_blocks, hipStream_t stream) {
-    if (n_blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(walk_gen_kernel, dim3(n_blocks), dim3(kWalkThreads), 0, stream, segs, n_segs);
+hipError_t launch_walk_gen(const WalkSegDev *segs, int n_segs, uint32_t n_blocks, const uint32_t *n_live,
+                           hipStream_t stream) {
+    if (n_blocks == 0 || n_segs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(walk_gen_kernel, dim3(n_blocks), dim3(kWalkThreads), 0, stream, segs, n_segs, n_live);
     return hipGetLastError();
 }
 

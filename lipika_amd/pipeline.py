@@ -451,12 +451,70 @@ class Backend:
         self.check(self.lib.lpk_last_kernel_ms(self.h, C.byref(ms)))
         return ms.value
 
+    def set_tuning(self, tile_shift: int = 8, xcd_map: int = 1) -> None:
+        """The render launch's tile size (2^tile_shift samples) and work split."""
+        self.check(self.lib.lpk_set_tuning(self.h, C.byref(_lib.Tuning(tile_shift, xcd_map))))
+
+    def tuning(self) -> tuple:
+        t = _lib.Tuning()
+        self.check(self.lib.lpk_get_tuning(self.h, C.byref(t)))
+        return t.tile_shift, t.xcd_map
+
+    def autotune(self, jobs: Sequence, form: int = _lib.FORM_PLAIN) -> tuple:
+        """Times every candidate tuning on `jobs` (device outputs only), keeps
+        the fastest and returns ((tile_shift, xcd_map), render-kernel ms)."""
+        arr = (_lib.BandChunk * max(len(jobs), 1))(*jobs)
+        t, ms = _lib.Tuning(), C.c_double()
+        self.check(self.lib.lpk_autotune(self.h, arr, len(jobs), form, C.byref(t), C.byref(ms)))
+        return (t.tile_shift, t.xcd_map), ms.value
+
+    def plan(self, bands: Sequence, chunk_samples: int, form: int = _lib.FORM_PLAIN) -> "Plan":
+        """A streaming plan over whole-band descriptors (see Plan)."""
+        return Plan(self, bands, chunk_samples, form)
+
     def taps(self, job, slot: int, form: int = _lib.FORM_PLAIN):
         n = job.clen
         te, rg, ph, dt = (np.empty(n, dtype=np.float64) for _ in range(4))
         self.check(self.lib.lpk_render_taps(self.h, C.byref(job), form, slot, _ptr(te, C.c_double),
                                             _ptr(rg, C.c_double), _ptr(ph, C.c_double), _ptr(dt, C.c_double)))
         return te, rg, ph, dt
+
+
+class Plan:
+    """lpk_plan_*: chunk k of every band per step, replayed as one captured
+    HIP graph (generate_band's chunk loop, src/pipeline.rs:905-990, for all
+    bands at once). `bands` are whole-band descriptors from Backend.job(...,
+    i0=0, clen=n, out_dev=... or None)."""
+
+    def __init__(self, backend: "Backend", bands: Sequence, chunk_samples: int, form: int):
+        self.be = backend
+        self._arr = (_lib.BandChunk * max(len(bands), 1))(*bands)
+        self.h = C.c_void_p()
+        backend.check(backend.lib.lpk_plan_create(backend.h, self._arr, len(bands), chunk_samples, form,
+                                                  C.byref(self.h)))
+        n, t = C.c_uint64(), C.c_uint32()
+        backend.check(backend.lib.lpk_plan_info(self.h, C.byref(n), C.byref(t)))
+        self.n_chunks, self.max_tiles = n.value, t.value
+
+    def step(self, k: int) -> None:
+        self.be.check(self.be.lib.lpk_plan_step(self.h, k))
+
+    def output(self, band: int) -> int:
+        """Device address of band `band`'s output for the last step."""
+        p = C.c_void_p()
+        self.be.check(self.be.lib.lpk_plan_output(self.h, band, C.byref(p)))
+        return p.value
+
+    def close(self) -> None:
+        if self.h:
+            self.be.lib.lpk_plan_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 _FORMS = {"plain": _lib.FORM_PLAIN, "precision": _lib.FORM_PRECISION}

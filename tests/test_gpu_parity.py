@@ -407,10 +407,97 @@ def test_full_config2_scale_properties(backend):
     assert hashes[0] == hashes[1] == hashes[2]
 
 
+def _units(sc):
+    return [(r, b) for r, rx in enumerate(sc.receivers) for b, _ in enumerate(rx.bands)]
+
+
+@pytest.mark.parametrize("form", [_lib.FORM_PLAIN, _lib.FORM_PRECISION])
+def test_streaming_plan_replays_match_eager_chunks(backend, form):
+    """Every replay of a plan's captured step graph equals an eager render of
+    the same chunk of every band (clock walk and kTB noise on), with eager
+    renders and synchronisations between the replays; restarting at chunk 0
+    replays the same bytes again."""
+    seed = 21
+    sc = prepare(workloads.scene2(duration_s=0.2))
+    backend.upload(sc, seed)
+    lib = _lib.load()
+    chunk = 100_000
+    units = _units(sc)
+    bands, keep = [], []
+    for w, (r, b) in enumerate(units):
+        j, k = backend.job(r, b, w, seed, 0, sc.receivers[r].bands[b].n)
+        bands.append(j)
+        keep.append(k)
+    checked = 0
+    with backend.plan(bands, chunk, form) as plan:
+        _say(f"plan: {plan.n_chunks} chunks of {chunk}, at most {plan.max_tiles} tiles per step")
+        for rnd, ks in enumerate((range(plan.n_chunks), range(2))):
+            for kk in ks:
+                plan.step(kk)
+                backend.sync()
+                got, jobs, want = [], [], []
+                for w, (r, b) in enumerate(units):
+                    n, i0 = sc.receivers[r].bands[b].n, kk * chunk
+                    if i0 >= n:
+                        continue
+                    clen = min(chunk, n - i0)
+                    g = np.empty(clen, dtype=np.complex64)
+                    backend.check(lib.lpk_copy_to_host(backend.h, g.ctypes.data, plan.output(w), clen * 8))
+                    o = np.empty(clen, dtype=np.complex64)
+                    j, k = backend.job(r, b, w, seed, i0, clen, out_host=o)
+                    keep.append(k)
+                    jobs.append(j)
+                    got.append(g)
+                    want.append(o)
+                backend.render(jobs, form)  # the eager render of the same chunk, between replays
+                backend.sync()
+                for g, o in zip(got, want):
+                    assert g.tobytes() == o.tobytes(), (rnd, kk)
+                    checked += 1
+    _say(f"plan replays: {checked} band-chunks bit-identical to eager renders")
+
+
+def test_tuning_and_autotune(backend):
+    """The plain form's samples do not depend on the tile size or the XCD
+    mapping; the precision form's stay within its bound of the plain form's.
+    The autotuner keeps one of its candidates on the context."""
+    seed = 5
+    sc = prepare(util.all_kinds_scene(duration_s=0.004))
+    backend.upload(sc, seed)
+    try:
+        backend.set_tuning(8, 1)
+        ref, _ = gpu_band(backend, sc, 0, 0, 0, seed)
+        peak = float(np.max(np.abs(ref)))
+        for ts in (7, 8, 9):
+            for xcd in (0, 1):
+                backend.set_tuning(ts, xcd)
+                assert backend.tuning() == (ts, xcd)
+                g, _ = gpu_band(backend, sc, 0, 0, 0, seed)
+                assert g.tobytes() == ref.tobytes(), (ts, xcd)
+                p, _ = gpu_band(backend, sc, 0, 0, 0, seed, form=_lib.FORM_PRECISION)
+                d = np.max(np.abs(p.astype(np.complex128) - ref.astype(np.complex128)))
+                _say(f"tile 2^{ts}, xcd map {xcd}: precision max|d|/peak = {d / peak:.3e}")
+                assert d <= 2e-6 * peak
+        lib = _lib.load()
+        n = sc.receivers[0].bands[0].n
+        dev = C.c_void_p()
+        backend.check(lib.lpk_device_alloc(backend.h, n * 8, C.byref(dev)))
+        try:
+            j, keep = backend.job(0, 0, 0, seed, 0, n, out_dev=dev.value)
+            best, ms = backend.autotune([j], _lib.FORM_PRECISION)
+            _say(f"autotune: tile 2^{best[0]}, xcd map {best[1]}, {ms:.3f} ms")
+            assert best[0] in (7, 8, 9) and best[1] in (0, 1) and ms > 0.0
+            assert backend.tuning() == best
+        finally:
+            backend.check(lib.lpk_device_free(backend.h, dev))
+    finally:
+        backend.set_tuning(8, 1)
+
+
 @pytest.mark.parametrize("form", [_lib.FORM_PLAIN, _lib.FORM_PRECISION])
 def test_render_call_larger_than_one_launch(backend, form):
-    """One render call of 2.3e9 samples (8.98 M tiles, more than one dispatch
-    can hold): chunks at the start, across the first slice boundary and at the
+    """One render call of 2.3e9 samples (8.98 M tiles of 256, three
+    dispatches): chunks at the start, across both slice boundaries and at the
     very end equal the same chunks rendered on their own."""
     import ctypes as C
     from lipika_amd.config import ReceiverBandConfig, SceneConfig, SceneMeta, StationaryEcef, TransmitterConfig, Cw
@@ -434,8 +521,9 @@ def test_render_call_larger_than_one_launch(backend, form):
         _say(f"one call: {len(jobs)} jobs, {n} samples, {(n + 255) // 256} tiles")
         backend.render(jobs, form)
         backend.sync()
-        boundary = ((1 << 23) * 256) // chunk
-        for q in (0, boundary - 1, boundary, boundary + 1, len(jobs) - 1):
+        # dispatches hold at most 2^22 tiles: check the chunks around each slice edge
+        edges = [(m * (1 << 22) * 256) // chunk for m in (1, 2)]
+        for q in sorted({0, len(jobs) - 1, *[e + d for e in edges for d in (-1, 0, 1)]}):
             i0 = q * chunk
             clen = min(chunk, n - i0)
             got = np.empty(clen, dtype=np.complex64)
