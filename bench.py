@@ -46,21 +46,13 @@ def log(*a):
 
 
 def plan_leg(be, lib, sc, bufs, seed, form, steps, warmup, chunk=1 << 20):
-    """The batching form: the autotuner picks the tile size and work split on
-    chunk 0 of every band, then a streaming plan renders chunk k of every band
-    per step as one graph replay; a pass is every step of the scene."""
+    """The batching form: a streaming plan renders chunk k of every band per
+    step as one graph replay, with the context's (autotuned) tuning; a pass is
+    every step of the scene."""
     from lipika_amd.pipeline import form_code
     units = [(r, b) for r, rx in enumerate(sc.receivers) for b, _ in enumerate(rx.bands)]
-    first, keep = [], []
-    for w, (r, b) in enumerate(units):
-        n = sc.receivers[r].bands[b].n
-        j, k = be.job(r, b, w, seed, 0, min(chunk, n), out_dev=bufs[w].value)
-        first.append(j)
-        keep.append(k)
-    tuning, tune_ms = be.autotune(first, form_code(form))
-    log(f"[bench] autotune on chunk 0 of every band ({form}): tile 2^{tuning[0]}, xcd map {tuning[1]}, "
-        f"{tune_ms:.3f} ms")
-    bands = []
+    tuning = be.tuning()
+    bands, keep = [], []
     for w, (r, b) in enumerate(units):
         j, k = be.job(r, b, w, seed, 0, sc.receivers[r].bands[b].n, out_dev=bufs[w].value)
         bands.append(j)
@@ -113,6 +105,8 @@ def gpu_leg(args, scene_n, forms, steps, warmup, batching=False):
             w += 1
     log(f"[bench] config {scene_n}: {len(jobs)} chunk jobs, {out_samples} output samples, "
         f"{contributions} contributions per step")
+    # the autotuner's sample: chunk 0 of every band
+    first = [jobs[i] for i in range(len(jobs)) if jobs[i].i0 == 0]
 
     def step(form):
         # a step is a whole fresh render: the clock walk is regenerated from index 0
@@ -122,6 +116,9 @@ def gpu_leg(args, scene_n, forms, steps, warmup, batching=False):
     results = {}
     for name in forms:
         form = form_code(name)
+        tuning, tune_ms = be.autotune(first, form)
+        log(f"[bench] config {scene_n} form {name}: autotune on chunk 0 of every band picks tile 2^{tuning[0]}, "
+            f"xcd map {tuning[1]} ({tune_ms:.3f} ms)")
         for s in range(warmup):
             step(form)
         be.sync()
@@ -137,7 +134,7 @@ def gpu_leg(args, scene_n, forms, steps, warmup, batching=False):
         kernel_ms = kern_ms.value / max(n_launch.value, 1)
         log(f"[bench] config {scene_n} form {name}: {steps} timed steps after {warmup} warmup: {ms_per_step:.3f} ms/step; "
             f"render kernels {kernel_ms:.3f} ms per render call over {n_launch.value} calls")
-        results[name] = (ms_per_step, kernel_ms)
+        results[name] = (ms_per_step, kernel_ms, tuning)
     plan = plan_leg(be, lib, sc, bufs, seed, "precision", steps, warmup) if batching else None
     for d in bufs:
         be.check(lib.lpk_device_free(be.h, d))
@@ -207,13 +204,14 @@ def main():
         sys.exit(2)
     forms = ["plain", "precision"] if args.form == "precision" else ["plain"]
     sc, out_samples, contributions, results, _ = gpu_leg(args, args.config, forms, args.steps, args.warmup)
-    ms, kernel_ms = results[args.form]
+    ms, kernel_ms, tuned = results[args.form]
     value = contributions / (ms / 1e3) / 1e6
     achieved = contributions * FLOPS_PER_CONTRIBUTION / (kernel_ms / 1e3) / 1e12
 
     def summary(results, contributions):
         out = {n: {"value": round(contributions / (m / 1e3) / 1e6, 3), "ms_per_step": round(m, 4),
-                   "kernel_ms": round(k, 4)} for n, (m, k) in results.items()}
+                   "kernel_ms": round(k, 4), "tile_shift": t[0], "xcd_map": t[1]}
+               for n, (m, k, t) in results.items()}
         if "plain" in results and "precision" in results:
             out["precision"]["speedup_vs_plain"] = round(results["plain"][0] / results["precision"][0], 3)
         return out
@@ -233,7 +231,7 @@ def main():
                      "traffic": TRAFFIC_CONFIG2[args.form] if args.config == 2 else None,
                      "traffic_source": "profiles/render_config2_hbm_traffic.md",
                      "compute_unit": "FP64 vector ALU (the path issues no matrix ops)",
-                     "kernel": f"render_kernel<{0 if args.form == 'plain' else 1}, false>",
+                     "kernel": f"render_kernel<{0 if args.form == 'plain' else 1}, false, {tuned[0]}>",
                      "kernel_ms": round(kernel_ms, 4),
                      "flops_per_contribution": FLOPS_PER_CONTRIBUTION,
                      "output_gb_s": round(out_samples * 8 / (kernel_ms / 1e3) / 1e9, 2)},
