@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define LPK_ABI_VERSION 2
+#define LPK_ABI_VERSION 3
 
 typedef enum {
     LPK_OK = 0,
@@ -69,7 +69,13 @@ typedef enum { LPK_POST_GAIN = 0, LPK_POST_NONLINEARITY = 1, LPK_POST_ADC = 2 } 
 /* Arithmetic form of a render. */
 typedef enum {
     LPK_FORM_PLAIN = 0,      /* the reference's f64 path, operation for operation */
-    LPK_FORM_PRECISION = 1   /* f64 geometry and phase, f32 signal path, compensated sum */
+    LPK_FORM_PRECISION = 1,  /* f64 geometry and phase, f32 signal path, compensated sum */
+    LPK_FORM_CONVERGED = 2,  /* the plain form with the emission-time solve (src/channel/mod.rs:70-99)
+                              * iterated on the unrounded time axis until an update is below
+                              * 1e-13 s, from the reference's start t - 0.07 s */
+    LPK_FORM_LEARNED = 3     /* LPK_FORM_CONVERGED started from a warm-start network's predicted
+                              * light time (lpk_set_warm_start); a start that has not converged
+                              * after 3 evaluations falls back to the reference's start */
 } lpk_form;
 
 /* One transmitter, as Transmitter::from_config leaves it (src/pipeline.rs:297-341). */
@@ -111,6 +117,10 @@ typedef struct {
     double gate_period_s;
     double gate_phase_s;         /* in [0, period) */
     const double *bursts;        /* n_bursts (start, end) pairs, end exclusive */
+    /* the orbit the grid was sampled from, CircularKeplerian (src/geometry/kepler.rs:19-25);
+     * only the learned form reads it (its network's features). has_kepler = 0: none */
+    int32_t has_kepler, _pad2;
+    double kepler_a_m, kepler_inc_deg, kepler_raan_deg, kepler_nu0_deg, kepler_epoch_unix;
 } lpk_tx_desc;
 
 /* One stationary receiver with its clock (src/pipeline.rs:344-364,
@@ -162,6 +172,7 @@ typedef struct {
 } lpk_band_chunk;
 
 typedef struct lpk_ctx lpk_ctx;
+typedef struct lpk_lws lpk_lws;
 
 /* ---- context (the Backend slot, src/backend/mod.rs:14-18) ---- */
 int lpk_abi_version(void);
@@ -281,6 +292,55 @@ int lpk_cw_anchor_phase(double render_epoch, double signal_epoch, double offset_
 int lpk_bands_overlap(double tx_carrier_hz, double tx_bw_hz, double band_lo_hz, double band_bw_hz);
 /* src/channel/mod.rs:310-312 */
 double lpk_thermal_noise_power_w(double temperature_k, double bandwidth_hz);
+
+
+/* ---- learned warm start (no counterpart in the reference: its solve always
+ * starts at t - 0.07 s, src/channel/mod.rs:75-96) ----
+ * A BF16 multilayer perceptron, 10 inputs -> 3 x 64 GELU -> 1, FP32
+ * accumulation on the matrix cores, predicting the light time tau of the
+ * emission equation as tau_sl + 1e-6 s * net(features), tau_sl = |P(t) - O| / c.
+ * Features: O / 6378137 m (3), (a - 1.7e7 m) / 1e7 m, cos i, sin i, cos and sin
+ * of the Earth-fixed node raan - w_E (t - t_el), cos and sin of the argument of
+ * latitude nu0 + n (t - t_el). Parameters (f32, LPK_LWS_PARAMS), W[out][in]:
+ *   W1[64][10] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] w4[64] b4.
+ * A network belongs to the context it was created on; destroy it before the
+ * context. */
+#define LPK_LWS_PARAMS 9089
+typedef struct {
+    uint64_t n_pairs;            /* training pairs generated on the device */
+    uint64_t data_seed;
+    uint32_t epochs;
+    uint32_t batch;              /* minibatch, a multiple of 64, <= 2^20 */
+    float lr, lr_final;          /* Adam step size, cosine-decayed from lr to lr_final */
+    uint64_t n_val;              /* validation pairs, drawn with an independent seed */
+} lpk_lws_train_cfg;
+typedef struct {
+    double loss_first, loss_last;   /* mean squared error (network units) over the first / last epoch */
+    double val_rms_s, val_max_s;    /* validation error of the predicted light time, seconds */
+    double gen_ms, train_ms;        /* device time of pair generation and of the training steps */
+    uint64_t steps;
+} lpk_lws_train_report;
+int lpk_lws_create(lpk_ctx *ctx, uint64_t init_seed, lpk_lws **out);   /* Glorot-uniform init */
+void lpk_lws_destroy(lpk_lws *net);
+int lpk_lws_get_params(lpk_lws *net, float *theta);        /* host [LPK_LWS_PARAMS] */
+int lpk_lws_set_params(lpk_lws *net, const float *theta);  /* also resets the optimiser */
+/* Training pairs: x_dev [n][10] features, y_dev [n] (tau - tau_sl) / 1e-6 s, and
+ * optionally geom_dev [n][8] = receiver ECEF xyz, a, i, node, u (rad), tau (s).
+ * Geometry: receiver lat U[-90, 90] deg, lon U[-180, 180) deg, h U[0, 2000] m; orbit
+ * LEO a U[6.9e6, 7.5e6] m or MEO a U[2.6e7, 2.7e7] m, i U[0, 180) deg, node and u U[0, 360) deg.
+ * tau is the converged root of the emission equation on the analytic orbit. Asynchronous. */
+int lpk_lws_generate(lpk_ctx *ctx, uint64_t n, uint64_t seed, float *x_dev, float *y_dev, double *geom_dev);
+int lpk_lws_infer(lpk_lws *net, const float *x_dev, uint64_t n, float *y_dev);   /* asynchronous */
+/* Gradient of the mean squared error over the n pairs (synchronous; loss may be NULL). */
+int lpk_lws_gradient(lpk_lws *net, const float *x_dev, const float *y_dev, uint64_t n, float *grad, double *loss);
+/* Generates the pairs and trains with Adam (beta 0.9 / 0.999, eps 1e-8); synchronous. */
+int lpk_lws_train(lpk_lws *net, const lpk_lws_train_cfg *cfg, lpk_lws_train_report *rep);
+/* The network LPK_FORM_LEARNED predicts with (NULL: none). */
+int lpk_set_warm_start(lpk_ctx *ctx, lpk_lws *net);
+/* Converged-form counters since the last reset: emission-equation evaluations,
+ * contributions, learned starts that fell back, solves that did not converge. */
+int lpk_solve_stats(lpk_ctx *ctx, uint64_t stats[4]);
+int lpk_solve_stats_reset(lpk_ctx *ctx);
 
 #ifdef __cplusplus
 }

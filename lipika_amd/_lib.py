@@ -12,7 +12,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblpk.so")
 
-ABI_VERSION = 2  # LPK_ABI_VERSION of the header these structures mirror
+ABI_VERSION = 3  # LPK_ABI_VERSION of the header these structures mirror
 LPK_OK = 0
 STATUS = {
     0: "LPK_OK", -1: "LPK_E_INVALID", -2: "LPK_E_NOMEM", -3: "LPK_E_HIP",
@@ -22,7 +22,8 @@ SIG_CW, SIG_BPSK, SIG_OOK, SIG_QAM16, SIG_CA = range(5)
 NAV_OFF, NAV_DUMMY, NAV_FULL = range(3)
 TRAJ_STATIONARY, TRAJ_GRID = range(2)
 POST_GAIN, POST_NONLINEARITY, POST_ADC = range(3)
-FORM_PLAIN, FORM_PRECISION = 0, 1
+FORM_PLAIN, FORM_PRECISION, FORM_CONVERGED, FORM_LEARNED = 0, 1, 2, 3
+LWS_PARAMS = 9089
 
 _d = C.c_double
 _u64 = C.c_uint64
@@ -50,6 +51,9 @@ class TxDesc(C.Structure):
         ("gated", _i32), ("n_bursts", _u32),
         ("gate_period_s", _d), ("gate_phase_s", _d),
         ("bursts", C.POINTER(_d)),
+        ("has_kepler", _i32), ("_pad2", _i32),
+        ("kepler_a_m", _d), ("kepler_inc_deg", _d), ("kepler_raan_deg", _d), ("kepler_nu0_deg", _d),
+        ("kepler_epoch_unix", _d),
     ]
 
 
@@ -86,6 +90,16 @@ class BandChunk(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("tile_shift", C.c_int32), ("xcd_map", C.c_int32)]
+
+
+class LwsTrainCfg(C.Structure):
+    _fields_ = [("n_pairs", _u64), ("data_seed", _u64), ("epochs", _u32), ("batch", _u32),
+                ("lr", C.c_float), ("lr_final", C.c_float), ("n_val", _u64)]
+
+
+class LwsTrainReport(C.Structure):
+    _fields_ = [("loss_first", _d), ("loss_last", _d), ("val_rms_s", _d), ("val_max_s", _d),
+                ("gen_ms", _d), ("train_ms", _d), ("steps", _u64)]
 
 
 class LpkError(RuntimeError):
@@ -146,6 +160,17 @@ def load() -> C.CDLL:
         "lpk_cw_anchor_phase": (_i32, [_d, _d, _d, P(_d)]),
         "lpk_bands_overlap": (_i32, [_d, _d, _d, _d]),
         "lpk_thermal_noise_power_w": (_d, [_d, _d]),
+        "lpk_lws_create": (_i32, [C.c_void_p, _u64, P(C.c_void_p)]),
+        "lpk_lws_destroy": (None, [C.c_void_p]),
+        "lpk_lws_get_params": (_i32, [C.c_void_p, P(C.c_float)]),
+        "lpk_lws_set_params": (_i32, [C.c_void_p, P(C.c_float)]),
+        "lpk_lws_generate": (_i32, [C.c_void_p, _u64, _u64, C.c_void_p, C.c_void_p, C.c_void_p]),
+        "lpk_lws_infer": (_i32, [C.c_void_p, C.c_void_p, _u64, C.c_void_p]),
+        "lpk_lws_gradient": (_i32, [C.c_void_p, C.c_void_p, C.c_void_p, _u64, P(C.c_float), P(_d)]),
+        "lpk_lws_train": (_i32, [C.c_void_p, P(LwsTrainCfg), P(LwsTrainReport)]),
+        "lpk_set_warm_start": (_i32, [C.c_void_p, C.c_void_p]),
+        "lpk_solve_stats": (_i32, [C.c_void_p, P(_u64)]),
+        "lpk_solve_stats_reset": (_i32, [C.c_void_p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -166,6 +191,9 @@ EXPORTED = (
     "lpk_bands_overlap", "lpk_thermal_noise_power_w",
     "lpk_set_tuning", "lpk_get_tuning", "lpk_autotune",
     "lpk_plan_create", "lpk_plan_step", "lpk_plan_info", "lpk_plan_output", "lpk_plan_destroy",
+    "lpk_lws_create", "lpk_lws_destroy", "lpk_lws_get_params", "lpk_lws_set_params", "lpk_lws_generate",
+    "lpk_lws_infer", "lpk_lws_gradient", "lpk_lws_train", "lpk_set_warm_start", "lpk_solve_stats",
+    "lpk_solve_stats_reset",
 )
 
 

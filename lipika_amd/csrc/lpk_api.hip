@@ -13,12 +13,24 @@
 
 #include "../../include/lpk.h"
 #include "lpk_device.hpp"
+#include "lpk_lws.hpp"
 
 namespace lpk {
 hipError_t launch_render(int form, bool antenna, int tile_shift, bool xcd_map, const JobDev *jobs, int n_jobs,
                          const LinkDev *links, const TxDev *txs, uint32_t n_tiles, const uint32_t *n_live,
                          double *tap_t_emit, double *tap_range, double *tap_phase, double *tap_delta_t,
-                         uint32_t tap_slot, hipStream_t stream);
+                         uint32_t tap_slot, const double *pred, unsigned long long *stats, hipStream_t stream);
+hipError_t launch_lws_predict(const JobDev *jobs, int n_jobs, const LinkDev *links, const TxDev *txs,
+                              const LwsNetDev &net, int tile_shift, uint64_t n_queries, double *pred,
+                              hipStream_t stream);
+hipError_t launch_lws_gen(uint64_t n, uint64_t seed, float *X, float *Y, double *G, hipStream_t s);
+hipError_t launch_lws_infer(const LwsNetDev &net, const float *X, uint64_t n, float *Y, hipStream_t s);
+hipError_t launch_lws_train(const LwsNetDev &net, const float *X, const float *Y, uint64_t base, uint64_t end,
+                            uint32_t n_slices, float scale, float *partials, hipStream_t s);
+hipError_t launch_lws_update(const float *partials, int n_rows, float *theta, float *m1, float *m2, void *packed,
+                             float lr, float beta1, float beta2, float eps, float c1, float c2, float *grad_out,
+                             float *loss_out, hipStream_t s);
+hipError_t launch_lws_pack(const float *theta, void *packed, hipStream_t s);
 hipError_t launch_walk_gen(const WalkSegDev *segs, int n_segs, uint32_t n_blocks, const uint32_t *n_live,
                            hipStream_t stream);
 hipError_t launch_walk_carry(const CarrySegDev *segs, int n_segs, hipStream_t stream);
@@ -103,6 +115,11 @@ struct lpk_ctx {
     int xcd_map = 1;
     // per-call buffers
     DevBuf jobs, links, dt_stage, out_stage, codes_stage, blocksum, walk_tables;
+    // converged and learned forms: the network the learned form predicts
+    // with, its prediction table and the solve counters
+    lpk_lws *lws = nullptr;
+    DevBuf pred;
+    unsigned long long *stats = nullptr;
     PinSlot pin[4];
     int pin_next = 0;
 };
@@ -135,6 +152,16 @@ struct lpk_plan {
     bool used[2] = {false, false};
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t exec[2] = {nullptr, nullptr};
+};
+
+// A warm-start network on its context's device: FP32 master parameters, the
+// Adam moments and the BF16 operand copies the matrix cores read.
+struct lpk_lws {
+    lpk_ctx *ctx = nullptr;
+    float *theta = nullptr, *m1 = nullptr, *m2 = nullptr;   // kLwsParams each
+    void *packed = nullptr;                                  // kLwsPacked BF16
+    uint64_t step = 0;                                       // Adam steps taken
+    DevBuf partials, grad, loss;
 };
 
 namespace {
@@ -190,6 +217,19 @@ void free_scene(lpk_ctx *ctx) {
 }
 
 bool is_fin(double x) { return std::isfinite(x); }
+
+LwsNetDev lws_net(const lpk_lws *net) {
+    return LwsNetDev{static_cast<const bf16x8 *>(net->packed), net->theta};
+}
+
+// The four solve counters of the converged forms (zeroed on first use).
+void ensure_stats(lpk_ctx *ctx) {
+    if (ctx->stats) return;
+    void *p = nullptr;
+    if (hipMalloc(&p, 4 * sizeof(unsigned long long)) != hipSuccess) throw Fail{LPK_E_NOMEM, "solve counters"};
+    ctx->stats = static_cast<unsigned long long *>(p);
+    hip_check(hipMemsetAsync(p, 0, 4 * sizeof(unsigned long long), ctx->stream), "solve counter reset");
+}
 
 TxDev make_tx(lpk_ctx *ctx, const lpk_tx_desc &d, uint32_t idx) {
     auto bad = [&](const char *why) {
@@ -272,6 +312,21 @@ TxDev make_tx(lpk_ctx *ctx, const lpk_tx_desc &d, uint32_t idx) {
         T.gate_period = d.gate_period_s;
         T.gate_phase = d.gate_phase_s;
         T.bursts = static_cast<const double *>(upload(ctx, d.bursts, 2 * d.n_bursts * sizeof(double)));
+    }
+    if (d.has_kepler) {
+        // CircularKeplerian (src/geometry/kepler.rs:19-36): radians and the mean motion
+        if (!(d.kepler_a_m > 0.0) || !is_fin(d.kepler_a_m) || !is_fin(d.kepler_inc_deg) ||
+            !is_fin(d.kepler_raan_deg) || !is_fin(d.kepler_nu0_deg) || !is_fin(d.kepler_epoch_unix))
+            throw Fail{LPK_E_NONFINITE, "transmitter " + std::to_string(idx) + ": orbital elements"};
+        const double rad = 3.141592653589793 / 180.0;
+        T.kep_on = 1;
+        T.kep_a = d.kepler_a_m;
+        T.kep_cos_i = std::cos(d.kepler_inc_deg * rad);
+        T.kep_sin_i = std::sin(d.kepler_inc_deg * rad);
+        T.kep_raan = d.kepler_raan_deg * rad;
+        T.kep_nu0 = d.kepler_nu0_deg * rad;
+        T.kep_n = std::sqrt(kGmEarth / (d.kepler_a_m * d.kepler_a_m * d.kepler_a_m));
+        T.kep_epoch = d.kepler_epoch_unix;
     }
     return T;
 }
@@ -530,7 +585,8 @@ void timing_end(lpk_ctx *ctx) {
 
 void render(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form, const TapReq *tap) {
     if (!ctx->have_scene) throw Fail{LPK_E_STATE, "no scene uploaded"};
-    if (form != LPK_FORM_PLAIN && form != LPK_FORM_PRECISION) throw Fail{LPK_E_INVALID, "unknown form"};
+    if (form < LPK_FORM_PLAIN || form > LPK_FORM_LEARNED) throw Fail{LPK_E_INVALID, "unknown form"};
+    if (form == LPK_FORM_LEARNED && !ctx->lws) throw Fail{LPK_E_STATE, "the learned form needs a network (lpk_set_warm_start)"};
     if (n_jobs && !jobs) throw Fail{LPK_E_INVALID, "jobs is NULL"};
     // ---- validation
     uint64_t total_links = 0, dt_total = 0, stage_total = 0, code_total = 0;
@@ -590,7 +646,7 @@ void render(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form,
     };
     std::vector<FlagJob> flag_jobs;
 
-    uint64_t tiles = 0, link_off = 0, dt_off = 0, st_off = 0, cd_off = 0;
+    uint64_t tiles = 0, link_off = 0, dt_off = 0, st_off = 0, cd_off = 0, n_pred = 0;
     uint32_t tap_slot = tap ? tap->slot : 0;
     std::vector<uint32_t> perm;
     std::vector<float *> out_ptr(n_jobs, nullptr);
@@ -605,6 +661,8 @@ void render(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form,
         tiles += J.n_tiles;
         if (tiles > 0xFFFFFFFFull) throw Fail{LPK_E_INVALID, "render call holds more than 2^32 tiles"};
         J.link0 = uint32_t(link_off);
+        J.pred0 = n_pred;
+        if (form == LPK_FORM_LEARNED && J.n_tiles) n_pred += uint64_t(B.n_active) * (uint64_t(J.n_tiles) + 1);
         if (tap)
             for (uint32_t s = 0; s < B.n_active; ++s)
                 if (perm[s] == tap->slot) tap_slot = s;
@@ -643,10 +701,18 @@ void render(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form,
     pinned_mark(ctx, slot);
 
     if (tiles) {
+        const bool converged = form == LPK_FORM_CONVERGED || form == LPK_FORM_LEARNED;
+        if (converged) ensure_stats(ctx);
+        double *pred = n_pred ? static_cast<double *>(ctx->pred.ensure(n_pred * 8, ctx->stream)) : nullptr;
         timing_begin(ctx);
-        hip_check(launch_render(form, any_antenna, ts, ctx->xcd_map != 0, jd, int(n_jobs), ld, ctx->tx_dev,
-                                uint32_t(tiles), nullptr, tap ? tap->t_emit : nullptr, tap ? tap->range : nullptr,
-                                tap ? tap->phase : nullptr, tap ? tap->delta_t : nullptr, tap_slot, ctx->stream),
+        if (pred)
+            hip_check(launch_lws_predict(jd, int(n_jobs), ld, ctx->tx_dev, lws_net(ctx->lws), ts, n_pred, pred,
+                                         ctx->stream),
+                      "warm-start prediction launch");
+        hip_check(launch_render(converged ? 2 : form, any_antenna, ts, ctx->xcd_map != 0, jd, int(n_jobs), ld,
+                                ctx->tx_dev, uint32_t(tiles), nullptr, tap ? tap->t_emit : nullptr,
+                                tap ? tap->range : nullptr, tap ? tap->phase : nullptr, tap ? tap->delta_t : nullptr,
+                                tap_slot, pred, converged ? ctx->stats : nullptr, ctx->stream),
                   "render launch");
         timing_end(ctx);
     }
@@ -798,7 +864,8 @@ void plan_free(lpk_plan *P) {
 
 lpk_plan *plan_create(lpk_ctx *ctx, const lpk_band_chunk *bands, uint32_t n_bands, uint64_t chunk, int form) {
     if (!ctx->have_scene) throw Fail{LPK_E_STATE, "no scene uploaded"};
-    if (form != LPK_FORM_PLAIN && form != LPK_FORM_PRECISION) throw Fail{LPK_E_INVALID, "unknown form"};
+    if (form != LPK_FORM_PLAIN && form != LPK_FORM_PRECISION)
+        throw Fail{LPK_E_INVALID, "plans take the plain or the precision form"};
     if (!n_bands || !bands) throw Fail{LPK_E_INVALID, "a plan needs bands"};
     if (chunk == 0 || chunk > (uint64_t(1) << 32)) throw Fail{LPK_E_INVALID, "chunk_samples out of range"};
     std::unique_ptr<lpk_plan, void (*)(lpk_plan *)> P(new lpk_plan, plan_free);
@@ -900,7 +967,7 @@ lpk_plan *plan_create(lpk_ctx *ctx, const lpk_band_chunk *bands, uint32_t n_band
             e = launch_render(form, P->antenna, P->tile_shift, P->xcd_map != 0,
                               reinterpret_cast<const JobDev *>(dev + P->off_jobs), int(n_bands),
                               reinterpret_cast<const LinkDev *>(dev + P->off_links), ctx->tx_dev, P->max_tiles, live,
-                              nullptr, nullptr, nullptr, nullptr, 0, ctx->stream);
+                              nullptr, nullptr, nullptr, nullptr, 0, nullptr, nullptr, ctx->stream);
         hipGraph_t g = nullptr;
         const hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
         P->graph[i] = g;
@@ -969,6 +1036,8 @@ void lpk_ctx_destroy(lpk_ctx *ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     free_scene(ctx);
+    ctx->pred.release();
+    if (ctx->stats) (void)hipFree(ctx->stats);
     ctx->jobs.release();
     ctx->links.release();
     ctx->dt_stage.release();
@@ -1197,6 +1266,244 @@ int lpk_plan_output(const lpk_plan *plan, uint32_t band, void **chunk_dev) {
 }
 
 void lpk_plan_destroy(lpk_plan *plan) { plan_free(plan); }
+
+// ---------------------------------------------------------------- learned warm start
+
+int lpk_lws_create(lpk_ctx *ctx, uint64_t init_seed, lpk_lws **out) {
+    if (!ctx || !out) return LPK_E_INVALID;
+    *out = nullptr;
+    return guarded(ctx, [&] {
+        // Glorot-uniform weights from SplitMix64, zero biases
+        std::vector<float> th(kLwsParams, 0.0f);
+        uint64_t x = init_seed;
+        auto next = [&]() {
+            x += 0x9E3779B97F4A7C15ULL;
+            uint64_t z = x;
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+            z ^= z >> 31;
+            return (double(z >> 11) * 0x1.0p-53) * 2.0 - 1.0;
+        };
+        auto fill = [&](int off, int n, int fan_in, int fan_out) {
+            const double lim = std::sqrt(6.0 / double(fan_in + fan_out));
+            for (int e = 0; e < n; ++e) th[off + e] = float(lim * next());
+        };
+        fill(kOffW1, 64 * kLwsIn, kLwsIn, 64);
+        fill(kOffW2, 64 * 64, 64, 64);
+        fill(kOffW3, 64 * 64, 64, 64);
+        fill(kOffW4, 64, 64, 1);
+        auto net = std::unique_ptr<lpk_lws>(new lpk_lws());
+        net->ctx = ctx;
+        void *p[4] = {nullptr, nullptr, nullptr, nullptr};
+        const size_t pb = kLwsParams * sizeof(float);
+        const size_t sizes[4] = {pb, pb, pb, kLwsPacked * 2};
+        for (int i = 0; i < 4; ++i)
+            if (hipMalloc(&p[i], sizes[i]) != hipSuccess) {
+                for (int k = 0; k < i; ++k) (void)hipFree(p[k]);
+                throw Fail{LPK_E_NOMEM, "network allocation"};
+            }
+        net->theta = static_cast<float *>(p[0]);
+        net->m1 = static_cast<float *>(p[1]);
+        net->m2 = static_cast<float *>(p[2]);
+        net->packed = p[3];
+        hip_check(hipMemcpyAsync(net->theta, th.data(), pb, hipMemcpyHostToDevice, ctx->stream), "network upload");
+        hip_check(hipMemsetAsync(net->m1, 0, pb, ctx->stream), "network init");
+        hip_check(hipMemsetAsync(net->m2, 0, pb, ctx->stream), "network init");
+        hip_check(launch_lws_pack(net->theta, net->packed, ctx->stream), "network pack");
+        hip_check(hipStreamSynchronize(ctx->stream), "network init");
+        *out = net.release();
+    });
+}
+
+void lpk_lws_destroy(lpk_lws *net) {
+    if (!net) return;
+    lpk_ctx *ctx = net->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->lws == net) ctx->lws = nullptr;
+    (void)hipFree(net->theta);
+    (void)hipFree(net->m1);
+    (void)hipFree(net->m2);
+    (void)hipFree(net->packed);
+    net->partials.release();
+    net->grad.release();
+    net->loss.release();
+    delete net;
+}
+
+int lpk_lws_get_params(lpk_lws *net, float *theta) {
+    if (!net || !theta) return LPK_E_INVALID;
+    return guarded(net->ctx, [&] {
+        hip_check(hipMemcpyAsync(theta, net->theta, kLwsParams * sizeof(float), hipMemcpyDeviceToHost,
+                                 net->ctx->stream),
+                  "parameter download");
+        hip_check(hipStreamSynchronize(net->ctx->stream), "parameter download");
+    });
+}
+
+int lpk_lws_set_params(lpk_lws *net, const float *theta) {
+    if (!net || !theta) return LPK_E_INVALID;
+    return guarded(net->ctx, [&] {
+        for (int p = 0; p < kLwsParams; ++p)
+            if (!std::isfinite(theta[p])) throw Fail{LPK_E_NONFINITE, "parameter " + std::to_string(p)};
+        lpk_ctx *ctx = net->ctx;
+        const size_t pb = kLwsParams * sizeof(float);
+        hip_check(hipMemcpyAsync(net->theta, theta, pb, hipMemcpyHostToDevice, ctx->stream), "parameter upload");
+        hip_check(hipMemsetAsync(net->m1, 0, pb, ctx->stream), "optimiser reset");
+        hip_check(hipMemsetAsync(net->m2, 0, pb, ctx->stream), "optimiser reset");
+        hip_check(launch_lws_pack(net->theta, net->packed, ctx->stream), "network pack");
+        hip_check(hipStreamSynchronize(ctx->stream), "parameter upload");
+        net->step = 0;
+    });
+}
+
+int lpk_lws_generate(lpk_ctx *ctx, uint64_t n, uint64_t seed, float *x_dev, float *y_dev, double *geom_dev) {
+    if (!ctx || (n && (!x_dev || !y_dev))) return LPK_E_INVALID;
+    return guarded(ctx, [&] { hip_check(launch_lws_gen(n, seed, x_dev, y_dev, geom_dev, ctx->stream), "pair generation"); });
+}
+
+int lpk_lws_infer(lpk_lws *net, const float *x_dev, uint64_t n, float *y_dev) {
+    if (!net || (n && (!x_dev || !y_dev))) return LPK_E_INVALID;
+    return guarded(net->ctx, [&] {
+        hip_check(launch_lws_infer(lws_net(net), x_dev, n, y_dev, net->ctx->stream), "inference launch");
+    });
+}
+
+int lpk_lws_gradient(lpk_lws *net, const float *x_dev, const float *y_dev, uint64_t n, float *grad, double *loss) {
+    if (!net || !x_dev || !y_dev || !grad || n == 0) return LPK_E_INVALID;
+    return guarded(net->ctx, [&] {
+        lpk_ctx *ctx = net->ctx;
+        const uint64_t rows = (n + kTrainQ - 1) / kTrainQ;
+        if (rows > (1u << 20)) throw Fail{LPK_E_INVALID, "gradient batch too large"};
+        float *part = static_cast<float *>(net->partials.ensure(rows * kLwsRow * sizeof(float), ctx->stream));
+        float *g = static_cast<float *>(net->grad.ensure((kLwsParams + 1) * sizeof(float), ctx->stream));
+        hip_check(launch_lws_train(lws_net(net), x_dev, y_dev, 0, n, uint32_t(rows), float(2.0 / double(n)), part,
+                                   ctx->stream),
+                  "training launch");
+        hip_check(launch_lws_update(part, int(rows), net->theta, net->m1, net->m2, net->packed, 0.f, 0.f, 0.f, 0.f,
+                                    0.f, 0.f, g, g + kLwsParams, ctx->stream),
+                  "gradient reduction");
+        std::vector<float> h(kLwsParams + 1);
+        hip_check(hipMemcpyAsync(h.data(), g, h.size() * sizeof(float), hipMemcpyDeviceToHost, ctx->stream),
+                  "gradient download");
+        hip_check(hipStreamSynchronize(ctx->stream), "gradient download");
+        std::memcpy(grad, h.data(), kLwsParams * sizeof(float));
+        if (loss) *loss = double(h[kLwsParams]) / double(n);
+    });
+}
+
+int lpk_lws_train(lpk_lws *net, const lpk_lws_train_cfg *cfg, lpk_lws_train_report *rep) {
+    if (!net || !cfg || !rep) return LPK_E_INVALID;
+    return guarded(net->ctx, [&] {
+        lpk_ctx *ctx = net->ctx;
+        const uint64_t n = cfg->n_pairs, batch = cfg->batch;
+        if (batch == 0 || batch % kTrainQ || batch > (1u << 20) || n < batch || cfg->epochs == 0 ||
+            !(cfg->lr > 0.0f) || !(cfg->lr_final >= 0.0f) || cfg->n_val == 0)
+            throw Fail{LPK_E_INVALID, "training configuration"};
+        *rep = lpk_lws_train_report{};
+        struct Bufs {
+            std::vector<void *> p;
+            ~Bufs() {
+                for (void *q : p) (void)hipFree(q);
+            }
+            float *get(size_t floats) {
+                void *q = nullptr;
+                if (hipMalloc(&q, std::max<size_t>(floats, 1) * sizeof(float)) != hipSuccess)
+                    throw Fail{LPK_E_NOMEM, "training data allocation"};
+                p.push_back(q);
+                return static_cast<float *>(q);
+            }
+        } bufs;
+        float *X = bufs.get(n * kLwsIn), *Y = bufs.get(n);
+        float *Xv = bufs.get(cfg->n_val * kLwsIn), *Yv = bufs.get(cfg->n_val), *Pv = bufs.get(cfg->n_val);
+        const uint64_t spe = n / batch, total = spe * cfg->epochs;
+        float *losses = bufs.get(total);
+        const uint32_t slices = uint32_t(batch / kTrainQ);
+        float *part = static_cast<float *>(net->partials.ensure(size_t(slices) * kLwsRow * sizeof(float), ctx->stream));
+        hipEvent_t ev[3];
+        for (auto &e : ev) hip_check(hipEventCreate(&e), "event create");
+        struct EvGuard {
+            hipEvent_t *e;
+            ~EvGuard() {
+                for (int i = 0; i < 3; ++i) (void)hipEventDestroy(e[i]);
+            }
+        } eg{ev};
+        hip_check(hipEventRecord(ev[0], ctx->stream), "event record");
+        hip_check(launch_lws_gen(n, cfg->data_seed, X, Y, nullptr, ctx->stream), "pair generation");
+        hip_check(launch_lws_gen(cfg->n_val, cfg->data_seed ^ 0x76616C6964617465ULL, Xv, Yv, nullptr, ctx->stream),
+                  "pair generation");
+        hip_check(hipEventRecord(ev[1], ctx->stream), "event record");
+        const LwsNetDev nd = lws_net(net);
+        const float b1 = 0.9f, b2 = 0.999f;
+        for (uint64_t st = 0; st < total; ++st) {
+            const uint64_t base = (st % spe) * batch;
+            const double prog = double(st) / double(total);
+            const float lr = float(cfg->lr_final + 0.5 * (cfg->lr - cfg->lr_final) * (1.0 + std::cos(3.141592653589793 * prog)));
+            const double t = double(net->step + 1);
+            const float c1 = float(1.0 / (1.0 - std::pow(double(b1), t)));
+            const float c2 = float(1.0 / (1.0 - std::pow(double(b2), t)));
+            hip_check(launch_lws_train(nd, X, Y, base, n, slices, float(2.0 / double(batch)), part, ctx->stream),
+                      "training launch");
+            hip_check(launch_lws_update(part, int(slices), net->theta, net->m1, net->m2, net->packed, lr, b1, b2,
+                                        1e-8f, c1, c2, nullptr, losses + st, ctx->stream),
+                      "update launch");
+            ++net->step;
+        }
+        hip_check(hipEventRecord(ev[2], ctx->stream), "event record");
+        hip_check(launch_lws_infer(nd, Xv, cfg->n_val, Pv, ctx->stream), "validation inference");
+        std::vector<float> hl(total), hy(cfg->n_val), hp(cfg->n_val);
+        hip_check(hipMemcpyAsync(hl.data(), losses, total * sizeof(float), hipMemcpyDeviceToHost, ctx->stream), "loss download");
+        hip_check(hipMemcpyAsync(hy.data(), Yv, hy.size() * sizeof(float), hipMemcpyDeviceToHost, ctx->stream), "validation download");
+        hip_check(hipMemcpyAsync(hp.data(), Pv, hp.size() * sizeof(float), hipMemcpyDeviceToHost, ctx->stream), "validation download");
+        hip_check(hipStreamSynchronize(ctx->stream), "training");
+        float ms = 0.0f;
+        hip_check(hipEventElapsedTime(&ms, ev[0], ev[1]), "event elapsed");
+        rep->gen_ms = ms;
+        hip_check(hipEventElapsedTime(&ms, ev[1], ev[2]), "event elapsed");
+        rep->train_ms = ms;
+        auto epoch_mean = [&](uint64_t e) {
+            double acc = 0.0;
+            for (uint64_t k = e * spe; k < (e + 1) * spe; ++k) acc += hl[k];
+            return acc / double(spe * batch);
+        };
+        rep->loss_first = epoch_mean(0);
+        rep->loss_last = epoch_mean(cfg->epochs - 1);
+        double se = 0.0, mx = 0.0;
+        for (uint64_t k = 0; k < cfg->n_val; ++k) {
+            const double d = (double(hp[k]) - double(hy[k])) * kLwsTargetScale;
+            se += d * d;
+            mx = std::max(mx, std::fabs(d));
+        }
+        rep->val_rms_s = std::sqrt(se / double(cfg->n_val));
+        rep->val_max_s = mx;
+        rep->steps = total;
+    });
+}
+
+int lpk_set_warm_start(lpk_ctx *ctx, lpk_lws *net) {
+    if (!ctx || (net && net->ctx != ctx)) return LPK_E_INVALID;
+    ctx->lws = net;
+    return LPK_OK;
+}
+
+int lpk_solve_stats(lpk_ctx *ctx, uint64_t stats[4]) {
+    if (!ctx || !stats) return LPK_E_INVALID;
+    return guarded(ctx, [&] {
+        ensure_stats(ctx);
+        unsigned long long h[4];
+        hip_check(hipMemcpyAsync(h, ctx->stats, sizeof h, hipMemcpyDeviceToHost, ctx->stream), "counter download");
+        hip_check(hipStreamSynchronize(ctx->stream), "counter download");
+        for (int k = 0; k < 4; ++k) stats[k] = h[k];
+    });
+}
+
+int lpk_solve_stats_reset(lpk_ctx *ctx) {
+    if (!ctx) return LPK_E_INVALID;
+    return guarded(ctx, [&] {
+        ensure_stats(ctx);
+        hip_check(hipMemsetAsync(ctx->stats, 0, 4 * sizeof(unsigned long long), ctx->stream), "counter reset");
+    });
+}
 
 int lpk_device_fnv1a(lpk_ctx *ctx, const void *dev, size_t bytes, uint64_t *hash) {
     if (!ctx || !hash || (bytes && !dev)) return LPK_E_INVALID;

@@ -3,6 +3,8 @@
 // per render call (JobDev, LinkDev).
 #pragma once
 
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
 
 namespace lpk {
@@ -96,7 +98,21 @@ struct TxDev {
     int32_t n_bursts, _pad;
     double gate_period, gate_phase;
     const double *bursts;
+    // circular Keplerian elements (src/geometry/kepler.rs:19-25), radians and
+    // rad/s, for the learned warm start's features; kep_on = 0 when the
+    // trajectory has none (the start is then the straight-line light time)
+    int32_t kep_on, _kpad;
+    double kep_a, kep_cos_i, kep_sin_i, kep_raan, kep_nu0, kep_n, kep_epoch;
 };
+
+// Converged emission-time solve (the converged and learned forms): the
+// reference's iteration t <- tu - |R(w (tu - t)) P(t) - O| / c on the
+// unrounded time axis, stopped at the first evaluation whose update is below
+// kSolveTol seconds. A learned start gets kLearnedEvals evaluations before it
+// falls back to the reference's start tu - 0.07 s with kColdEvals.
+constexpr double kSolveTol = 1e-13;
+constexpr int kLearnedEvals = 3;
+constexpr int kColdEvals = 12;
 
 // One (band, active transmitter) pair of a render call, in summation order.
 struct LinkDev {
@@ -147,6 +163,28 @@ struct JobDev {
     int32_t ant_on, ant_cut;
     double ant_peak, ant_horizon, ant_fb, ant_cutoff;
     uint32_t *unocc;         // per link of the job: set to 1 where some sample's gain is non-zero
+    // learned form: the job's predicted light times start at pred[pred0], one
+    // row of n_tiles + 1 tile-boundary values per link
+    uint64_t pred0;
 };
+
+// Philox4x64-10 (Salmon, Moraes, Dror, Shaw, SC'11).
+__device__ __forceinline__ void philox4x64(uint64_t c[4], uint64_t k0, uint64_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r) {
+            k0 += 0x9E3779B97F4A7C15ULL;
+            k1 += 0xBB67AE8584CAA73BULL;
+        }
+        const uint64_t m0 = 0xD2E7470EE14C6C93ULL, m1 = 0xCA5A826395121157ULL;
+        const uint64_t hi0 = __umul64hi(m0, c[0]), lo0 = m0 * c[0];
+        const uint64_t hi1 = __umul64hi(m1, c[2]), lo1 = m1 * c[2];
+        const uint64_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0;
+        c[1] = lo1;
+        c[2] = n2;
+        c[3] = lo0;
+    }
+}
 
 }  // namespace lpk

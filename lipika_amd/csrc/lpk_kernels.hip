@@ -22,6 +22,13 @@
 // sine and cosine (hardware, in turns) and the mix run in f32, and the f32
 // contributions are summed exactly in f64 (the compensated sum).
 //
+// FORM 2 (converged) is the plain form with the emission-time solve iterated
+// to a residual tolerance (kSolveTol) on the unrounded time axis instead of
+// the reference's two fixed iterations; it starts either from the
+// reference's t - 0.07 s or, in the learned form, from light times that
+// lws_predict_kernel's network predicted at the tile's boundaries, falling
+// back to the reference's start when those do not converge quickly.
+//
 // walk_gen_kernel: the receiver clock's white-FM walk (src/clock/mod.rs:
 // 299-307) on the master grid, as fixed 65 536-index blocks; values depend
 // only on (seed, index), never on how the render is chunked.
@@ -33,6 +40,7 @@
 #include <cstdint>
 
 #include "lpk_device.hpp"
+#include "lpk_lws.hpp"
 
 namespace lpk {
 
@@ -60,25 +68,6 @@ __device__ __forceinline__ int64_t rem_euclid(int64_t a, int64_t n) {
 __device__ __forceinline__ double rem_euclid_f64(double a, double b) {
     const double r = fmod(a, b);
     return r < 0.0 ? r + fabs(b) : r;
-}
-
-// Philox4x64-10 (Salmon, Moraes, Dror, Shaw, SC'11).
-__device__ __forceinline__ void philox4x64(uint64_t c[4], uint64_t k0, uint64_t k1) {
-#pragma unroll
-    for (int r = 0; r < 10; ++r) {
-        if (r) {
-            k0 += 0x9E3779B97F4A7C15ULL;
-            k1 += 0xBB67AE8584CAA73BULL;
-        }
-        const uint64_t m0 = 0xD2E7470EE14C6C93ULL, m1 = 0xCA5A826395121157ULL;
-        const uint64_t hi0 = __umul64hi(m0, c[0]), lo0 = m0 * c[0];
-        const uint64_t hi1 = __umul64hi(m1, c[2]), lo1 = m1 * c[2];
-        const uint64_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
-        c[0] = n0;
-        c[1] = lo1;
-        c[2] = n2;
-        c[3] = lo0;
-    }
 }
 
 // Box-Muller pair from one 64-bit Philox word: u1 = (m1 + 1/2) 2^-24 from bits
@@ -194,6 +183,58 @@ __device__ __forceinline__ void solve_emission(const TxDev &T, const double *obs
         t = tu - r * kInvC;
     }
     t_emit = t;
+}
+
+// PrecomputedTrajectory::position_ecef (src/geometry/precomputed.rs:108-121)
+// at time trel after the grid's node 0, given directly instead of as an
+// absolute time, so it is not rounded at the scale of the epoch.
+__device__ __forceinline__ void tx_position_rel(const TxDev &T, double trel, double &x, double &y, double &z) {
+    if (T.traj_kind == 0) {
+        x = T.pos[0];
+        y = T.pos[1];
+        z = T.pos[2];
+        return;
+    }
+    const double f = trel / T.grid_dt;
+    const double flo = fmin(fmax(floor(f), 0.0), static_cast<double>(T.grid_n - 2));
+    const int64_t lo = static_cast<int64_t>(flo);
+    double frac = f - flo;
+    frac = frac < 0.0 ? 0.0 : (frac > 1.0 ? 1.0 : frac);
+    const double omf = 1.0 - frac;
+    const double *g = T.grid;
+    const int64_t n = T.grid_n;
+    x = g[lo] * omf + g[lo + 1] * frac;
+    y = g[n + lo] * omf + g[n + lo + 1] * frac;
+    z = g[2 * n + lo] * omf + g[2 * n + lo + 1] * frac;
+}
+
+// The converged solve: the reference's iteration (src/channel/mod.rs:75-96)
+// from emission time t, with positions at eoff + t after node 0 (eoff = scene
+// epoch - grid start), until an update |t_next - t| < kSolveTol. Returns the
+// number of evaluations, or -1 when max_evals did not converge; t_emit and r
+// are the last iterate and the range it came from (r / c = tu - t_emit).
+__device__ __forceinline__ int solve_converged(const TxDev &T, const double *obs, double tu, double eoff, double t,
+                                               int max_evals, double &t_emit, double &r) {
+    for (int e = 1; e <= max_evals; ++e) {
+        double sx, sy, sz;
+        tx_position_rel(T, eoff + t, sx, sy, sz);
+        const double wtau = kEarthRotation * (tu - t);
+        const double sx_rot = sx + wtau * sy;
+        const double sy_rot = sy - wtau * sx;
+        const double dx = sx_rot - obs[0];
+        const double dy = sy_rot - obs[1];
+        const double dz = sz - obs[2];
+        r = sqrt(dx * dx + dy * dy + dz * dz);
+        const double tn = tu - r * kInvC;
+        const double step = tn - t;
+        t = tn;
+        if (fabs(step) < kSolveTol) {
+            t_emit = t;
+            return e;
+        }
+    }
+    t_emit = t;
+    return -1;
 }
 
 // Antenna::amplitude_at (src/rx/mod.rs:48-58) at the arrival elevation of
@@ -559,16 +600,19 @@ struct RenderLaunch {
     uint32_t tile_base, n_tiles;  // this dispatch's slice of the call's tiles
     const uint32_t *n_live;       // if set: tiles from *n_live on have no work (streaming plans)
     uint32_t xcd_map;             // deal contiguous tile runs to each XCD
+    const double *pred;           // converged form: predicted light times (JobDev::pred0), or null
+    unsigned long long *stats;    // converged form: evaluations, contributions, fallbacks, unconverged
 };
 
 template <int FORM, bool ANT, int TS>
-__global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM ? 5 : 6))
+__global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM == 0 ? 6 : 5))
     render_kernel(const JobDev *__restrict__ jobs, int n_jobs, const LinkDev *__restrict__ links,
                   const TxDev *__restrict__ txs, RenderLaunch lx, TapOut tap) {
     constexpr int kTile = 1 << TS;
     __shared__ uint64_t s_noise[kTile];              // one Philox word per sample
     __shared__ double s_anc_t[3][3];                  // tu, t_local, delta_t at the anchors
-    __shared__ Anchor s_anc[FORM ? kAnchorGroup : 1];
+    __shared__ Anchor s_anc[FORM == 1 ? kAnchorGroup : 1];
+    __shared__ unsigned long long s_stats[4];
 
     const uint32_t tile = lx.tile_base + (lx.xcd_map ? xcd_tile(blockIdx.x, lx.n_tiles) : blockIdx.x);
     if (lx.n_live && tile >= *lx.n_live) return;
@@ -606,13 +650,47 @@ __global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM ? 5 : 6))
     float acc_re = 0.0f, acc_im = 0.0f;
     double sum_re = 0.0, sum_im = 0.0;  // exact sum of the f32 contributions (precision form)
 
-    if (FORM == 0) {
+    if (FORM != 1) {
+        // FORM 2: where this lane sits between the tile's two predicted light
+        // times (boundary samples clamped into the job, as the predictor
+        // places them)
+        const double *prow = nullptr;
+        double frac = 0.0;
+        uint32_t n_evals = 0, n_fallback = 0, n_unconv = 0;
+        if (FORM == 2 && lx.pred) {
+            const uint64_t last = J.i0 + J.clen - 1;
+            const uint64_t ib0 = tile_start < J.i0 ? J.i0 : tile_start;
+            const uint64_t ib1 = tile_start + kTile > last ? last : tile_start + kTile;
+            frac = ib1 > ib0 ? static_cast<double>(ic - ib0) / static_cast<double>(ib1 - ib0) : 0.0;
+            prow = lx.pred + J.pred0 + (tile - J.tile0);
+        }
         for (uint32_t s = 0; s < J.n_active; ++s) {
             const LinkDev L = links[J.link0 + s];
             const TxDev &T = txs[L.tx];
             double t_emit, r;
-            int64_t c0, c1;
-            solve_emission(T, J.obs, tu, J.epoch, t_emit, r, c0, c1);
+            if (FORM == 0) {
+                int64_t c0, c1;
+                solve_emission(T, J.obs, tu, J.epoch, t_emit, r, c0, c1);
+            } else {
+                const double eoff = T.traj_kind ? J.epoch - T.grid_t0 : 0.0;
+                int e = -1;
+                if (prow) {
+                    const double *pp = prow + static_cast<uint64_t>(s) * (J.n_tiles + 1u);
+                    const double ta = pp[0], tb = pp[1];
+                    e = solve_converged(T, J.obs, tu, eoff, tu - (ta + (tb - ta) * frac), kLearnedEvals, t_emit, r);
+                    if (e < 0) {
+                        n_evals += kLearnedEvals;
+                        ++n_fallback;
+                    }
+                }
+                if (e < 0) e = solve_converged(T, J.obs, tu, eoff, tu - kEmissionInitialLag, kColdEvals, t_emit, r);
+                if (e < 0) {
+                    n_evals += kColdEvals;
+                    ++n_unconv;
+                } else {
+                    n_evals += static_cast<uint32_t>(e);
+                }
+            }
             // src/channel/mod.rs:255-267
             const double delay = r * kInvC;
             const double phase = T.neg_tau_src * (delay + dlt) + L.tau_if * t_local;
@@ -636,6 +714,21 @@ __global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM ? 5 : 6))
             const float amplitude = static_cast<float>(T.amp_factor / r) * g;
             acc_re = acc_re + amplitude * (bre * cpf - bim * spf);
             acc_im = acc_im + amplitude * (bre * spf + bim * cpf);
+        }
+        if (FORM == 2 && lx.stats) {
+            // per-wave sums, one LDS add per wave, one global add per workgroup
+            if (threadIdx.x < 4) s_stats[threadIdx.x] = 0;
+            __syncthreads();
+            auto wave_add = [&](uint32_t x, int k) {
+                for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o);
+                if ((threadIdx.x & 63) == 0 && x) atomicAdd(&s_stats[k], static_cast<unsigned long long>(x));
+            };
+            wave_add(valid ? n_evals : 0u, 0);
+            wave_add(valid ? J.n_active : 0u, 1);
+            wave_add(valid ? n_fallback : 0u, 2);
+            wave_add(valid ? n_unconv : 0u, 3);
+            __syncthreads();
+            if (threadIdx.x < 4 && s_stats[threadIdx.x]) atomicAdd(&lx.stats[threadIdx.x], s_stats[threadIdx.x]);
         }
     } else {
         const int a_first = static_cast<int>(first - tile_start);
@@ -805,6 +898,56 @@ __global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM ? 5 : 6))
     if (J.codes) reinterpret_cast<short2 *>(J.codes)[j] = make_short2(code_re, code_im);
 }
 
+// ---------------------------------------------------------------- learned start
+
+// Predicted light time at every tile boundary of every link of a render call
+// (query q = pred0 + link * (n_tiles + 1) + boundary of its job); boundary b
+// sits at sample ((i0 >> TS) + b) << TS clamped into the job. The prediction is
+// the straight-line light time at that sample's reception time plus the
+// network's correction (none for a trajectory without orbital elements).
+// 128 queries per workgroup, 32 per wave on the matrix cores.
+__global__ void __launch_bounds__(256) lws_predict_kernel(const JobDev *__restrict__ jobs, int n_jobs,
+                                                          const LinkDev *__restrict__ links,
+                                                          const TxDev *__restrict__ txs, LwsNetDev net,
+                                                          int tile_shift, uint64_t n_queries,
+                                                          double *__restrict__ pred) {
+    const int lane = static_cast<int>(threadIdx.x & 63);
+    const uint64_t q = (uint64_t(blockIdx.x) * 4 + (threadIdx.x >> 6)) * 32 + (lane & 31);
+    const bool valid = q < n_queries;
+    const uint64_t qc = valid ? q : n_queries - 1;
+    int lo = 0, hi = n_jobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].pred0 <= qc) lo = mid;
+        else hi = mid - 1;
+    }
+    const JobDev &J = jobs[lo];
+    const uint64_t pitch = uint64_t(J.n_tiles) + 1;
+    const uint64_t local = qc - J.pred0;
+    const uint64_t s = local / pitch;
+    const uint64_t b = local - s * pitch;
+    const uint64_t last = J.i0 + J.clen - 1;
+    uint64_t ib = ((J.i0 >> tile_shift) + b) << tile_shift;
+    ib = ib < J.i0 ? J.i0 : (ib > last ? last : ib);
+    const double t_local = static_cast<double>(ib) / J.fs;
+    const double dlt = J.delta_t ? J.delta_t[ib - J.i0] : clock_delta(J, t_local);
+    const double tu = t_local - dlt;
+    const TxDev &T = txs[links[J.link0 + s].tx];
+    const double eoff = T.traj_kind ? J.epoch - T.grid_t0 : 0.0;
+    double sx, sy, sz;
+    tx_position_rel(T, eoff + tu, sx, sy, sz);
+    const double dx = sx - J.obs[0], dy = sy - J.obs[1], dz = sz - J.obs[2];
+    const double tau_sl = sqrt(dx * dx + dy * dy + dz * dz) * kInvC;
+    float x[kLwsIn] = {};
+    if (T.kep_on) {
+        const double dt = (J.epoch - T.kep_epoch) + tu;
+        lws_features(J.obs, T.kep_a, T.kep_cos_i, T.kep_sin_i, T.kep_raan - kEarthRotation * dt,
+                     T.kep_nu0 + T.kep_n * dt, x);
+    }
+    const float y = lws_forward(net, x, nullptr, 0);
+    if (valid && lane < 32) pred[q] = T.kep_on ? tau_sl + static_cast<double>(y) * kLwsTargetScale : tau_sl;
+}
+
 // ---------------------------------------------------------------- clock walk
 
 // One 65 536-index block of the walk per workgroup, over the segments of
@@ -932,20 +1075,32 @@ RenderFn<FORM, ANT> render_fn(int tile_shift) {
 hipError_t launch_render(int form, bool antenna, int tile_shift, bool xcd_map, const JobDev *jobs, int n_jobs,
                          const LinkDev *links, const TxDev *txs, uint32_t n_tiles, const uint32_t *n_live,
                          double *tap_t_emit, double *tap_range, double *tap_phase, double *tap_delta_t,
-                         uint32_t tap_slot, hipStream_t stream) {
+                         uint32_t tap_slot, const double *pred, unsigned long long *stats, hipStream_t stream) {
     TapOut tap{tap_t_emit, tap_range, tap_phase, tap_delta_t, tap_slot, tap_t_emit ? 1u : 0u};
-    if (tile_shift < 7 || tile_shift > 9) return hipErrorInvalidValue;
+    if (tile_shift < 7 || tile_shift > 9 || form < 0 || form > 2) return hipErrorInvalidValue;
     void (*kernel)(const JobDev *, int, const LinkDev *, const TxDev *, RenderLaunch, TapOut) =
-        form == 0 ? (antenna ? render_fn<0, true>(tile_shift) : render_fn<0, false>(tile_shift))
-                  : (antenna ? render_fn<1, true>(tile_shift) : render_fn<1, false>(tile_shift));
+        form == 0   ? (antenna ? render_fn<0, true>(tile_shift) : render_fn<0, false>(tile_shift))
+        : form == 1 ? (antenna ? render_fn<1, true>(tile_shift) : render_fn<1, false>(tile_shift))
+                    : (antenna ? render_fn<2, true>(tile_shift) : render_fn<2, false>(tile_shift));
     for (uint32_t base = 0; base < n_tiles; base += kMaxLaunchTiles) {
         const uint32_t n = n_tiles - base < kMaxLaunchTiles ? n_tiles - base : kMaxLaunchTiles;
-        const RenderLaunch lx{base, n, n_live, xcd_map ? 1u : 0u};
+        const RenderLaunch lx{base, n, n_live, xcd_map ? 1u : 0u, pred, stats};
         hipLaunchKernelGGL(kernel, dim3(n), dim3(1u << tile_shift), 0, stream, jobs, n_jobs, links, txs, lx, tap);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
+}
+
+hipError_t launch_lws_predict(const JobDev *jobs, int n_jobs, const LinkDev *links, const TxDev *txs,
+                              const LwsNetDev &net, int tile_shift, uint64_t n_queries, double *pred,
+                              hipStream_t stream) {
+    if (!n_queries || n_jobs <= 0) return hipSuccess;
+    const uint64_t blocks = (n_queries + 127) / 128;
+    if (blocks > 0x7FFFFFFFull) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(lws_predict_kernel, dim3(uint32_t(blocks)), dim3(256), 0, stream, jobs, n_jobs, links, txs,
+                       net, tile_shift, n_queries, pred);
+    return hipGetLastError();
 }
 
 hipError_t launch_walk_gen(const WalkSegDev *segs, int n_segs, uint32_t n_blocks, const uint32_t *n_live,

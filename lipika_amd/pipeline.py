@@ -89,6 +89,8 @@ class PreparedTx:
     gate_period: float = 0.0
     gate_phase: float = 0.0
     bursts: Optional[np.ndarray] = None        # float64 (k, 2): (start, end)
+    # (a m, i deg, raan deg, nu0 deg, epoch unix) of a CircularKeplerian trajectory
+    kepler: Optional[tuple] = None
 
     @property
     def n_symbols(self) -> int:
@@ -166,6 +168,8 @@ def build_transmitter(cfg: TransmitterConfig, meta: SceneMeta, time_base: TimeBa
             grid.ctypes.data_as(C.POINTER(C.c_double)), n))
         tx.traj_kind = _lib.TRAJ_GRID
         tx.grid, tx.grid_t0, tx.grid_dt = grid, t0.value, TX_TRAJECTORY_GRID_DT_S
+        tx.kepler = (traj.semi_major_axis_m, traj.inclination_deg, traj.raan_deg, traj.true_anomaly_deg_at_epoch,
+                     traj.epoch_unix)
     else:
         raise NotImplementedError(f"trajectory {traj!r} is not on the GPU path")
 
@@ -385,6 +389,10 @@ class Backend:
                 keep.append(bb)
                 d.gated, d.n_bursts, d.gate_period_s, d.gate_phase_s = 1, len(bb), t.gate_period, t.gate_phase
                 d.bursts = _ptr(bb, C.c_double)
+            if t.kepler is not None:
+                d.has_kepler = 1
+                (d.kepler_a_m, d.kepler_inc_deg, d.kepler_raan_deg, d.kepler_nu0_deg,
+                 d.kepler_epoch_unix) = (float(v) for v in t.kepler)
         rx_arr = (_lib.RxDesc * max(n_rx, 1))()
         for r_idx, r in enumerate(scene.receivers):
             d = rx_arr[r_idx]
@@ -472,6 +480,20 @@ class Backend:
         """A streaming plan over whole-band descriptors (see Plan)."""
         return Plan(self, bands, chunk_samples, form)
 
+    def warm_start(self, net: Optional["WarmStartNet"]) -> None:
+        """The network FORM_LEARNED predicts light times with (None: none)."""
+        self.check(self.lib.lpk_set_warm_start(self.h, net.h if net is not None else None))
+        self._warm = net
+
+    def solve_stats(self) -> dict:
+        """Converged-form counters since the last reset."""
+        a = (C.c_uint64 * 4)()
+        self.check(self.lib.lpk_solve_stats(self.h, a))
+        return dict(evaluations=a[0], contributions=a[1], fallbacks=a[2], unconverged=a[3])
+
+    def solve_stats_reset(self) -> None:
+        self.check(self.lib.lpk_solve_stats_reset(self.h))
+
     def taps(self, job, slot: int, form: int = _lib.FORM_PLAIN):
         n = job.clen
         te, rg, ph, dt = (np.empty(n, dtype=np.float64) for _ in range(4))
@@ -517,7 +539,94 @@ class Plan:
         self.close()
 
 
-_FORMS = {"plain": _lib.FORM_PLAIN, "precision": _lib.FORM_PRECISION}
+class DeviceArray:
+    """A device buffer of the backend's context (lpk_device_alloc)."""
+
+    def __init__(self, backend: Backend, shape, dtype):
+        self.be, self.shape, self.dtype = backend, tuple(np.atleast_1d(shape)), np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        backend.check(backend.lib.lpk_device_alloc(backend.h, max(self.nbytes, 1), C.byref(p)))
+        self.ptr = p.value
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=self.dtype)
+        self.be.check(self.be.lib.lpk_copy_to_host(self.be.h, out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            self.be.lib.lpk_device_free(self.be.h, self.ptr)
+            self.ptr = None
+
+
+class WarmStartNet:
+    """lpk_lws_*: the learned warm start's network on the backend's device.
+    Training pairs come from the device's own converged light-time solve;
+    training runs the device's forward (matrix cores) and backward kernels."""
+
+    def __init__(self, backend: Backend, init_seed: int = 1):
+        self.be = backend
+        h = C.c_void_p()
+        backend.check(backend.lib.lpk_lws_create(backend.h, init_seed & U64, C.byref(h)))
+        self.h = h
+
+    def close(self) -> None:
+        if self.h:
+            if getattr(self.be, "_warm", None) is self:
+                self.be.warm_start(None)
+            self.be.lib.lpk_lws_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def params(self) -> np.ndarray:
+        th = np.empty(_lib.LWS_PARAMS, dtype=np.float32)
+        self.be.check(self.be.lib.lpk_lws_get_params(self.h, _ptr(th, C.c_float)))
+        return th
+
+    @params.setter
+    def params(self, theta) -> None:
+        th = np.ascontiguousarray(theta, dtype=np.float32)
+        assert th.shape == (_lib.LWS_PARAMS,)
+        self.be.check(self.be.lib.lpk_lws_set_params(self.h, _ptr(th, C.c_float)))
+
+    def generate(self, n: int, seed: int, geometry: bool = False) -> tuple:
+        """(X, Y, G) device arrays of n training pairs (G None unless asked)."""
+        X, Y = DeviceArray(self.be, (n, 10), np.float32), DeviceArray(self.be, (n,), np.float32)
+        G = DeviceArray(self.be, (n, 8), np.float64) if geometry else None
+        self.be.check(self.be.lib.lpk_lws_generate(self.be.h, n, seed & U64, X.ptr, Y.ptr, G.ptr if G else None))
+        return X, Y, G
+
+    def infer(self, X: DeviceArray, n: int) -> np.ndarray:
+        Y = DeviceArray(self.be, (n,), np.float32)
+        try:
+            self.be.check(self.be.lib.lpk_lws_infer(self.h, X.ptr, n, Y.ptr))
+            return Y.numpy()
+        finally:
+            Y.free()
+
+    def gradient(self, X: DeviceArray, Y: DeviceArray, n: int) -> tuple:
+        g = np.empty(_lib.LWS_PARAMS, dtype=np.float32)
+        loss = C.c_double()
+        self.be.check(self.be.lib.lpk_lws_gradient(self.h, X.ptr, Y.ptr, n, _ptr(g, C.c_float), C.byref(loss)))
+        return loss.value, g
+
+    def train(self, n_pairs: int = 10_000_000, data_seed: int = 2026, epochs: int = 8, batch: int = 16384,
+              lr: float = 2e-3, lr_final: float = 2e-5, n_val: int = 100_000) -> dict:
+        cfg = _lib.LwsTrainCfg(n_pairs, data_seed & U64, epochs, batch, lr, lr_final, n_val)
+        rep = _lib.LwsTrainReport()
+        self.be.check(self.be.lib.lpk_lws_train(self.h, C.byref(cfg), C.byref(rep)))
+        return {k: getattr(rep, k) for k, _ in rep._fields_}
+
+
+_FORMS = {"plain": _lib.FORM_PLAIN, "precision": _lib.FORM_PRECISION, "converged": _lib.FORM_CONVERGED,
+          "learned": _lib.FORM_LEARNED}
 
 
 def form_code(form) -> int:

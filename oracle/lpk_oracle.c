@@ -133,9 +133,7 @@ uint64_t orc_grid_sample_kepler(double a_m, double inc_deg, double raan_deg, dou
 }
 
 /* src/geometry/precomputed.rs:111-121 (position_ecef: division by dt) */
-void orc_grid_position(double t0_unix, double dt_s, uint64_t n, const double *xyz,
-                       double t_unix, double out[3]) {
-    double f = (t_unix - t0_unix) / dt_s;
+static void grid_lerp(double f, uint64_t n, const double *xyz, double out[3]) {
     double fl = floor(f);
     int64_t max_lo = (int64_t)n - 2;
     int64_t lo = sat_i64(fl);
@@ -150,6 +148,11 @@ void orc_grid_position(double t0_unix, double dt_s, uint64_t n, const double *xy
     out[2] = xyz[2 * n + lo] * omf + xyz[2 * n + lo + 1] * frac;
 }
 
+void orc_grid_position(double t0_unix, double dt_s, uint64_t n, const double *xyz,
+                       double t_unix, double out[3]) {
+    grid_lerp((t_unix - t0_unix) / dt_s, n, xyz, out);
+}
+
 static void tx_position(const orc_tx *tx, double t_unix, double out[3]) {
     if (tx->traj_kind == ORC_TRAJ_STATIONARY) {   /* src/geometry/stationary.rs:21-24 */
         out[0] = tx->pos_ecef[0];
@@ -158,6 +161,47 @@ static void tx_position(const orc_tx *tx, double t_unix, double out[3]) {
     } else {
         orc_grid_position(tx->grid_t0_unix, tx->grid_dt_s, tx->grid_n, tx->grid_xyz, t_unix, out);
     }
+}
+
+/* The converged solve: the iteration of src/channel/mod.rs:75-96 from emission
+ * time t_start, with the grid queried at (epoch - grid_t0) + t after its node 0
+ * (src/geometry/precomputed.rs:108-121 without rounding the absolute time),
+ * until an update |t_next - t| < tol. Returns the evaluations used, or -1 if
+ * max_evals did not converge; *t_emit = last iterate, *range = the range it
+ * came from. */
+int orc_solve_converged(const orc_tx *tx, const double obs[3], double tu, double epoch, double t_start,
+                        double tol, int max_evals, double *t_emit, double *range) {
+    const double inv_c = 1.0 / SPEED_OF_LIGHT_M_S;
+    const double eoff = tx->traj_kind == ORC_TRAJ_STATIONARY ? 0.0 : epoch - tx->grid_t0_unix;
+    double t = t_start, r = 0.0;
+    for (int e = 1; e <= max_evals; ++e) {
+        double s[3];
+        if (tx->traj_kind == ORC_TRAJ_STATIONARY) {
+            s[0] = tx->pos_ecef[0];
+            s[1] = tx->pos_ecef[1];
+            s[2] = tx->pos_ecef[2];
+        } else {
+            grid_lerp((eoff + t) / tx->grid_dt_s, tx->grid_n, tx->grid_xyz, s);
+        }
+        double wtau = EARTH_ROTATION_RATE_RAD_S * (tu - t);
+        double sx_rot = s[0] + wtau * s[1];
+        double sy_rot = s[1] - wtau * s[0];
+        double dx = sx_rot - obs[0];
+        double dy = sy_rot - obs[1];
+        double dz = s[2] - obs[2];
+        r = sqrt(dx * dx + dy * dy + dz * dz);
+        double tn = tu - r * inv_c;
+        double step = tn - t;
+        t = tn;
+        if (fabs(step) < tol) {
+            *t_emit = t;
+            *range = r;
+            return e;
+        }
+    }
+    *t_emit = t;
+    *range = r;
+    return -1;
 }
 
 /* src/channel/mod.rs:70-99 (one sample of solve_emission_times_for) */
@@ -668,8 +712,13 @@ static void tx_contribution(const orc_tx *tx, const orc_band *band, const double
                             uint64_t n, double lo, double epoch, double *t_emit, double *ranges,
                             float *bb, float *out, uint8_t *unoccluded, double *tap_t_emit,
                             double *tap_range, double *tap_phase) {
-    for (uint64_t i = 0; i < n; ++i)
-        orc_solve_emission(tx, obs, t_utc[i], epoch, NEWTON_ITERATIONS, &t_emit[i], &ranges[i]);
+    for (uint64_t i = 0; i < n; ++i) {
+        if (band->converged)
+            orc_solve_converged(tx, obs, t_utc[i], epoch, t_utc[i] - EMISSION_INITIAL_LAG_S, 1e-13, 12, &t_emit[i],
+                                &ranges[i]);
+        else
+            orc_solve_emission(tx, obs, t_utc[i], epoch, NEWTON_ITERATIONS, &t_emit[i], &ranges[i]);
+    }
     orc_baseband(tx, t_emit, n, bb);
     int any = 0;
     double src_carrier = tx->carrier_hz;

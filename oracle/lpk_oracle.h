@@ -100,6 +100,9 @@ typedef struct {
     int32_t ant_on, ant_has_cutoff;
     double ant_peak_dbi, ant_horizon_dbi, ant_fb_db, ant_cutoff_deg;
     uint8_t *unoccluded;         /* optional output [n_active]: ORed over the rendered chunks */
+    /* 1: the converged solve (orc_solve_converged from t - 0.07 s, tolerance
+     * 1e-13 s, at most 12 evaluations) in place of the reference's two iterations */
+    int32_t converged, _p2;
 } orc_band;
 
 /* ---- antenna (src/rx/mod.rs:48-58, src/channel/mod.rs:190-207) ---- */
@@ -117,6 +120,8 @@ uint64_t orc_grid_sample_kepler(double a_m, double inc_deg, double raan_deg, dou
                                 double grid_dt_s, double *t0_out, double *xyz_out);
 void orc_grid_position(double t0_unix, double dt_s, uint64_t n, const double *xyz,
                        double t_unix, double out[3]);
+int orc_solve_converged(const orc_tx *tx, const double obs[3], double tu, double epoch, double t_start,
+                        double tol, int max_evals, double *t_emit, double *range);
 void orc_solve_emission(const orc_tx *tx, const double obs[3], double t_utc_rel, double epoch,
                         int iterations, double *t_emit, double *range);
 

@@ -52,7 +52,7 @@ class OrcBand(C.Structure):
                 ("post", OrcPost * 4),
                 ("ant_on", C.c_int32), ("ant_has_cutoff", C.c_int32),
                 ("ant_peak_dbi", _d), ("ant_horizon_dbi", _d), ("ant_fb_db", _d), ("ant_cutoff_deg", _d),
-                ("unoccluded", P(C.c_uint8))]
+                ("unoccluded", P(C.c_uint8)), ("converged", C.c_int32), ("_p2", C.c_int32)]
 
 
 class OrcClock(C.Structure):
@@ -82,6 +82,7 @@ def load() -> C.CDLL:
         "orc_grid_sample_kepler": (_u64, [_d, _d, _d, _d, _d, _d, _d, _d, P(_d), P(_d)]),
         "orc_grid_position": (None, [_d, _d, _u64, P(_d), _d, P(_d)]),
         "orc_solve_emission": (None, [P(OrcTx), P(_d), _d, _d, C.c_int, P(_d), P(_d)]),
+        "orc_solve_converged": (C.c_int, [P(OrcTx), P(_d), _d, _d, _d, _d, C.c_int, P(_d), P(_d)]),
         "orc_ca_code": (C.c_int, [C.c_int, P(C.c_int8)]),
         "orc_nav_dummy_bit_pm": (C.c_int8, [_u64, _i64]),
         "orc_nav_fold_seed": (_u64, [_u64, _u64]),
@@ -190,13 +191,16 @@ class OracleScene:
                     b.ant_has_cutoff, b.ant_cutoff_deg = 1, cutoff
 
     def render_band(self, work_idx: int, n: int | None = None, chunk_samples: int = 1 << 20,
-                    tap_slot: int | None = None, codes: bool = False, unoccluded: np.ndarray | None = None):
+                    tap_slot: int | None = None, codes: bool = False, unoccluded: np.ndarray | None = None,
+                    converged: bool = False):
         """Renders samples [0, n) of unit `work_idx`; returns (iq, codes, taps).
         `unoccluded` (uint8 [n_active], zeroed by the caller) receives the
-        per-transmitter flags."""
+        per-transmitter flags. `converged` swaps the reference's two-iteration
+        solve for the converged one (orc_solve_converged)."""
         lib = load()
         r_idx, pb = self.units[work_idx]
         self.bands[work_idx].unoccluded = _ptr(unoccluded, C.c_uint8)
+        self.bands[work_idx].converged = 1 if converged else 0
         n = pb.n if n is None else n
         out = np.empty(n, dtype=np.complex64)
         cd = np.empty(2 * n, dtype=np.int16) if codes else None
