@@ -11,6 +11,9 @@ timed region starts.
 Metric: contributions/s = sum over (receiver band, in-band transmitter) of
 samples / wall time, in Msamples/s. Output samples/s is printed beside it.
 
+Configurations 4 (uneven bands, the batching form) and 5 (the learned warm
+start) are measured after the headline and reported under config.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--form plain|precision]
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -38,6 +41,8 @@ WORKLOADS = {
     2: "BPSK LEO 437 MHz + 6 C/A MEO 1575.42 MHz; bands 48 kHz and 2.048 MHz; clock walk and kTB noise on; 60 s",
     4: "40 transmitters (CW, OOK, BPSK, 16-QAM, C/A) x 64 receivers with 130 bands at 48 kHz to 8.184 MHz; "
        "clock walks and kTB noise on; 30 s; every chunk of every band in one launch",
+    5: "32 C/A MEO transmitters x 256 receivers, one L1 band each at 2.048 MHz, seeded clocks, kTB noise on; "
+       "the first {chunks} Mi-sample chunks of every band ({secs:.3f} s of the 60 s scene) per step",
 }
 
 
@@ -142,6 +147,94 @@ def gpu_leg(args, scene_n, forms, steps, warmup, batching=False):
     return sc, out_samples, contributions, results, plan
 
 
+def learned_leg(steps, warmup, chunks=2):
+    """Configuration 5: the learned warm start. The network is trained on the
+    device (1e7 pairs from the device's converged light-time solve), then the
+    scene prefix is rendered in the plain form (the reference's two fixed
+    iterations), the converged form (iterated to a 1e-13 s update from the
+    reference's start) and the learned form (the same, from the network's
+    prediction), reporting emission-equation evaluations per contribution."""
+    from lipika_amd import _lib, workloads
+    from lipika_amd.pipeline import Backend, WarmStartNet, form_code, prepare
+
+    seed = 2026
+    sc = prepare(workloads.scene5())
+    be = Backend(0)
+    lib = _lib.load()
+    be.upload(sc, seed)
+    net = WarmStartNet(be, init_seed=1)
+    train = net.train(n_pairs=10_000_000, data_seed=seed, epochs=16, batch=16384, lr=3e-3, lr_final=3e-5,
+                      n_val=200_000)
+    log(f"[bench] config 5: trained the warm-start network: {train}")
+    be.warm_start(net)
+    chunk = 1 << 20
+    bufs, jobs, keep = [], [], []
+    contributions = 0
+    w = 0
+    for r, rx in enumerate(sc.receivers):
+        for b, pb in enumerate(rx.bands):
+            n = min(pb.n, chunks * chunk)
+            dev = C.c_void_p()
+            be.check(lib.lpk_device_alloc(be.h, max(n, 1) * 8, C.byref(dev)))
+            bufs.append(dev)
+            for i0 in range(0, n, chunk):
+                j, k = be.job(r, b, w, seed, i0, min(chunk, n - i0), out_dev=dev.value + i0 * 8)
+                jobs.append(j)
+                keep.append(k)
+            contributions += n * len(pb.active)
+            w += 1
+    first = [j for j in jobs if j.i0 == 0]
+    forms = {}
+    for name in ("plain", "converged", "learned"):
+        form = form_code(name)
+        tuning, _ = be.autotune(first, form)
+
+        def step():
+            be.check(lib.lpk_clock_rewind(be.h))
+            be.render(jobs, form)
+
+        for _ in range(warmup):
+            step()
+        be.sync()
+        be.solve_stats_reset()
+        be.check(lib.lpk_timing_reset(be.h))
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        be.sync()
+        t1 = time.perf_counter()
+        kern_ms, n_launch = C.c_double(), C.c_uint64()
+        be.check(lib.lpk_timing_read(be.h, C.byref(kern_ms), C.byref(n_launch)))
+        ms = (t1 - t0) * 1e3 / steps
+        entry = {"value": round(contributions / (ms / 1e3) / 1e6, 3), "ms_per_step": round(ms, 4),
+                 "kernel_ms": round(kern_ms.value / max(n_launch.value, 1), 4), "tile_shift": tuning[0],
+                 "xcd_map": tuning[1]}
+        if name == "plain":
+            entry["evaluations_per_contribution"] = 2.0    # the reference's fixed two iterations
+        else:
+            st = be.solve_stats()
+            entry["evaluations_per_contribution"] = round(st["evaluations"] / max(st["contributions"], 1), 4)
+            entry["fallbacks"] = st["fallbacks"]
+            entry["unconverged"] = st["unconverged"]
+        log(f"[bench] config 5 form {name}: {ms:.3f} ms/step, {entry}")
+        forms[name] = entry
+    net.close()
+    for d in bufs:
+        be.check(lib.lpk_device_free(be.h, d))
+    be.close()
+    secs = chunks * chunk / 2.048e6
+    lf, cf, pf = forms["learned"], forms["converged"], forms["plain"]
+    return {"workload": "config 5: " + WORKLOADS[5].format(chunks=chunks, secs=secs),
+            "contributions_per_step": contributions, "steps": steps, "forms": forms,
+            "evaluations_saved_per_contribution": round(cf["evaluations_per_contribution"]
+                                                        - lf["evaluations_per_contribution"], 4),
+            "learned_speedup_vs_converged": round(cf["ms_per_step"] / lf["ms_per_step"], 3),
+            "learned_speedup_vs_plain": round(pf["ms_per_step"] / lf["ms_per_step"], 3),
+            "training": {"pairs": 10_000_000, "epochs": 16, "batch": 16384, "steps": train["steps"],
+                         "train_ms": round(train["train_ms"], 1), "pair_generation_ms": round(train["gen_ms"], 2),
+                         "validation_rms_s": train["val_rms_s"], "validation_max_s": train["val_max_s"]}}
+
+
 def cpu_leg(sc, seed, target_s):
     """The oracle (the project's C restatement of the reference CPU path,
     with its two-level parallelism) on a bounded prefix of the same scene."""
@@ -197,6 +290,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-config4", dest="config4", action="store_false",
                     help="skip the configuration-4 (batching) measurement")
+    ap.add_argument("--no-config5", dest="config5", action="store_false",
+                    help="skip the configuration-5 (learned warm start) measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -253,6 +348,8 @@ def main():
             cpu = c4["cpu_baseline"]["value"]
             c4["precision_vs_cpu"] = round(c4["forms"]["precision"]["value"] / cpu, 1)
         res["config"]["config4"] = c4
+    if args.config5:
+        res["config"]["config5"] = learned_leg(max(1, args.steps // 10), 1)
     print(json.dumps(res), flush=True)
 
 

@@ -246,6 +246,7 @@ TxDev make_tx(lpk_ctx *ctx, const lpk_tx_desc &d, uint32_t idx) {
             throw Fail{LPK_E_NONFINITE, "transmitter " + std::to_string(idx) + ": grid start/step"};
         T.grid_t0 = d.grid_t0_unix;
         T.grid_dt = d.grid_dt_s;
+        T.grid_inv_dt = 1.0 / d.grid_dt_s;
         T.grid_n = int64_t(d.grid_n);
         T.grid = static_cast<const double *>(upload(ctx, d.grid_xyz, 3 * d.grid_n * sizeof(double)));
     } else {
@@ -662,7 +663,14 @@ void render(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form,
         if (tiles > 0xFFFFFFFFull) throw Fail{LPK_E_INVALID, "render call holds more than 2^32 tiles"};
         J.link0 = uint32_t(link_off);
         J.pred0 = n_pred;
-        if (form == LPK_FORM_LEARNED && J.n_tiles) n_pred += uint64_t(B.n_active) * (uint64_t(J.n_tiles) + 1);
+        J.pred_qs = 1;
+        J.pred_nb = 0;
+        if (form == LPK_FORM_LEARNED && J.n_tiles) {
+            const double per_tile = double(1u << ts) / B.fs_hz;
+            J.pred_qs = uint32_t(std::max(1.0, std::floor(kLwsSpacing / per_tile)));
+            J.pred_nb = (J.n_tiles + J.pred_qs - 1) / J.pred_qs + 1;
+            n_pred += uint64_t(B.n_active) * J.pred_nb;
+        }
         if (tap)
             for (uint32_t s = 0; s < B.n_active; ++s)
                 if (perm[s] == tap->slot) tap_slot = s;

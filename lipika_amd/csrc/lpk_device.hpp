@@ -71,6 +71,7 @@ struct TxDev {
     int32_t signal_kind;     // lpk_signal_kind
     double pos[3];
     double grid_t0, grid_dt;
+    double grid_inv_dt;      // 1 / grid_dt (the converged form's grid coordinate)
     int64_t grid_n;
     const double *grid;      // (3, n)
     double carrier;
@@ -110,7 +111,12 @@ struct TxDev {
 // unrounded time axis, stopped at the first evaluation whose update is below
 // kSolveTol seconds. A learned start gets kLearnedEvals evaluations before it
 // falls back to the reference's start tu - 0.07 s with kColdEvals.
+// The learned start is predicted every kLwsSpacing seconds of reception time
+// (whole tiles, at least one) and interpolated linearly in between: the
+// light time's curvature (|d2tau/dt2| < 4e-7 s/s^2 even for a LEO overhead
+// pass) keeps that interpolation within 2e-13 s over 2 ms.
 constexpr double kSolveTol = 1e-13;
+constexpr double kLwsSpacing = 2e-3;
 constexpr int kLearnedEvals = 3;
 constexpr int kColdEvals = 12;
 
@@ -164,8 +170,10 @@ struct JobDev {
     double ant_peak, ant_horizon, ant_fb, ant_cutoff;
     uint32_t *unocc;         // per link of the job: set to 1 where some sample's gain is non-zero
     // learned form: the job's predicted light times start at pred[pred0], one
-    // row of n_tiles + 1 tile-boundary values per link
+    // row of pred_nb values per link, at every pred_qs-th tile boundary
+    // (the last clamped to the job's end)
     uint64_t pred0;
+    uint32_t pred_qs, pred_nb;
 };
 
 // Philox4x64-10 (Salmon, Moraes, Dror, Shaw, SC'11).

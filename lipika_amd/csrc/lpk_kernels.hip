@@ -187,7 +187,9 @@ __device__ __forceinline__ void solve_emission(const TxDev &T, const double *obs
 
 // PrecomputedTrajectory::position_ecef (src/geometry/precomputed.rs:108-121)
 // at time trel after the grid's node 0, given directly instead of as an
-// absolute time, so it is not rounded at the scale of the epoch.
+// absolute time, so it is not rounded at the scale of the epoch, and scaled
+// by 1 / dt instead of divided by dt (one rounding more in the grid
+// coordinate: under 1e-10 m of position).
 __device__ __forceinline__ void tx_position_rel(const TxDev &T, double trel, double &x, double &y, double &z) {
     if (T.traj_kind == 0) {
         x = T.pos[0];
@@ -195,7 +197,7 @@ __device__ __forceinline__ void tx_position_rel(const TxDev &T, double trel, dou
         z = T.pos[2];
         return;
     }
-    const double f = trel / T.grid_dt;
+    const double f = trel * T.grid_inv_dt;
     const double flo = fmin(fmax(floor(f), 0.0), static_cast<double>(T.grid_n - 2));
     const int64_t lo = static_cast<int64_t>(flo);
     double frac = f - flo;
@@ -208,24 +210,49 @@ __device__ __forceinline__ void tx_position_rel(const TxDev &T, double trel, dou
     z = g[2 * n + lo] * omf + g[2 * n + lo + 1] * frac;
 }
 
-// The converged solve: the reference's iteration (src/channel/mod.rs:75-96)
-// from emission time t, with positions at eoff + t after node 0 (eoff = scene
-// epoch - grid start), until an update |t_next - t| < kSolveTol. Returns the
-// number of evaluations, or -1 when max_evals did not converge; t_emit and r
-// are the last iterate and the range it came from (r / c = tu - t_emit).
+// One evaluation of the reference's iteration (src/channel/mod.rs:75-96) at
+// emission time t, with positions at eoff + t after the grid's node 0 (eoff =
+// scene epoch - grid start): returns tu - r / c and sets r.
+__device__ __forceinline__ double emission_step(const TxDev &T, const double *obs, double tu, double eoff, double t,
+                                                double &r) {
+    double sx, sy, sz;
+    tx_position_rel(T, eoff + t, sx, sy, sz);
+    const double wtau = kEarthRotation * (tu - t);
+    const double sx_rot = sx + wtau * sy;
+    const double sy_rot = sy - wtau * sx;
+    const double dx = sx_rot - obs[0];
+    const double dy = sy_rot - obs[1];
+    const double dz = sz - obs[2];
+    r = sqrt(dx * dx + dy * dy + dz * dz);
+    return tu - r * kInvC;
+}
+
+// The converged solve from emission time t: iterate until an update
+// |t_next - t| < kSolveTol. Returns the number of evaluations, or -1 when
+// max_evals did not converge; t_emit and r are the last iterate and the range
+// it came from (r / c = tu - t_emit). The first K evaluations run back to back
+// and their updates are tested afterwards, in order, so the result is that of
+// the plain loop (which stops at the first small update) while the common
+// case is straight-line code.
+template <int K>
 __device__ __forceinline__ int solve_converged(const TxDev &T, const double *obs, double tu, double eoff, double t,
                                                int max_evals, double &t_emit, double &r) {
-    for (int e = 1; e <= max_evals; ++e) {
-        double sx, sy, sz;
-        tx_position_rel(T, eoff + t, sx, sy, sz);
-        const double wtau = kEarthRotation * (tu - t);
-        const double sx_rot = sx + wtau * sy;
-        const double sy_rot = sy - wtau * sx;
-        const double dx = sx_rot - obs[0];
-        const double dy = sy_rot - obs[1];
-        const double dz = sz - obs[2];
-        r = sqrt(dx * dx + dy * dy + dz * dz);
-        const double tn = tu - r * kInvC;
+    double tk[K + 1], rk[K];
+    tk[0] = t;
+#pragma unroll
+    for (int k = 0; k < K; ++k) tk[k + 1] = emission_step(T, obs, tu, eoff, tk[k], rk[k]);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        if (fabs(tk[k + 1] - tk[k]) < kSolveTol) {
+            t_emit = tk[k + 1];
+            r = rk[k];
+            return k + 1;
+        }
+    }
+    t = tk[K];
+    r = rk[K - 1];
+    for (int e = K + 1; e <= max_evals; ++e) {
+        const double tn = emission_step(T, obs, tu, eoff, t, r);
         const double step = tn - t;
         t = tn;
         if (fabs(step) < kSolveTol) {
@@ -605,7 +632,7 @@ struct RenderLaunch {
 };
 
 template <int FORM, bool ANT, int TS>
-__global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM == 0 ? 6 : 5))
+__global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM == 1 ? 5 : 6))
     render_kernel(const JobDev *__restrict__ jobs, int n_jobs, const LinkDev *__restrict__ links,
                   const TxDev *__restrict__ txs, RenderLaunch lx, TapOut tap) {
     constexpr int kTile = 1 << TS;
@@ -656,13 +683,20 @@ __global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM == 0 ? 6 : 5))
         // places them)
         const double *prow = nullptr;
         double frac = 0.0;
-        uint32_t n_evals = 0, n_fallback = 0, n_unconv = 0;
+        // solve counters of this wave, wave-uniform (scalar registers):
+        // evaluations beyond `nominal` per contribution, fallbacks, unconverged
+        int64_t w_extra = 0;
+        uint32_t w_fb = 0, w_unc = 0;
         if (FORM == 2 && lx.pred) {
+            const uint32_t bq = (tile - J.tile0) / J.pred_qs;   // the query boundary at or before this tile
             const uint64_t last = J.i0 + J.clen - 1;
-            const uint64_t ib0 = tile_start < J.i0 ? J.i0 : tile_start;
-            const uint64_t ib1 = tile_start + kTile > last ? last : tile_start + kTile;
+            const uint64_t base = J.i0 >> TS;
+            uint64_t ib0 = (base + uint64_t(bq) * J.pred_qs) << TS;
+            uint64_t ib1 = (base + uint64_t(bq + 1) * J.pred_qs) << TS;
+            ib0 = ib0 < J.i0 ? J.i0 : ib0;
+            ib1 = ib1 > last ? last : ib1;
             frac = ib1 > ib0 ? static_cast<double>(ic - ib0) / static_cast<double>(ib1 - ib0) : 0.0;
-            prow = lx.pred + J.pred0 + (tile - J.tile0);
+            prow = lx.pred + J.pred0 + bq;
         }
         for (uint32_t s = 0; s < J.n_active; ++s) {
             const LinkDev L = links[J.link0 + s];
@@ -674,21 +708,34 @@ __global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM == 0 ? 6 : 5))
             } else {
                 const double eoff = T.traj_kind ? J.epoch - T.grid_t0 : 0.0;
                 int e = -1;
+                uint32_t used = 0;   // evaluations before the reference's start
                 if (prow) {
-                    const double *pp = prow + static_cast<uint64_t>(s) * (J.n_tiles + 1u);
+                    const double *pp = prow + static_cast<uint64_t>(s) * J.pred_nb;
                     const double ta = pp[0], tb = pp[1];
-                    e = solve_converged(T, J.obs, tu, eoff, tu - (ta + (tb - ta) * frac), kLearnedEvals, t_emit, r);
-                    if (e < 0) {
-                        n_evals += kLearnedEvals;
-                        ++n_fallback;
-                    }
+                    e = solve_converged<2>(T, J.obs, tu, eoff, tu - (ta + (tb - ta) * frac), kLearnedEvals, t_emit, r);
+                    if (e < 0) used = kLearnedEvals;
                 }
-                if (e < 0) e = solve_converged(T, J.obs, tu, eoff, tu - kEmissionInitialLag, kColdEvals, t_emit, r);
-                if (e < 0) {
-                    n_evals += kColdEvals;
-                    ++n_unconv;
-                } else {
-                    n_evals += static_cast<uint32_t>(e);
+                // the reference's start: three evaluations unrolled (the usual count
+                // from it), one at a time on a learned start's rare fallback
+                if (e < 0)
+                    e = prow ? solve_converged<1>(T, J.obs, tu, eoff, tu - kEmissionInitialLag, kColdEvals, t_emit, r)
+                             : solve_converged<3>(T, J.obs, tu, eoff, tu - kEmissionInitialLag, kColdEvals, t_emit, r);
+                if (lx.stats) {
+                    // lanes whose count differs from the nominal one, counted by value
+                    const uint32_t nominal = prow ? 2u : 3u;
+                    const uint32_t ev = used + (e < 0 ? static_cast<uint32_t>(kColdEvals) : static_cast<uint32_t>(e));
+                    const bool dev = valid && ev != nominal;
+                    uint64_t m = __ballot(dev);
+                    if (m) {
+                        w_fb += __popcll(__ballot(valid && used != 0));
+                        w_unc += __popcll(__ballot(valid && e < 0));
+                        while (m) {
+                            const uint32_t v = __builtin_amdgcn_readlane(ev, __ffsll(static_cast<long long>(m)) - 1);
+                            const uint64_t mm = __ballot(dev && ev == v);
+                            w_extra += (static_cast<int64_t>(v) - static_cast<int64_t>(nominal)) * __popcll(mm);
+                            m &= ~mm;
+                        }
+                    }
                 }
             }
             // src/channel/mod.rs:255-267
@@ -716,17 +763,17 @@ __global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM == 0 ? 6 : 5))
             acc_im = acc_im + amplitude * (bre * spf + bim * cpf);
         }
         if (FORM == 2 && lx.stats) {
-            // per-wave sums, one LDS add per wave, one global add per workgroup
+            // one LDS add per wave and counter, one global add per workgroup
             if (threadIdx.x < 4) s_stats[threadIdx.x] = 0;
             __syncthreads();
-            auto wave_add = [&](uint32_t x, int k) {
-                for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o);
-                if ((threadIdx.x & 63) == 0 && x) atomicAdd(&s_stats[k], static_cast<unsigned long long>(x));
-            };
-            wave_add(valid ? n_evals : 0u, 0);
-            wave_add(valid ? J.n_active : 0u, 1);
-            wave_add(valid ? n_fallback : 0u, 2);
-            wave_add(valid ? n_unconv : 0u, 3);
+            const uint64_t contrib = static_cast<uint64_t>(__popcll(__ballot(valid))) * J.n_active;
+            const uint64_t evals = (prow ? 2u : 3u) * contrib + static_cast<uint64_t>(w_extra);
+            if ((threadIdx.x & 63) == 0) {
+                if (evals) atomicAdd(&s_stats[0], static_cast<unsigned long long>(evals));
+                if (contrib) atomicAdd(&s_stats[1], static_cast<unsigned long long>(contrib));
+                if (w_fb) atomicAdd(&s_stats[2], static_cast<unsigned long long>(w_fb));
+                if (w_unc) atomicAdd(&s_stats[3], static_cast<unsigned long long>(w_unc));
+            }
             __syncthreads();
             if (threadIdx.x < 4 && s_stats[threadIdx.x]) atomicAdd(&lx.stats[threadIdx.x], s_stats[threadIdx.x]);
         }
@@ -900,9 +947,9 @@ __global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM == 0 ? 6 : 5))
 
 // ---------------------------------------------------------------- learned start
 
-// Predicted light time at every tile boundary of every link of a render call
-// (query q = pred0 + link * (n_tiles + 1) + boundary of its job); boundary b
-// sits at sample ((i0 >> TS) + b) << TS clamped into the job. The prediction is
+// Predicted light time at every pred_qs-th tile boundary of every link of a
+// render call (query q = pred0 + link * pred_nb + j of its job); query j sits
+// at sample ((i0 >> TS) + j * pred_qs) << TS clamped into the job. The prediction is
 // the straight-line light time at that sample's reception time plus the
 // network's correction (none for a trajectory without orbital elements).
 // 128 queries per workgroup, 32 per wave on the matrix cores.
@@ -922,10 +969,10 @@ __global__ void __launch_bounds__(256) lws_predict_kernel(const JobDev *__restri
         else hi = mid - 1;
     }
     const JobDev &J = jobs[lo];
-    const uint64_t pitch = uint64_t(J.n_tiles) + 1;
+    const uint64_t pitch = J.pred_nb;
     const uint64_t local = qc - J.pred0;
     const uint64_t s = local / pitch;
-    const uint64_t b = local - s * pitch;
+    const uint64_t b = (local - s * pitch) * J.pred_qs;
     const uint64_t last = J.i0 + J.clen - 1;
     uint64_t ib = ((J.i0 >> tile_shift) + b) << tile_shift;
     ib = ib < J.i0 ? J.i0 : (ib > last ? last : ib);

@@ -84,14 +84,21 @@ struct LwsNetDev {
 
 __device__ __forceinline__ int lws_row(int j, int h) { return 8 * (j >> 2) + 4 * h + (j & 3); }
 
+// tanh(u) = sign(u) (1 - e) / (1 + e) with e = 2^(-2 |u| log2 e), on the
+// hardware exp2 and reciprocal (a few ulp; absolute error < 3e-7).
+__device__ __forceinline__ float lws_tanh(float u) {
+    const float e = __builtin_amdgcn_exp2f(-2.8853900817779268f * fabsf(u));
+    return copysignf((1.0f - e) * __builtin_amdgcn_rcpf(1.0f + e), u);
+}
+
 __device__ __forceinline__ float lws_gelu(float z) {
     const float u = kGeluK0 * (z + kGeluK1 * z * z * z);
-    return 0.5f * z * (1.0f + tanhf(u));
+    return 0.5f * z * (1.0f + lws_tanh(u));
 }
 
 __device__ __forceinline__ float lws_gelu_grad(float z) {
     const float u = kGeluK0 * (z + kGeluK1 * z * z * z);
-    const float th = tanhf(u);
+    const float th = lws_tanh(u);
     return 0.5f * (1.0f + th) + 0.5f * z * (1.0f - th * th) * kGeluK0 * (1.0f + 3.0f * kGeluK1 * z * z);
 }
 

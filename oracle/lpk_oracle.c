@@ -165,7 +165,8 @@ static void tx_position(const orc_tx *tx, double t_unix, double out[3]) {
 
 /* The converged solve: the iteration of src/channel/mod.rs:75-96 from emission
  * time t_start, with the grid queried at (epoch - grid_t0) + t after its node 0
- * (src/geometry/precomputed.rs:108-121 without rounding the absolute time),
+ * (src/geometry/precomputed.rs:108-121 without rounding the absolute time, and
+ * with the grid coordinate scaled by 1 / dt rather than divided by dt),
  * until an update |t_next - t| < tol. Returns the evaluations used, or -1 if
  * max_evals did not converge; *t_emit = last iterate, *range = the range it
  * came from. */
@@ -173,6 +174,7 @@ int orc_solve_converged(const orc_tx *tx, const double obs[3], double tu, double
                         double tol, int max_evals, double *t_emit, double *range) {
     const double inv_c = 1.0 / SPEED_OF_LIGHT_M_S;
     const double eoff = tx->traj_kind == ORC_TRAJ_STATIONARY ? 0.0 : epoch - tx->grid_t0_unix;
+    const double inv_dt = tx->traj_kind == ORC_TRAJ_STATIONARY ? 0.0 : 1.0 / tx->grid_dt_s;
     double t = t_start, r = 0.0;
     for (int e = 1; e <= max_evals; ++e) {
         double s[3];
@@ -181,7 +183,7 @@ int orc_solve_converged(const orc_tx *tx, const double obs[3], double tu, double
             s[1] = tx->pos_ecef[1];
             s[2] = tx->pos_ecef[2];
         } else {
-            grid_lerp((eoff + t) / tx->grid_dt_s, tx->grid_n, tx->grid_xyz, s);
+            grid_lerp((eoff + t) * inv_dt, tx->grid_n, tx->grid_xyz, s);
         }
         double wtau = EARTH_ROTATION_RATE_RAD_S * (tu - t);
         double sx_rot = s[0] + wtau * s[1];
