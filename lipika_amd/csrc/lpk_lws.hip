@@ -289,19 +289,27 @@ struct AdamArgs {
     float c1, c2;   // 1 / (1 - beta1^t), 1 / (1 - beta2^t)
 };
 
-// Parameter p: g = sum over the n_rows slice rows in row order. With grad_out
-// set, g is written there and nothing is updated; otherwise Adam updates the
-// master value, its moments and the parameter's BF16 operand copy. The loss
-// column goes to *loss_out when that is set.
+// 64 parameters per workgroup: lane group q (of 4) sums rows q, q + 4, ...
+// of its parameter in row order, and the four partial sums are added in
+// group order, so g is the same on every run. With grad_out set, g is
+// written there and nothing is updated; otherwise Adam updates the master
+// value, its moments and the parameter's BF16 operand copy. The loss column
+// goes to *loss_out when that is set.
 __global__ void __launch_bounds__(256) lws_update_kernel(const float *__restrict__ partials, int n_rows,
                                                          float *__restrict__ theta, float *__restrict__ m1,
                                                          float *__restrict__ m2, __bf16 *__restrict__ packed,
                                                          AdamArgs a, float *__restrict__ grad_out,
                                                          float *__restrict__ loss_out) {
-    const int p = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    if (p >= kLwsRow) return;
-    float g = 0.0f;
-    for (int r = 0; r < n_rows; ++r) g += partials[size_t(r) * kLwsRow + p];
+    __shared__ float s_part[4][64];
+    const int pl = static_cast<int>(threadIdx.x & 63), q = static_cast<int>(threadIdx.x >> 6);
+    const int p = static_cast<int>(blockIdx.x) * 64 + pl;
+    float acc = 0.0f;
+    if (p < kLwsRow)
+        for (int r = q; r < n_rows; r += 4) acc += partials[size_t(r) * kLwsRow + p];
+    s_part[q][pl] = acc;
+    __syncthreads();
+    if (q != 0 || p >= kLwsRow) return;
+    const float g = ((s_part[0][pl] + s_part[1][pl]) + s_part[2][pl]) + s_part[3][pl];
     if (p == kLwsParams) {
         if (loss_out) *loss_out = g;
         return;
@@ -375,7 +383,7 @@ hipError_t launch_lws_update(const float *partials, int n_rows, float *theta, fl
                              float lr, float beta1, float beta2, float eps, float c1, float c2, float *grad_out,
                              float *loss_out, hipStream_t s) {
     const AdamArgs a{lr, beta1, beta2, eps, c1, c2};
-    hipLaunchKernelGGL(lws_update_kernel, dim3((kLwsRow + 255) / 256), dim3(256), 0, s, partials, n_rows, theta,
+    hipLaunchKernelGGL(lws_update_kernel, dim3((kLwsRow + 63) / 64), dim3(256), 0, s, partials, n_rows, theta,
                        m1, m2, static_cast<__bf16 *>(packed), a, grad_out, loss_out);
     return hipGetLastError();
 }

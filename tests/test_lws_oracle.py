@@ -150,3 +150,79 @@ def test_features_are_unit_scale():
     X = L.features(obs, a, inc, node, u)
     assert X.dtype == np.float32 and X.shape == (1000, 10)
     assert np.abs(X).max() <= 1.01
+
+
+# ---------------------------------------------------------------- converged solve (orc_solve_converged)
+
+def _grid_tx(orc, orbit, t0, dur, dt=0.01):
+    n = orc.orc_grid_size(dur, dt)
+    xyz = np.empty(3 * n)
+    t0o = C.c_double()
+    orc.orc_grid_sample_kepler(*orbit, t0, dur, dt, C.byref(t0o), xyz.ctypes.data_as(C.POINTER(C.c_double)))
+    tx = pyoracle.OrcTx()
+    tx.traj_kind, tx.grid_t0_unix, tx.grid_dt_s, tx.grid_n = 1, t0o.value, dt, n
+    tx.grid_xyz = xyz.ctypes.data_as(C.POINTER(C.c_double))
+    return tx, xyz
+
+
+def _solve(orc, tx, obs, tu, epoch, start, max_evals):
+    te, r = C.c_double(), C.c_double()
+    o = (C.c_double * 3)(*obs)
+    e = orc.orc_solve_converged(C.byref(tx), o, tu, epoch, start, 1e-13, max_evals, C.byref(te), C.byref(r))
+    return e, te.value, r.value
+
+
+def test_converged_solve_is_the_reference_iteration(orc):
+    """Cut at two evaluations from t - 0.07 s, the converged solve is the
+    reference's two-iteration map bit for bit (a stationary source, where the
+    absolute-time rounding cannot differ)."""
+    tx = pyoracle.OrcTx()
+    tx.traj_kind = 0
+    tx.pos_ecef[:] = [6378137.0, 1000.0, 1000.0]
+    obs = [6378137.0, 0.0, 0.0]
+    for tu in (0.0, 0.37, 12.5):
+        e, te, r = _solve(orc, tx, obs, tu, 1717260840.0, tu - 0.07, 2)
+        te_ref, r_ref = C.c_double(), C.c_double()
+        orc.orc_solve_emission(C.byref(tx), (C.c_double * 3)(*obs), tu, 1717260840.0, 2, C.byref(te_ref),
+                               C.byref(r_ref))
+        assert e == -1                      # two evaluations do not reach 1e-13 s from 0.07 s
+        assert te == te_ref.value and r == r_ref.value
+
+
+def test_converged_solve_finds_the_root_found_by_bisection(orc):
+    """The converged emission time solves t + |R(w (tu - t)) P(t) - O| / c = tu
+    with P the grid lerp, against a bisection of that equation; scene epoch 0
+    so the absolute query times are exact."""
+    rng = np.random.default_rng(23)
+    for k in range(12):
+        a = 6.9e6 if k % 2 == 0 else 2.66e7
+        orbit = (a, float(rng.uniform(0, 180)), float(rng.uniform(0, 360)), float(rng.uniform(0, 360)), 0.0)
+        tx, keep = _grid_tx(orc, orbit, 0.0, 20.0)
+        lat, lon = rng.uniform(-60, 60), rng.uniform(-180, 180)
+        o = (C.c_double * 3)()
+        orc.orc_geodetic_to_ecef(lat, lon, 50.0, o)
+        obs = list(o[:])
+        tu = float(rng.uniform(1.0, 19.0))
+        e, te, r = _solve(orc, tx, obs, tu, 0.0, tu - 0.07, 12)
+        assert 2 <= e <= 5
+        assert abs((tu - te) - r / L.C_LIGHT) <= 2 * math.ulp(tu)   # t_emit = tu - r / c, rounded
+
+        def g(t):
+            p = (C.c_double * 3)()
+            orc.orc_grid_position(tx.grid_t0_unix, tx.grid_dt_s, tx.grid_n, keep.ctypes.data_as(C.POINTER(C.c_double)),
+                                  t, p)
+            w = L.OMEGA_E * (tu - t)
+            d = np.array([p[0] + w * p[1], p[1] - w * p[0], p[2]]) - np.array(obs)
+            return t + math.sqrt(float(d @ d)) / L.C_LIGHT - tu
+
+        lo, hi = tu - 0.2, tu
+        assert g(lo) < 0 < g(hi)
+        for _ in range(80):
+            mid = 0.5 * (lo + hi)
+            lo, hi = (mid, hi) if g(mid) < 0 else (lo, mid)
+        assert abs(te - 0.5 * (lo + hi)) <= 4e-15, (k, te - 0.5 * (lo + hi))
+        # the straight two-iteration map stops short of it
+        te2 = C.c_double()
+        r2 = C.c_double()
+        orc.orc_solve_emission(C.byref(tx), o, tu, 0.0, 2, C.byref(te2), C.byref(r2))
+        assert te2.value != te or k % 2 == 1
