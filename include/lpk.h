@@ -124,7 +124,7 @@ typedef struct {
 } lpk_tx_desc;
 
 /* One stationary receiver with its clock (src/pipeline.rs:344-364,
- * NodeClockSpec src/clock/mod.rs:20-46 without flicker). */
+ * NodeClockSpec src/clock/mod.rs:20-46). */
 typedef struct {
     double pos_ecef[3];
     double allan_deviation_1s;   /* white-FM sigma_y(1 s); 0 = no walk */
@@ -132,6 +132,7 @@ typedef struct {
     double initial_time_offset_s;
     double master_rate_hz;       /* highest band rate of the receiver, src/pipeline.rs:864-866 */
     uint64_t clock_seed;         /* src/pipeline.rs:825-827 */
+    double flicker_floor;        /* flicker-FM Allan floor (src/clock/mod.rs:26-30); 0 = none */
 } lpk_rx_desc;
 
 typedef struct {

@@ -62,7 +62,7 @@ class RxDesc(C.Structure):
         ("pos_ecef", _d * 3),
         ("allan_deviation_1s", _d), ("frequency_offset_y0", _d),
         ("initial_time_offset_s", _d), ("master_rate_hz", _d),
-        ("clock_seed", _u64),
+        ("clock_seed", _u64), ("flicker_floor", _d),
     ]
 
 

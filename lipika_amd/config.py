@@ -3,8 +3,8 @@
 fields the IQ path consumes.
 
 Parts of the schema the GPU path does not implement raise ``NotImplementedError``
-when a scene uses them (TLE and RINEX trajectories, full LNAV, flicker FM,
-moving receivers); nothing is silently dropped.
+when a scene uses them (TLE and RINEX trajectories, full LNAV, moving
+receivers); nothing is silently dropped.
 """
 from __future__ import annotations
 

@@ -81,8 +81,23 @@ struct PinSlot {
     hipEvent_t done = nullptr;
 };
 
+// Flicker FM of one receiver (src/clock/mod.rs:55-149). Its 128 Hz series is
+// small (7 680 steps per minute), so it is stepped here on the host, with the
+// device's Philox4x64-10 / Box-Muller normals, and the render kernels
+// interpolate its integral D[k] (uploaded to `dev`).
+struct FlickerGen {
+    bool on = false;
+    uint64_t seed = 0;
+    double alpha[kFlickerPoles] = {}, gain[kFlickerPoles] = {}, state[kFlickerPoles] = {};
+    double scale = 0.0;
+    std::vector<double> D;      // D[k] = sum of the steps 0..k of y / 128
+    DevBuf dev;
+    size_t dev_n = 0;           // entries of D uploaded
+};
+
 struct RxState {
     lpk_rx_desc d{};
+    FlickerGen flick;
     int walk = 0;
     double sigma_step = 0.0;
     double *base = nullptr;     // base[b] = eps(b*kWalkBlock - 1), base[0] = 0
@@ -207,6 +222,7 @@ void free_scene(lpk_ctx *ctx) {
         r.base = nullptr;
         r.local.release();
         r.toff.release();
+        r.flick.dev.release();
     }
     ctx->rx.clear();
     ctx->tx_host.clear();
@@ -217,6 +233,78 @@ void free_scene(lpk_ctx *ctx) {
 }
 
 bool is_fin(double x) { return std::isfinite(x); }
+
+// Normal n of a receiver's flicker stream: Philox4x64-10 block n >> 3 of
+// (clock seed, "flickern"), word (n >> 1) & 3, the cos branch of its
+// Box-Muller pair for even n and the sin branch for odd n. Step k of pole j
+// uses n = (k + 1) * 20 + j; n < 20 start the poles.
+double flicker_normal(uint64_t seed, uint64_t n) {
+    uint64_t w[4] = {n >> 3, 0, 0, 0};
+    philox4x64(w, seed, kFlickerTag);
+    float zc, zs;
+    box_muller(w[(n >> 1) & 3], zc, zs);
+    return static_cast<double>((n & 1) ? zs : zc);
+}
+
+// The AR(1) poles of src/clock/mod.rs:71-80 and the Allan deviation of their
+// unit-scaled sum at tau = m / 128 s: for a pole a (autocorrelation a^|d|,
+// unit variance) the non-overlapping estimator of src/clock/mod.rs:150-163 has
+// expectation (E[S1^2] - E[S1 S2]) / m^2, where S1 and S2 are sums over two
+// adjacent blocks of m steps:
+//   E[S1^2]   = m + 2 sum_{d=1}^{m-1} (m - d) a^d,
+//   E[S1 S2]  = sum_{d=1}^{m} d a^d + sum_{d=m+1}^{2m-1} (2m - d) a^d.
+// The reference measures that deviation once on a random realisation; the
+// expectation makes the calibration exact and the same on every run.
+void flicker_poles(double *alpha, double *gain) {
+    const double f_hi = kFlickerRate / 2.0;
+    for (int j = 0; j < kFlickerPoles; ++j) {
+        const double f = kFlickerFLo * std::pow(f_hi / kFlickerFLo, double(j) / double(kFlickerPoles - 1));
+        const double a = std::exp(-2.0 * 3.141592653589793 * f / kFlickerRate);
+        alpha[j] = a;
+        gain[j] = std::sqrt(1.0 - a * a);
+    }
+}
+
+double flicker_unit_adev(const double *alpha, int m) {
+    double var = 0.0;
+    for (int j = 0; j < kFlickerPoles; ++j) {
+        const double a = alpha[j];
+        double s11 = double(m), s12 = 0.0, p = 1.0;
+        for (int d = 1; d < 2 * m; ++d) {
+            p *= a;
+            if (d < m) s11 += 2.0 * double(m - d) * p;
+            s12 += (d <= m ? double(d) : double(2 * m - d)) * p;
+        }
+        var += (s11 - s12) / (double(m) * double(m));
+    }
+    return std::sqrt(var);
+}
+
+// Flicker::new without the burn-in: each pole starts from its stationary
+// distribution N(0, 1), which the reference approaches by 200 000 warm-up steps.
+void flicker_init(FlickerGen &F, double floor_adev, uint64_t seed) {
+    F.on = floor_adev > 0.0;
+    F.D.clear();
+    F.dev_n = 0;
+    if (!F.on) return;
+    F.seed = seed;
+    flicker_poles(F.alpha, F.gain);
+    F.scale = floor_adev / flicker_unit_adev(F.alpha, kFlickerCalibM);
+    for (int j = 0; j < kFlickerPoles; ++j) F.state[j] = flicker_normal(seed, uint64_t(j));
+}
+
+// Flicker::advance_to (src/clock/mod.rs:125-133) until D holds index k_hi.
+void flicker_extend(FlickerGen &F, int64_t k_hi) {
+    for (int64_t k = int64_t(F.D.size()); k <= k_hi; ++k) {
+        double y = 0.0;
+        for (int j = 0; j < kFlickerPoles; ++j) {
+            F.state[j] = F.alpha[j] * F.state[j] + F.gain[j] * flicker_normal(F.seed, uint64_t(k + 1) * kFlickerPoles + j);
+            y += F.state[j];
+        }
+        y = y * F.scale;
+        F.D.push_back((k ? F.D[size_t(k - 1)] : 0.0) + y / kFlickerRate);
+    }
+}
 
 LwsNetDev lws_net(const lpk_lws *net) {
     return LwsNetDev{static_cast<const bf16x8 *>(net->packed), net->theta};
@@ -359,6 +447,24 @@ void *pinned_slot(lpk_ctx *ctx, size_t bytes) {
 }
 void pinned_mark(lpk_ctx *ctx, int slot_used) {
     hip_check(hipEventRecord(ctx->pin[slot_used].done, ctx->stream), "event record");
+}
+
+// Extend a receiver's flicker integral to cover samples up to (end) / fs of a
+// band and upload it; returns the device table (null without flicker).
+const double *ensure_flicker(lpk_ctx *ctx, RxState &R, uint64_t end, double fs) {
+    FlickerGen &F = R.flick;
+    if (!F.on) return nullptr;
+    const int64_t k_hi = int64_t(std::floor(double(end) / fs * kFlickerRate)) + 2;
+    if (int64_t(F.D.size()) <= k_hi || !F.dev.p) {
+        // the previous upload may still read D; it is about to grow
+        hip_check(hipStreamSynchronize(ctx->stream), "flicker table");
+        flicker_extend(F, ((k_hi >> 10) + 1) << 10);
+        void *d = F.dev.ensure(F.D.size() * sizeof(double), ctx->stream);
+        hip_check(hipMemcpyAsync(d, F.D.data(), F.D.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream),
+                  "flicker table upload");
+        F.dev_n = F.D.size();
+    }
+    return static_cast<const double *>(F.dev.p);
 }
 
 // Grow a receiver's walk base table to hold base[0 .. b_hi + 1].
@@ -657,6 +763,7 @@ void render(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form,
         const lpk_band_chunk &B = jobs[q];
         const RxState &R = ctx->rx[B.rx];
         JobDev J = fill_job(ctx, B, B.i0, B.clen, form, lh + link_off, perm);
+        J.flicker = B.delta_t ? nullptr : ensure_flicker(ctx, ctx->rx[B.rx], B.i0 + B.clen, B.fs_hz);
         J.tile0 = uint32_t(tiles);
         J.n_tiles = tiles_of(B.i0, B.clen, ts);
         tiles += J.n_tiles;
@@ -833,6 +940,7 @@ void plan_fill(lpk_plan *P, uint64_t k, char *dst) {
         uint64_t i0, clen;
         plan_chunk(P, b, k, i0, clen);
         JobDev J = fill_job(ctx, B, i0, clen, P->form, lh + P->link0[b], perm);
+        J.flicker = B.delta_t ? nullptr : static_cast<const double *>(ctx->rx[B.rx].flick.dev.p);
         J.tile0 = uint32_t(tiles);
         J.n_tiles = tiles_of(i0, clen, P->tile_shift);
         tiles += J.n_tiles;
@@ -905,6 +1013,9 @@ lpk_plan *plan_create(lpk_ctx *ctx, const lpk_band_chunk *bands, uint32_t n_band
         P->link0[b] = P->total_links;
         P->total_links += P->bands[b].n_active;
     }
+    // flicker tables over every band's whole extent (steps only read them)
+    for (const lpk_band_chunk &B : P->bands)
+        if (!B.delta_t) ensure_flicker(ctx, ctx->rx[B.rx], B.i0 + B.clen, B.fs_hz);
     // the receivers the plan walks, and a dry run of every step for capacities
     P->walk_slot.assign(ctx->rx.size(), -1);
     for (const auto &B : P->bands)
@@ -1091,9 +1202,11 @@ int lpk_scene_upload(lpk_ctx *ctx, const lpk_tx_desc *tx, uint32_t n_tx, const l
             if (!is_fin(d.allan_deviation_1s) || d.allan_deviation_1s < 0.0 || !is_fin(d.frequency_offset_y0) ||
                 !is_fin(d.initial_time_offset_s))
                 throw Fail{LPK_E_NONFINITE, "receiver clock parameters"};
+            if (!is_fin(d.flicker_floor) || d.flicker_floor < 0.0) throw Fail{LPK_E_NONFINITE, "receiver flicker floor"};
             RxState &R = ctx->rx[r];
             R.d = d;
             R.walk = d.allan_deviation_1s > 0.0;  // src/clock/mod.rs:274-279
+            flicker_init(R.flick, d.flicker_floor, d.clock_seed);
             R.sigma_step = R.walk ? d.allan_deviation_1s / std::sqrt(d.master_rate_hz) : 0.0;
         }
         ctx->have_scene = true;

@@ -25,6 +25,15 @@ constexpr int kNewtonIterations = 2;                    // src/pipeline.rs:56
 // noise stream drawn with the same seed.
 constexpr uint64_t kClockTag = 0x636C6F636B77616CULL;   // "clockwal"
 constexpr uint64_t kAwgnTag = 0x6177676E6E6F6973ULL;    // "awgnnois"
+constexpr uint64_t kFlickerTag = 0x666C69636B65726EULL; // "flickern"
+
+// Flicker FM (src/clock/mod.rs:55-149): a sum of kFlickerPoles AR(1) processes
+// log-spaced in frequency from kFlickerFLo to half of kFlickerRate, stepped on
+// a kFlickerRate coarse grid; its integral D[k] is interpolated linearly.
+constexpr double kFlickerRate = 128.0;                  // src/clock/mod.rs:67
+constexpr int kFlickerPoles = 20;                       // src/clock/mod.rs:68
+constexpr double kFlickerFLo = 1e-4;                    // src/clock/mod.rs:69
+constexpr int kFlickerCalibM = 256;                     // tau = 2 s (src/clock/mod.rs:106)
 
 // Clock walk: blocks of kWalkBlock master-grid increments; each of the
 // kWalkThreads threads of a block owns kWalkPerThread consecutive ones.
@@ -156,6 +165,7 @@ struct JobDev {
     const float *walk_local; // per index of the window
     int64_t eps_k0;          // first master index of the window (block aligned)
     double c0, y0, master;
+    const double *flicker;   // flicker integral D[k] on the 128 Hz grid, or null
     int32_t walk;
     // AWGN and post-sum
     int32_t n_post;
@@ -176,8 +186,26 @@ struct JobDev {
     uint32_t pred_qs, pred_nb;
 };
 
+// High 64 bits of a 64 x 64-bit product, and count of leading zeros, on the
+// device's instructions or the host's.
+__host__ __device__ __forceinline__ uint64_t mulhi64(uint64_t a, uint64_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umul64hi(a, b);
+#else
+    return static_cast<uint64_t>((static_cast<unsigned __int128>(a) * b) >> 64);
+#endif
+}
+
+__host__ __device__ __forceinline__ int clz32(uint32_t x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __clz(static_cast<int>(x));
+#else
+    return __builtin_clz(x);
+#endif
+}
+
 // Philox4x64-10 (Salmon, Moraes, Dror, Shaw, SC'11).
-__device__ __forceinline__ void philox4x64(uint64_t c[4], uint64_t k0, uint64_t k1) {
+__host__ __device__ __forceinline__ void philox4x64(uint64_t c[4], uint64_t k0, uint64_t k1) {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
         if (r) {
@@ -185,14 +213,75 @@ __device__ __forceinline__ void philox4x64(uint64_t c[4], uint64_t k0, uint64_t 
             k1 += 0xBB67AE8584CAA73BULL;
         }
         const uint64_t m0 = 0xD2E7470EE14C6C93ULL, m1 = 0xCA5A826395121157ULL;
-        const uint64_t hi0 = __umul64hi(m0, c[0]), lo0 = m0 * c[0];
-        const uint64_t hi1 = __umul64hi(m1, c[2]), lo1 = m1 * c[2];
+        const uint64_t hi0 = mulhi64(m0, c[0]), lo0 = m0 * c[0];
+        const uint64_t hi1 = mulhi64(m1, c[2]), lo1 = m1 * c[2];
         const uint64_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
         c[0] = n0;
         c[1] = lo1;
         c[2] = n2;
         c[3] = lo0;
     }
+}
+
+// Box-Muller pair from one 64-bit Philox word: u1 = (m1 + 1/2) 2^-24 from bits
+// 63..40 and the angle u2 = m2 2^-24 turns from bits 39..16. The pair is
+// evaluated in f64 with correctly rounded operations only (+ - * / sqrt, no
+// FMA, no libm), so every IEEE-754 machine yields the same bits; each value is
+// then rounded once to f32.
+//   ln u1: u1 = n 2^-25 with n = 2 m1 + 1 odd; n = f 2^e with f in
+//   [sqrt(1/2), sqrt(2)] split exactly; ln f = 2 atanh(s), s = (f-1)/(f+1),
+//   |s| <= 0.1716, by its series through s^13 (truncation < 2e-13 relative).
+//   sin/cos of 2 pi u2: the quadrant (top 2 bits) and the reflection about
+//   pi/4 are exact integer steps; sin and cos of the remaining angle
+//   (<= pi/4) come from their Taylor series through the 13th and 14th powers.
+__host__ __device__ __forceinline__ void box_muller(uint64_t w, float &zc, float &zs) {
+    const uint32_t n = (static_cast<uint32_t>(w >> 40) << 1) | 1u;
+    int e = 31 - clz32(n);
+    double f = ldexp(static_cast<double>(n), -e);
+    if (f > 1.4142135623730951) {
+        f = f * 0.5;
+        e += 1;
+    }
+    const double s = (f - 1.0) / (f + 1.0);
+    const double z = s * s;
+    double p = 1.0 / 13.0;
+    p = p * z + 1.0 / 11.0;
+    p = p * z + 1.0 / 9.0;
+    p = p * z + 1.0 / 7.0;
+    p = p * z + 1.0 / 5.0;
+    p = p * z + 1.0 / 3.0;
+    p = p * z + 1.0;
+    const double ln_u1 = static_cast<double>(e - 25) * 0.6931471805599453 + (2.0 * s) * p;
+    const double r = sqrt(-2.0 * ln_u1);
+
+    const uint32_t m2 = static_cast<uint32_t>(w >> 16) & 0xFFFFFFu;
+    const uint32_t quad = m2 >> 22, v = m2 & 0x3FFFFFu;
+    const bool flip = v > 0x200000u;
+    const double th = static_cast<double>(flip ? 0x400000u - v : v) * (1.5707963267948966 * 0x1.0p-22);
+    const double t2 = th * th;
+    double ps = 1.0 / 6227020800.0;
+    ps = ps * t2 - 1.0 / 39916800.0;
+    ps = ps * t2 + 1.0 / 362880.0;
+    ps = ps * t2 - 1.0 / 5040.0;
+    ps = ps * t2 + 1.0 / 120.0;
+    ps = ps * t2 - 1.0 / 6.0;
+    double pc = -1.0 / 87178291200.0;
+    pc = pc * t2 + 1.0 / 479001600.0;
+    pc = pc * t2 - 1.0 / 3628800.0;
+    pc = pc * t2 + 1.0 / 40320.0;
+    pc = pc * t2 - 1.0 / 720.0;
+    pc = pc * t2 + 1.0 / 24.0;
+    pc = pc * t2 - 0.5;
+    const double sn = th + (th * t2) * ps;
+    const double cs = 1.0 + t2 * pc;
+    const double a = flip ? cs : sn, b = flip ? sn : cs;  // sin, cos within the quadrant
+    double sv, cv;
+    if (quad == 0) { sv = a; cv = b; }
+    else if (quad == 1) { sv = b; cv = -a; }
+    else if (quad == 2) { sv = -a; cv = -b; }
+    else { sv = -b; cv = a; }
+    zc = static_cast<float>(r * cv);
+    zs = static_cast<float>(r * sv);
 }
 
 }  // namespace lpk

@@ -70,67 +70,6 @@ __device__ __forceinline__ double rem_euclid_f64(double a, double b) {
     return r < 0.0 ? r + fabs(b) : r;
 }
 
-// Box-Muller pair from one 64-bit Philox word: u1 = (m1 + 1/2) 2^-24 from bits
-// 63..40 and the angle u2 = m2 2^-24 turns from bits 39..16. The pair is
-// evaluated in f64 with correctly rounded operations only (+ - * / sqrt, no
-// FMA, no libm), so every IEEE-754 machine yields the same bits; each value is
-// then rounded once to f32.
-//   ln u1: u1 = n 2^-25 with n = 2 m1 + 1 odd; n = f 2^e with f in
-//   [sqrt(1/2), sqrt(2)] split exactly; ln f = 2 atanh(s), s = (f-1)/(f+1),
-//   |s| <= 0.1716, by its series through s^13 (truncation < 2e-13 relative).
-//   sin/cos of 2 pi u2: the quadrant (top 2 bits) and the reflection about
-//   pi/4 are exact integer steps; sin and cos of the remaining angle
-//   (<= pi/4) come from their Taylor series through the 13th and 14th powers.
-__device__ __forceinline__ void box_muller(uint64_t w, float &zc, float &zs) {
-    const uint32_t n = (static_cast<uint32_t>(w >> 40) << 1) | 1u;
-    int e = 31 - __clz(static_cast<int>(n));
-    double f = ldexp(static_cast<double>(n), -e);
-    if (f > 1.4142135623730951) {
-        f = f * 0.5;
-        e += 1;
-    }
-    const double s = (f - 1.0) / (f + 1.0);
-    const double z = s * s;
-    double p = 1.0 / 13.0;
-    p = p * z + 1.0 / 11.0;
-    p = p * z + 1.0 / 9.0;
-    p = p * z + 1.0 / 7.0;
-    p = p * z + 1.0 / 5.0;
-    p = p * z + 1.0 / 3.0;
-    p = p * z + 1.0;
-    const double ln_u1 = static_cast<double>(e - 25) * 0.6931471805599453 + (2.0 * s) * p;
-    const double r = sqrt(-2.0 * ln_u1);
-
-    const uint32_t m2 = static_cast<uint32_t>(w >> 16) & 0xFFFFFFu;
-    const uint32_t quad = m2 >> 22, v = m2 & 0x3FFFFFu;
-    const bool flip = v > 0x200000u;
-    const double th = static_cast<double>(flip ? 0x400000u - v : v) * (1.5707963267948966 * 0x1.0p-22);
-    const double t2 = th * th;
-    double ps = 1.0 / 6227020800.0;
-    ps = ps * t2 - 1.0 / 39916800.0;
-    ps = ps * t2 + 1.0 / 362880.0;
-    ps = ps * t2 - 1.0 / 5040.0;
-    ps = ps * t2 + 1.0 / 120.0;
-    ps = ps * t2 - 1.0 / 6.0;
-    double pc = -1.0 / 87178291200.0;
-    pc = pc * t2 + 1.0 / 479001600.0;
-    pc = pc * t2 - 1.0 / 3628800.0;
-    pc = pc * t2 + 1.0 / 40320.0;
-    pc = pc * t2 - 1.0 / 720.0;
-    pc = pc * t2 + 1.0 / 24.0;
-    pc = pc * t2 - 0.5;
-    const double sn = th + (th * t2) * ps;
-    const double cs = 1.0 + t2 * pc;
-    const double a = flip ? cs : sn, b = flip ? sn : cs;  // sin, cos within the quadrant
-    double sv, cv;
-    if (quad == 0) { sv = a; cv = b; }
-    else if (quad == 1) { sv = b; cv = -a; }
-    else if (quad == 2) { sv = -a; cv = -b; }
-    else { sv = -b; cv = a; }
-    zc = static_cast<float>(r * cv);
-    zs = static_cast<float>(r * sv);
-}
-
 // ---------------------------------------------------------------- geometry
 
 // PrecomputedTrajectory::position_ecef (src/geometry/precomputed.rs:111-121):
@@ -403,10 +342,19 @@ __device__ __forceinline__ double walk_eps(const JobDev &J, int64_t k) {
 // ClockStream::delta_t_at (src/clock/mod.rs:329-346); clock start epoch 0.0.
 __device__ __forceinline__ double clock_delta(const JobDev &J, double t) {
     const double elapsed = t - 0.0;
+    // Flicker::delta_at (src/clock/mod.rs:139-148): D interpolated on the 128 Hz grid
+    double flick = 0.0;
+    if (J.flicker) {
+        const double xf = fmax(elapsed, 0.0) * kFlickerRate;
+        const double mf = floor(xf);
+        const int64_t m0 = static_cast<int64_t>(mf);
+        const double fr = xf - mf;
+        flick = J.flicker[m0] * (1.0 - fr) + J.flicker[m0 + 1] * fr;
+    }
     const double x = elapsed * J.master;
     if (x <= 0.0) {
         const double e0 = J.walk ? walk_eps(J, 0) : 0.0;
-        return (e0 + 0.0) + (J.c0 + J.y0 * 0.0);
+        return (e0 + flick) + (J.c0 + J.y0 * 0.0);
     }
     const int64_t k0 = static_cast<int64_t>(floor(x));
     const double frac = x - static_cast<double>(k0);
@@ -416,7 +364,7 @@ __device__ __forceinline__ double clock_delta(const JobDev &J, double t) {
         ehi = walk_eps(J, k0 + 1);
     }
     const double eps = elo * (1.0 - frac) + ehi * frac;
-    return (eps + 0.0) + (J.c0 + J.y0 * elapsed);
+    return (eps + flick) + (J.c0 + J.y0 * elapsed);
 }
 
 // ---------------------------------------------------------------- render

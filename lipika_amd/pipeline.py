@@ -301,8 +301,6 @@ def prepare(scene: SceneConfig, time_base: TimeBase = TimeBase.SCENE) -> Prepare
     for r in scene.receivers:
         if not r.bands:
             raise ValueError(f"receiver '{r.name}' has no bands; at least one [[receiver.band]] required")
-        if r.clock.flicker_floor > 0.0:
-            raise NotImplementedError("flicker FM is not on the GPU path")
         pos = station_ecef(r.trajectory)
         master = 0.0
         for b in r.bands:
@@ -402,6 +400,7 @@ class Backend:
             d.initial_time_offset_s = r.clock.initial_time_offset_s
             d.master_rate_hz = r.master_rate_hz
             d.clock_seed = clock_seed(rng_seed_base, r_idx)
+            d.flicker_floor = r.clock.flicker_floor
         self.check(self.lib.lpk_scene_upload(self.h, tx_arr, n_tx, rx_arr, n_rx))
         del keep  # tables are copied by lpk_scene_upload
         self.scene = scene

@@ -518,6 +518,11 @@ void orc_philox4x64(const uint64_t ctr_in[4], const uint64_t key_in[2], uint64_t
 /* Domain tags that separate the clock-walk and AWGN streams of one seed. */
 #define LPK_CLOCK_TAG 0x636C6F636B77616CULL   /* "clockwal" */
 #define LPK_AWGN_TAG 0x6177676E6E6F6973ULL    /* "awgnnois" */
+#define LPK_FLICKER_TAG 0x666C69636B65726EULL /* "flickern" */
+/* src/clock/mod.rs:67-69, and tau = 2 s of the calibration (:106) */
+static const double FLICKER_RATE_HZ = 128.0;
+static const double FLICKER_F_LO = 1e-4;
+static const int FLICKER_CALIB_M = 256;
 
 /* Box-Muller pair from one 64-bit Philox word: u1 = (m1 + 1/2) 2^-24 from bits
  * 63..40, angle u2 = m2 2^-24 turns from bits 39..16. The project defines the
@@ -596,9 +601,79 @@ void orc_awgn_normals(uint64_t noise_seed, uint64_t i, float *z_re, float *z_im)
     orc_box_muller(w[i & 3], z_re, z_im);
 }
 
+/* Normal n of a receiver's flicker stream: block n >> 3 of Philox4x64-10
+ * (ctr = (n >> 3, 0, 0, 0), key = (seed, "flickern")), word (n >> 1) & 3, the
+ * cos branch for even n and the sin branch for odd n. Step k of pole j uses
+ * n = (k + 1) * 20 + j; n < 20 start the poles. */
+double orc_flicker_normal(uint64_t clock_seed, uint64_t n) {
+    uint64_t ctr[4] = {n >> 3, 0, 0, 0}, key[2] = {clock_seed, LPK_FLICKER_TAG}, w[4];
+    orc_philox4x64(ctr, key, w);
+    float zc, zs;
+    orc_box_muller(w[(n >> 1) & 3], &zc, &zs);
+    return (double)((n & 1) ? zs : zc);
+}
+
+/* src/clock/mod.rs:71-80 */
+void orc_flicker_poles(double *alpha, double *gain) {
+    double f_hi = FLICKER_RATE_HZ / 2.0;
+    for (int j = 0; j < ORC_FLICKER_POLES; ++j) {
+        double f = FLICKER_F_LO * pow(f_hi / FLICKER_F_LO, (double)j / (double)(ORC_FLICKER_POLES - 1));
+        double a = exp(-2.0 * PI * f / FLICKER_RATE_HZ);
+        alpha[j] = a;
+        gain[j] = sqrt(1.0 - a * a);
+    }
+}
+
+/* The calibration of src/clock/mod.rs:98-108 (scale = floor / the Allan
+ * deviation at tau = 2 s of the unit-scaled process), with the estimator's
+ * expectation in place of one random realisation: for a unit AR(1) pole a,
+ * (E[S1^2] - E[S1 S2]) / m^2 over two adjacent blocks of m steps (the
+ * non-overlapping estimator of :150-163), summed over the independent poles. */
+double orc_flicker_unit_adev(const double *alpha, int m) {
+    double var = 0.0;
+    for (int j = 0; j < ORC_FLICKER_POLES; ++j) {
+        double a = alpha[j], s11 = (double)m, s12 = 0.0, p = 1.0;
+        for (int d = 1; d < 2 * m; ++d) {
+            p *= a;
+            if (d < m) s11 += 2.0 * (double)(m - d) * p;
+            s12 += (d <= m ? (double)d : (double)(2 * m - d)) * p;
+        }
+        var += (s11 - s12) / ((double)m * (double)m);
+    }
+    return sqrt(var);
+}
+
+/* src/clock/mod.rs:125-133 */
+static void flicker_advance_to(orc_clock *c, int64_t m) {
+    while (c->f_frontier < m) {
+        c->f_lo = c->f_hi;
+        double y = 0.0;
+        for (int j = 0; j < ORC_FLICKER_POLES; ++j) {
+            c->f_state[j] = c->f_alpha[j] * c->f_state[j] +
+                            c->f_gain[j] * orc_flicker_normal(c->seed, (uint64_t)(c->f_frontier + 2) * ORC_FLICKER_POLES + j);
+            y += c->f_state[j];
+        }
+        y = y * c->f_scale;
+        c->f_hi += y / FLICKER_RATE_HZ;
+        c->f_frontier += 1;
+    }
+}
+
+/* src/clock/mod.rs:139-148 */
+static double flicker_delta_at(orc_clock *c, double elapsed) {
+    if (!c->flick) return 0.0;
+    double x = fmax(elapsed, 0.0) * FLICKER_RATE_HZ;
+    int64_t m0 = sat_i64(floor(x));
+    double frac = x - (double)m0;
+    flicker_advance_to(c, m0 + 1);
+    return c->f_lo * (1.0 - frac) + c->f_hi * frac;
+}
+
 /* ---------------------------------------------------------------- the clock */
 
-/* src/clock/mod.rs:268-295 (white-FM only: flicker_floor is not on the path) */
+/* src/clock/mod.rs:268-295. The flicker process starts its poles from their
+ * stationary distribution N(0, 1), which the reference's 200 000 warm-up steps
+ * approach (:110-116). */
 void orc_clock_init(orc_clock *c, const orc_rx *rx) {
     c->c0 = rx->initial_time_offset_s;
     c->y0 = rx->frequency_offset_y0;
@@ -609,6 +684,15 @@ void orc_clock_init(orc_clock *c, const orc_rx *rx) {
     c->frontier = -1;
     c->eps_hi = 0.0;
     c->eps_lo = 0.0;
+    c->flick = rx->flicker_floor > 0.0;
+    c->f_frontier = -1;
+    c->f_hi = 0.0;
+    c->f_lo = 0.0;
+    if (c->flick) {
+        orc_flicker_poles(c->f_alpha, c->f_gain);
+        c->f_scale = rx->flicker_floor / orc_flicker_unit_adev(c->f_alpha, FLICKER_CALIB_M);
+        for (int j = 0; j < ORC_FLICKER_POLES; ++j) c->f_state[j] = orc_flicker_normal(c->seed, (uint64_t)j);
+    }
 }
 
 /* src/clock/mod.rs:299-307 */
@@ -625,7 +709,8 @@ double orc_clock_delta_t_at_master_index(orc_clock *c, int64_t k) {
     clock_advance_to(c, k > 0 ? k : 0);
     double eps = k < 0 ? 0.0 : c->eps_hi;
     double elapsed = (double)(k > 0 ? k : 0) / c->master;
-    return (eps + 0.0) + (c->c0 + c->y0 * elapsed);
+    double flick = flicker_delta_at(c, elapsed);
+    return (eps + flick) + (c->c0 + c->y0 * elapsed);
 }
 
 /* src/clock/mod.rs:329-346 (clock start epoch 0.0, src/pipeline.rs:869-872) */
@@ -634,13 +719,15 @@ double orc_clock_delta_t_at(orc_clock *c, double t) {
     double x = elapsed * c->master;
     if (x <= 0.0) {
         clock_advance_to(c, 0);
-        return (c->eps_hi + 0.0) + (c->c0 + c->y0 * 0.0);
+        double flick = flicker_delta_at(c, 0.0);
+        return (c->eps_hi + flick) + (c->c0 + c->y0 * 0.0);
     }
     int64_t k0 = sat_i64(floor(x));
     double frac = x - (double)k0;
     clock_advance_to(c, k0 + 1);
     double eps = c->eps_lo * (1.0 - frac) + c->eps_hi * frac;
-    return (eps + 0.0) + (c->c0 + c->y0 * elapsed);
+    double flick = flicker_delta_at(c, elapsed);
+    return (eps + flick) + (c->c0 + c->y0 * elapsed);
 }
 
 /* ---------------------------------------------------------- receiver ops */

@@ -85,6 +85,7 @@ typedef struct {
     double allan_deviation_1s, frequency_offset_y0, initial_time_offset_s;
     double master_rate_hz;
     uint64_t clock_seed;
+    double flicker_floor;        /* flicker-FM Allan floor, src/clock/mod.rs:26-30 (0 = none) */
 } orc_rx;
 
 typedef struct {
@@ -151,13 +152,26 @@ double orc_clock_normal(uint64_t clock_seed, uint64_t k);
 void orc_awgn_normals(uint64_t noise_seed, uint64_t i, float *z_re, float *z_im);
 
 /* ---- clock ---- */
+#define ORC_FLICKER_POLES 20
 typedef struct {
     double c0, y0, sigma_step, master;
     int walk;
     uint64_t seed;
     int64_t frontier;
     double eps_hi, eps_lo;
+    /* flicker FM (src/clock/mod.rs:55-149) */
+    int flick;
+    double f_alpha[ORC_FLICKER_POLES], f_gain[ORC_FLICKER_POLES], f_state[ORC_FLICKER_POLES];
+    double f_scale;
+    int64_t f_frontier;
+    double f_hi, f_lo;
 } orc_clock;
+/* Flicker pieces, exposed for the pins: the poles (alpha, gain), the Allan
+ * deviation of the unit-scaled pole sum at tau = m / 128 s, and normal n of a
+ * receiver's flicker stream. */
+void orc_flicker_poles(double *alpha, double *gain);
+double orc_flicker_unit_adev(const double *alpha, int m);
+double orc_flicker_normal(uint64_t clock_seed, uint64_t n);
 void orc_clock_init(orc_clock *c, const orc_rx *rx);
 double orc_clock_delta_t_at(orc_clock *c, double t);
 double orc_clock_delta_t_at_master_index(orc_clock *c, int64_t k);

@@ -42,7 +42,8 @@ class OrcPost(C.Structure):
 
 class OrcRx(C.Structure):
     _fields_ = [("pos_ecef", _d * 3), ("allan_deviation_1s", _d), ("frequency_offset_y0", _d),
-                ("initial_time_offset_s", _d), ("master_rate_hz", _d), ("clock_seed", _u64)]
+                ("initial_time_offset_s", _d), ("master_rate_hz", _d), ("clock_seed", _u64),
+                ("flicker_floor", _d)]
 
 
 class OrcBand(C.Structure):
@@ -57,7 +58,9 @@ class OrcBand(C.Structure):
 
 class OrcClock(C.Structure):
     _fields_ = [("c0", _d), ("y0", _d), ("sigma_step", _d), ("master", _d), ("walk", C.c_int),
-                ("seed", _u64), ("frontier", _i64), ("eps_hi", _d), ("eps_lo", _d)]
+                ("seed", _u64), ("frontier", _i64), ("eps_hi", _d), ("eps_lo", _d),
+                ("flick", C.c_int), ("f_alpha", _d * 20), ("f_gain", _d * 20), ("f_state", _d * 20),
+                ("f_scale", _d), ("f_frontier", _i64), ("f_hi", _d), ("f_lo", _d)]
 
 
 _LIB = None
@@ -100,6 +103,9 @@ def load() -> C.CDLL:
         "orc_philox4x64": (None, [P(_u64), P(_u64), P(_u64)]),
         "orc_box_muller": (None, [_u64, P(C.c_float), P(C.c_float)]),
         "orc_clock_normal": (_d, [_u64, _u64]),
+        "orc_flicker_normal": (_d, [_u64, _u64]),
+        "orc_flicker_poles": (None, [P(_d), P(_d)]),
+        "orc_flicker_unit_adev": (_d, [P(_d), C.c_int]),
         "orc_antenna_amplitude": (C.c_float, [P(OrcBand), _d]),
         "orc_arrival_elevation_deg": (_d, [P(_d), P(_d)]),
         "orc_awgn_normals": (None, [_u64, _u64, P(C.c_float), P(C.c_float)]),
@@ -169,6 +175,7 @@ class OracleScene:
             d.initial_time_offset_s = r.clock.initial_time_offset_s
             d.master_rate_hz = r.master_rate_hz
             d.clock_seed = clock_seed(rng_seed_base, r_idx)
+            d.flicker_floor = r.clock.flicker_floor
             for pb in r.bands:
                 bands.append((r_idx, pb))
         self.units = bands

@@ -249,9 +249,10 @@ def test_invalid_scenes_raise():
     with pytest.raises(NotImplementedError):
         prepare(SceneConfig(SceneMeta(0.0, 1.0), [TransmitterConfig("f", 1e9, CaCode(1, 0.0, "full"),
                                                                      StationaryEcef(7e6, 0, 0))], []))
-    with pytest.raises(NotImplementedError):
-        prepare(SceneConfig(SceneMeta(0.0, 1.0), [], [ReceiverConfig(
-            "r", StationaryEcef(1, 0, 0), NodeClockSpec(flicker_floor=1e-10), [ReceiverBandConfig("x", 1e9, 1e6)])]))
+    # flicker FM is on the path: a receiver with a flicker floor prepares
+    sc = prepare(SceneConfig(SceneMeta(0.0, 1.0), [], [ReceiverConfig(
+        "r", StationaryEcef(1, 0, 0), NodeClockSpec(flicker_floor=1e-10), [ReceiverBandConfig("x", 1e9, 1e6)])]))
+    assert sc.receivers[0].clock.flicker_floor == 1e-10
 
 
 def test_window_argument_validation():

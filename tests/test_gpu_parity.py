@@ -585,3 +585,39 @@ def test_host_delta_t_path_equals_device_clock(backend):
     backend.render([jb])
     backend.sync()
     assert a.tobytes() == b.tobytes()
+
+
+# ---------------------------------------------------------------- flicker FM
+
+def test_flicker_clock_matches_oracle(backend):
+    """Flicker FM (src/clock/mod.rs:55-149) on top of the white walk and a
+    drift: the host-stepped flicker integral interpolated on the device
+    against the oracle's clock stream. With the walk off the clock offsets
+    are bit-identical; IQ and taps are compared as for the walk alone, in the
+    plain and precision forms."""
+    import dataclasses
+    seed = 17
+    scene = noise_free(workloads.scene1(duration_s=2.0))
+    rx = scene.receivers[0]
+    for clock, label in ((NodeClockSpec(flicker_floor=1e-9, frequency_offset_y0=2e-7), "flicker only"),
+                         (NodeClockSpec(allan_deviation_1s=1e-11, flicker_floor=1e-9, frequency_offset_y0=2e-7),
+                          "flicker + white FM")):
+        scene.receivers[0] = dataclasses.replace(rx, clock=clock)
+        sc = prepare(scene)
+        backend.upload(sc, seed)
+        orc = OracleScene(sc, seed)
+        check_band_both_clocks(backend, orc, sc, 0, 0, 0, seed, f"config 1 LEO, {label}", chunk=30_000)
+        _, ddel = compare_taps(backend, orc, sc, 0, 0, 0, seed, 0, 96_000, label=label)
+        _, _, taps = orc.render_band(0, n=96_000, tap_slot=0)
+        j, keep = backend.job(0, 0, 0, seed, 0, 96_000)
+        dt = backend.taps(j, 0, _lib.FORM_PLAIN)[3]
+        if clock.allan_deviation_1s == 0.0:
+            np.testing.assert_array_equal(dt, taps[3])
+        # the flicker moved the clock: offsets minus drift span well over a ps
+        resid = taps[3] - 2e-7 * (np.arange(96_000) / 48_000.0)
+        assert np.ptp(resid) > 1e-12
+        g, _ = gpu_band(backend, sc, 0, 0, 0, seed, form=_lib.FORM_PRECISION, chunk=40_000)
+        o, _, _ = orc.render_band(0)
+        d = np.abs(g.astype(np.complex128) - o.astype(np.complex128)).max()
+        _say(f"flicker, {label}, precision form: max|d|/peak = {d / np.abs(o).max():.3e}")
+        assert d <= 2e-6 * np.abs(o).max()
