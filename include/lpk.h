@@ -341,7 +341,17 @@ int lpk_set_warm_start(lpk_ctx *ctx, lpk_lws *net);
 /* Converged-form counters since the last reset: emission-equation evaluations,
  * contributions, learned starts that fell back, solves that did not converge. */
 int lpk_solve_stats(lpk_ctx *ctx, uint64_t stats[4]);
-int lpk_solve_stats_reset(lpk_ctx *ctx);
+int lpk_solve_stats_reset(lpk_ctx *ctx);   /* also resets the horizon-gate counters */
+
+/* ---- horizon gate (src/channel/horizon.rs) ----
+ * In bands with an antenna cutoff, a (tile, transmitter) pair whose arrival
+ * elevation is certainly below the cutoff for the whole tile
+ * (certainly_below_cutoff, src/channel/horizon.rs:150-210) is skipped; every
+ * sample would have had zero gain, so the output is byte-identical with the
+ * gate off (the reference's *_without_horizon_gate path). On by default. */
+int lpk_set_horizon_gate(lpk_ctx *ctx, int on);
+/* (tile, transmitter) pairs skipped and tested since lpk_solve_stats_reset */
+int lpk_horizon_gate_stats(lpk_ctx *ctx, uint64_t stats[2]);
 
 #ifdef __cplusplus
 }

@@ -171,6 +171,8 @@ def load() -> C.CDLL:
         "lpk_set_warm_start": (_i32, [C.c_void_p, C.c_void_p]),
         "lpk_solve_stats": (_i32, [C.c_void_p, P(_u64)]),
         "lpk_solve_stats_reset": (_i32, [C.c_void_p]),
+        "lpk_set_horizon_gate": (_i32, [C.c_void_p, C.c_int]),
+        "lpk_horizon_gate_stats": (_i32, [C.c_void_p, P(_u64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -193,7 +195,7 @@ EXPORTED = (
     "lpk_plan_create", "lpk_plan_step", "lpk_plan_info", "lpk_plan_output", "lpk_plan_destroy",
     "lpk_lws_create", "lpk_lws_destroy", "lpk_lws_get_params", "lpk_lws_set_params", "lpk_lws_generate",
     "lpk_lws_infer", "lpk_lws_gradient", "lpk_lws_train", "lpk_set_warm_start", "lpk_solve_stats",
-    "lpk_solve_stats_reset",
+    "lpk_solve_stats_reset", "lpk_set_horizon_gate", "lpk_horizon_gate_stats",
 )
 
 

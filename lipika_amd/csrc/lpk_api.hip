@@ -19,7 +19,8 @@ namespace lpk {
 hipError_t launch_render(int form, bool antenna, int tile_shift, bool xcd_map, const JobDev *jobs, int n_jobs,
                          const LinkDev *links, const TxDev *txs, uint32_t n_tiles, const uint32_t *n_live,
                          double *tap_t_emit, double *tap_range, double *tap_phase, double *tap_delta_t,
-                         uint32_t tap_slot, const double *pred, unsigned long long *stats, hipStream_t stream);
+                         uint32_t tap_slot, const double *pred, unsigned long long *stats, bool gate,
+                         unsigned long long *gate_stats, hipStream_t stream);
 hipError_t launch_lws_predict(const JobDev *jobs, int n_jobs, const LinkDev *links, const TxDev *txs,
                               const LwsNetDev &net, int tile_shift, uint64_t n_queries, double *pred,
                               hipStream_t stream);
@@ -134,7 +135,9 @@ struct lpk_ctx {
     // with, its prediction table and the solve counters
     lpk_lws *lws = nullptr;
     DevBuf pred;
+    // counters: [0..3] converged-form solves, [4..5] horizon gate (skipped, tested)
     unsigned long long *stats = nullptr;
+    int gate = 1;   // apply the horizon gate in antenna scenes
     PinSlot pin[4];
     int pin_next = 0;
 };
@@ -310,13 +313,14 @@ LwsNetDev lws_net(const lpk_lws *net) {
     return LwsNetDev{static_cast<const bf16x8 *>(net->packed), net->theta};
 }
 
-// The four solve counters of the converged forms (zeroed on first use).
+// The solve and gate counters (zeroed on first use).
+constexpr int kNumStats = 6;
 void ensure_stats(lpk_ctx *ctx) {
     if (ctx->stats) return;
     void *p = nullptr;
-    if (hipMalloc(&p, 4 * sizeof(unsigned long long)) != hipSuccess) throw Fail{LPK_E_NOMEM, "solve counters"};
+    if (hipMalloc(&p, kNumStats * sizeof(unsigned long long)) != hipSuccess) throw Fail{LPK_E_NOMEM, "solve counters"};
     ctx->stats = static_cast<unsigned long long *>(p);
-    hip_check(hipMemsetAsync(p, 0, 4 * sizeof(unsigned long long), ctx->stream), "solve counter reset");
+    hip_check(hipMemsetAsync(p, 0, kNumStats * sizeof(unsigned long long), ctx->stream), "solve counter reset");
 }
 
 TxDev make_tx(lpk_ctx *ctx, const lpk_tx_desc &d, uint32_t idx) {
@@ -340,6 +344,33 @@ TxDev make_tx(lpk_ctx *ctx, const lpk_tx_desc &d, uint32_t idx) {
     } else {
         throw bad("unknown trajectory kind");
     }
+    // horizon-gate bounds (src/geometry/precomputed.rs:86-106, src/channel/horizon.rs:164-170)
+    if (d.traj_kind == LPK_TRAJ_STATIONARY) {
+        T.max_speed = 0.0;
+        T.max_radius = std::sqrt(d.pos_ecef[0] * d.pos_ecef[0] + d.pos_ecef[1] * d.pos_ecef[1] +
+                                 d.pos_ecef[2] * d.pos_ecef[2]);
+        T.gate_slack = 0.0;
+    } else {
+        const uint64_t n = d.grid_n;
+        const double *g = d.grid_xyz;
+        double seg2 = 0.0, r2 = 0.0;
+        bool fin = true;
+        for (uint64_t i = 0; i < n; ++i) {
+            const double x = g[i], y = g[n + i], z = g[2 * n + i];
+            const double q = x * x + y * y + z * z;
+            fin = fin && std::isfinite(q);
+            if (q > r2) r2 = q;
+            if (i > 0) {
+                const double dx = x - g[i - 1], dy = y - g[n + i - 1], dz = z - g[2 * n + i - 1];
+                const double s2 = dx * dx + dy * dy + dz * dz;
+                if (s2 > seg2) seg2 = s2;
+            }
+        }
+        T.max_speed = fin ? std::sqrt(seg2) / d.grid_dt_s : NAN;
+        T.max_radius = fin ? std::sqrt(r2) : NAN;
+        T.gate_slack = 1.0;
+    }
+    T.gate_ok = std::isfinite(T.max_speed) && std::isfinite(T.max_radius) ? 1 : 0;
     if (!is_fin(d.carrier_hz) || !(d.carrier_hz > 0.0)) throw bad("carrier_hz must be positive and finite");
     if (!is_fin(d.tx_power_w) || d.tx_power_w < 0.0) throw bad("tx_power_w must be finite and >= 0");
     T.carrier = d.carrier_hz;
@@ -817,7 +848,7 @@ void render(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form,
 
     if (tiles) {
         const bool converged = form == LPK_FORM_CONVERGED || form == LPK_FORM_LEARNED;
-        if (converged) ensure_stats(ctx);
+        ensure_stats(ctx);
         double *pred = n_pred ? static_cast<double *>(ctx->pred.ensure(n_pred * 8, ctx->stream)) : nullptr;
         timing_begin(ctx);
         if (pred)
@@ -827,7 +858,8 @@ void render(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form,
         hip_check(launch_render(converged ? 2 : form, any_antenna, ts, ctx->xcd_map != 0, jd, int(n_jobs), ld,
                                 ctx->tx_dev, uint32_t(tiles), nullptr, tap ? tap->t_emit : nullptr,
                                 tap ? tap->range : nullptr, tap ? tap->phase : nullptr, tap ? tap->delta_t : nullptr,
-                                tap_slot, pred, converged ? ctx->stats : nullptr, ctx->stream),
+                                tap_slot, pred, converged ? ctx->stats : nullptr, ctx->gate != 0,
+                                any_antenna ? ctx->stats + 4 : nullptr, ctx->stream),
                   "render launch");
         timing_end(ctx);
     }
@@ -1086,7 +1118,8 @@ lpk_plan *plan_create(lpk_ctx *ctx, const lpk_band_chunk *bands, uint32_t n_band
             e = launch_render(form, P->antenna, P->tile_shift, P->xcd_map != 0,
                               reinterpret_cast<const JobDev *>(dev + P->off_jobs), int(n_bands),
                               reinterpret_cast<const LinkDev *>(dev + P->off_links), ctx->tx_dev, P->max_tiles, live,
-                              nullptr, nullptr, nullptr, nullptr, 0, nullptr, nullptr, ctx->stream);
+                              nullptr, nullptr, nullptr, nullptr, 0, nullptr, nullptr, ctx->gate != 0, nullptr,
+                              ctx->stream);
         hipGraph_t g = nullptr;
         const hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
         P->graph[i] = g;
@@ -1611,7 +1644,7 @@ int lpk_solve_stats(lpk_ctx *ctx, uint64_t stats[4]) {
     if (!ctx || !stats) return LPK_E_INVALID;
     return guarded(ctx, [&] {
         ensure_stats(ctx);
-        unsigned long long h[4];
+        unsigned long long h[kNumStats];
         hip_check(hipMemcpyAsync(h, ctx->stats, sizeof h, hipMemcpyDeviceToHost, ctx->stream), "counter download");
         hip_check(hipStreamSynchronize(ctx->stream), "counter download");
         for (int k = 0; k < 4; ++k) stats[k] = h[k];
@@ -1622,7 +1655,25 @@ int lpk_solve_stats_reset(lpk_ctx *ctx) {
     if (!ctx) return LPK_E_INVALID;
     return guarded(ctx, [&] {
         ensure_stats(ctx);
-        hip_check(hipMemsetAsync(ctx->stats, 0, 4 * sizeof(unsigned long long), ctx->stream), "counter reset");
+        hip_check(hipMemsetAsync(ctx->stats, 0, kNumStats * sizeof(unsigned long long), ctx->stream), "counter reset");
+    });
+}
+
+int lpk_set_horizon_gate(lpk_ctx *ctx, int on) {
+    if (!ctx) return LPK_E_INVALID;
+    ctx->gate = on ? 1 : 0;
+    return LPK_OK;
+}
+
+int lpk_horizon_gate_stats(lpk_ctx *ctx, uint64_t stats[2]) {
+    if (!ctx || !stats) return LPK_E_INVALID;
+    return guarded(ctx, [&] {
+        ensure_stats(ctx);
+        unsigned long long h[kNumStats];
+        hip_check(hipMemcpyAsync(h, ctx->stats, sizeof h, hipMemcpyDeviceToHost, ctx->stream), "counter download");
+        hip_check(hipStreamSynchronize(ctx->stream), "counter download");
+        stats[0] = h[4];
+        stats[1] = h[5];
     });
 }
 

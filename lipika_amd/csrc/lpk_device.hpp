@@ -113,7 +113,18 @@ struct TxDev {
     // trajectory has none (the start is then the straight-line light time)
     int32_t kep_on, _kpad;
     double kep_a, kep_cos_i, kep_sin_i, kep_raan, kep_nu0, kep_n, kep_epoch;
+    // horizon gate (src/channel/horizon.rs:164-170): speed and distance bounds
+    // of the trajectory and the position slack; gate_ok = 0 when not finite
+    int32_t gate_ok, _gpad2;
+    double max_speed, max_radius, gate_slack;
 };
+
+// Horizon gate (src/channel/horizon.rs): a (tile, transmitter) pair whose
+// arrival elevation is certainly below the antenna cutoff for every sample of
+// the tile is skipped; the per-sample path would give each of them zero.
+constexpr double kGateMarginDeg = 1.0e-3;     // src/channel/horizon.rs:64
+constexpr double kGateMaxLightTime = 100.0;   // src/channel/horizon.rs:68
+constexpr int kGateMaxLinks = 1024;           // links per tile the gate tests (more go per sample)
 
 // Converged emission-time solve (the converged and learned forms): the
 // reference's iteration t <- tu - |R(w (tu - t)) P(t) - O| / c on the

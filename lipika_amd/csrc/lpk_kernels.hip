@@ -203,6 +203,40 @@ __device__ __forceinline__ int solve_converged(const TxDev &T, const double *obs
     return -1;
 }
 
+// certainly_below_cutoff (src/channel/horizon.rs:150-210) for a stationary
+// receiver (ball radius 0 around O) over reception times [t_min, t_max]
+// (epoch-relative). True only if every sample of that span would get zero gain.
+__device__ __forceinline__ bool gate_below_cutoff(const JobDev &J, const TxDev &T, double t_min, double t_max) {
+    if (!T.gate_ok || !(t_min <= t_max)) return false;
+    const double ox = J.obs[0], oy = J.obs[1], oz = J.obs[2];
+    const double o_norm = sqrt(ox * ox + oy * oy + oz * oz);
+    if (!isfinite(o_norm)) return false;
+    const double lt = (1.01 * T.max_radius + o_norm + 0.0) * kInvC;
+    const double light_time = lt > kEmissionInitialLag ? lt : kEmissionInitialLag;
+    if (!(light_time <= kGateMaxLightTime)) return false;
+    const double t_mag = fabs(t_min) > fabs(t_max) ? fabs(t_min) : fabs(t_max);
+    const double lag = light_time * (1.0 + 1e-9) + 1e-6 + 1e-12 * t_mag;
+    const double t_lo = J.epoch + (t_min - lag);
+    const double t_hi = J.epoch + t_max;
+    const double t_mid = t_lo + 0.5 * (t_hi - t_lo);
+    const double h1 = t_mid - t_lo, h2 = t_hi - t_mid;
+    const double half_span = h1 > h2 ? h1 : h2;
+    double sx, sy, sz;
+    int64_t cell;
+    tx_position(T, t_mid, sx, sy, sz, cell);
+    const double source_radius = T.max_speed * half_span * (1.0 + 1e-9) + T.gate_slack;
+    const double dx = sx - ox, dy = sy - oy, dz = sz - oz;
+    const double range = sqrt(dx * dx + dy * dy + dz * dz);
+    const double rho = source_radius + 0.0;
+    if (!(rho < range) || !(0.0 < o_norm)) return false;
+    const double sin_el = (dx * ox + dy * oy + dz * oz) / (range * o_norm);
+    const double c = sin_el < -1.0 ? -1.0 : (sin_el > 1.0 ? 1.0 : sin_el);
+    const double deg = 180.0 / 3.141592653589793;
+    const double el_centre = asin(c) * deg;
+    const double spread = asin(rho / range) * deg + asin(0.0 / o_norm) * deg;
+    return el_centre + spread + kGateMarginDeg < J.ant_cutoff;
+}
+
 // Antenna::amplitude_at (src/rx/mod.rs:48-58) at the arrival elevation of
 // the source at the solved emission time (src/channel/mod.rs:190-207): the
 // position is queried at epoch + t_emit (no Sagnac rotation), sin(el) =
@@ -382,6 +416,7 @@ struct TapOut {
 // first part of the record holds everything the sample loop reads on its
 // fast path, so the loop touches no transmitter table in global memory.
 constexpr int32_t kLinkSmooth = 1;
+constexpr int32_t kLinkGated = 2;   // below the antenna cutoff for the whole tile
 struct alignas(16) Anchor {
     // read by every lane of the sample loop
     double r0m, q0d1, q0c2, rd0;   // r0(tu) = r0m + x (q0d1 + q0c2 x) + rd0 del0, x = tu - tu_mid
@@ -477,6 +512,7 @@ __device__ __forceinline__ void precision_link(const Anchor &A, const TxDev *__r
                                                double t_local, double dlt, double x, double del0, double &sum_re,
                                                double &sum_im, bool tap_here, double &tap_te, double &tap_r,
                                                double &tap_ph) {
+    if (ANT && (A.flags & kLinkGated)) return;   // below the cutoff for the whole tile
     double r;
     if (A.flags & kLinkSmooth) {
         const double r0 = fma(A.rd0, del0, fma(x, fma(A.q0c2, x, A.q0d1), A.r0m));
@@ -577,6 +613,8 @@ struct RenderLaunch {
     uint32_t xcd_map;             // deal contiguous tile runs to each XCD
     const double *pred;           // converged form: predicted light times (JobDev::pred0), or null
     unsigned long long *stats;    // converged form: evaluations, contributions, fallbacks, unconverged
+    uint32_t gate;                // antenna kernels: apply the horizon gate
+    unsigned long long *gate_stats;  // (tile, link) pairs the gate skipped, and pairs it tested
 };
 
 template <int FORM, bool ANT, int TS>
@@ -622,6 +660,37 @@ __global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM == 1 ? 5 : 6))
     const double dlt = J.delta_t ? J.delta_t[ic - J.i0] : clock_delta(J, t_local);
     const double tu = t_local - dlt;
 
+    // Horizon gate (antenna kernels): the span of the tile's reception times
+    // (src/channel/horizon.rs:97-145 for a stationary receiver); a
+    // non-finite time anywhere turns the gate off for the tile.
+    const bool gating = ANT && lx.gate && !tap.on && J.ant_on && J.ant_cut;
+    double g_tmin = 0.0, g_tmax = 0.0;
+    bool g_ok = false;
+    if (ANT && gating) {
+        __shared__ double s_tspan[2 * (kTile / 64)];
+        __shared__ uint32_t s_tbad[kTile / 64];
+        double mn = tu, mx = tu;
+        for (int o = 32; o >= 1; o >>= 1) {
+            mn = fmin(mn, __shfl_xor(mn, o));
+            mx = fmax(mx, __shfl_xor(mx, o));
+        }
+        const bool bad = __ballot(!isfinite(tu)) != 0;
+        if ((threadIdx.x & 63) == 0) {
+            s_tspan[2 * (threadIdx.x >> 6)] = mn;
+            s_tspan[2 * (threadIdx.x >> 6) + 1] = mx;
+            s_tbad[threadIdx.x >> 6] = bad ? 1u : 0u;
+        }
+        __syncthreads();
+        g_tmin = s_tspan[0];
+        g_tmax = s_tspan[1];
+        g_ok = s_tbad[0] == 0;
+        for (int w = 1; w < kTile / 64; ++w) {
+            g_tmin = fmin(g_tmin, s_tspan[2 * w]);
+            g_tmax = fmax(g_tmax, s_tspan[2 * w + 1]);
+            g_ok = g_ok && s_tbad[w] == 0;
+        }
+    }
+
     float acc_re = 0.0f, acc_im = 0.0f;
     double sum_re = 0.0, sum_im = 0.0;  // exact sum of the f32 contributions (precision form)
 
@@ -635,6 +704,24 @@ __global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM == 1 ? 5 : 6))
         // evaluations beyond `nominal` per contribution, fallbacks, unconverged
         int64_t w_extra = 0;
         uint32_t w_fb = 0, w_unc = 0;
+        // the links the gate skips for this tile, as a bitmap
+        __shared__ uint32_t s_gated[ANT ? kGateMaxLinks / 32 : 1];
+        if (ANT && gating) {
+            for (uint32_t w = threadIdx.x; w < kGateMaxLinks / 32; w += kTile) s_gated[w] = 0u;
+            __syncthreads();
+            const uint32_t ng = J.n_active < kGateMaxLinks ? J.n_active : kGateMaxLinks;
+            if (g_ok)
+                for (uint32_t s = threadIdx.x; s < ng; s += kTile)
+                    if (gate_below_cutoff(J, txs[links[J.link0 + s].tx], g_tmin, g_tmax))
+                        atomicOr(&s_gated[s >> 5], 1u << (s & 31));
+            __syncthreads();
+            if (threadIdx.x == 0 && lx.gate_stats) {
+                uint32_t n = 0;
+                for (uint32_t w = 0; w < (ng + 31) / 32; ++w) n += __popc(s_gated[w]);
+                if (n) atomicAdd(&lx.gate_stats[0], static_cast<unsigned long long>(n));
+                atomicAdd(&lx.gate_stats[1], static_cast<unsigned long long>(ng));
+            }
+        }
         if (FORM == 2 && lx.pred) {
             const uint32_t bq = (tile - J.tile0) / J.pred_qs;   // the query boundary at or before this tile
             const uint64_t last = J.i0 + J.clen - 1;
@@ -647,6 +734,7 @@ __global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM == 1 ? 5 : 6))
             prow = lx.pred + J.pred0 + bq;
         }
         for (uint32_t s = 0; s < J.n_active; ++s) {
+            if (ANT && gating && s < kGateMaxLinks && ((s_gated[s >> 5] >> (s & 31)) & 1u)) continue;
             const LinkDev L = links[J.link0 + s];
             const TxDev &T = txs[L.tx];
             double t_emit, r;
@@ -809,6 +897,11 @@ __global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM == 1 ? 5 : 6))
                 A.rate = T.signal_kind == 0 ? T.omega : T.rate;
                 A.code = T.ca_code;
                 A.flags = smooth ? kLinkSmooth : 0;
+                if (ANT && gating && g_ok && gate_below_cutoff(J, T, g_tmin, g_tmax)) {
+                    A.flags |= kLinkGated;
+                    if (lx.gate_stats) atomicAdd(&lx.gate_stats[0], 1ull);
+                }
+                if (ANT && gating && lx.gate_stats) atomicAdd(&lx.gate_stats[1], 1ull);
                 A.tx = L.tx;
                 // Values of the symbol (or NAV bit) the tile starts in and of
                 // the next one; lanes take them when their own exact index
@@ -1070,7 +1163,8 @@ RenderFn<FORM, ANT> render_fn(int tile_shift) {
 hipError_t launch_render(int form, bool antenna, int tile_shift, bool xcd_map, const JobDev *jobs, int n_jobs,
                          const LinkDev *links, const TxDev *txs, uint32_t n_tiles, const uint32_t *n_live,
                          double *tap_t_emit, double *tap_range, double *tap_phase, double *tap_delta_t,
-                         uint32_t tap_slot, const double *pred, unsigned long long *stats, hipStream_t stream) {
+                         uint32_t tap_slot, const double *pred, unsigned long long *stats, bool gate,
+                         unsigned long long *gate_stats, hipStream_t stream) {
     TapOut tap{tap_t_emit, tap_range, tap_phase, tap_delta_t, tap_slot, tap_t_emit ? 1u : 0u};
     if (tile_shift < 7 || tile_shift > 9 || form < 0 || form > 2) return hipErrorInvalidValue;
     void (*kernel)(const JobDev *, int, const LinkDev *, const TxDev *, RenderLaunch, TapOut) =
@@ -1079,7 +1173,7 @@ hipError_t launch_render(int form, bool antenna, int tile_shift, bool xcd_map, c
                     : (antenna ? render_fn<2, true>(tile_shift) : render_fn<2, false>(tile_shift));
     for (uint32_t base = 0; base < n_tiles; base += kMaxLaunchTiles) {
         const uint32_t n = n_tiles - base < kMaxLaunchTiles ? n_tiles - base : kMaxLaunchTiles;
-        const RenderLaunch lx{base, n, n_live, xcd_map ? 1u : 0u, pred, stats};
+        const RenderLaunch lx{base, n, n_live, xcd_map ? 1u : 0u, pred, stats, gate ? 1u : 0u, gate_stats};
         hipLaunchKernelGGL(kernel, dim3(n), dim3(1u << tile_shift), 0, stream, jobs, n_jobs, links, txs, lx, tap);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;

@@ -491,7 +491,18 @@ class Backend:
         return dict(evaluations=a[0], contributions=a[1], fallbacks=a[2], unconverged=a[3])
 
     def solve_stats_reset(self) -> None:
+        """Zeroes the converged-form and horizon-gate counters."""
         self.check(self.lib.lpk_solve_stats_reset(self.h))
+
+    def set_horizon_gate(self, on: bool) -> None:
+        """Skip (tile, transmitter) pairs certainly below the antenna cutoff
+        (src/channel/horizon.rs); on by default, output identical either way."""
+        self.check(self.lib.lpk_set_horizon_gate(self.h, 1 if on else 0))
+
+    def horizon_gate_stats(self) -> dict:
+        a = (C.c_uint64 * 2)()
+        self.check(self.lib.lpk_horizon_gate_stats(self.h, a))
+        return dict(skipped=a[0], tested=a[1])
 
     def taps(self, job, slot: int, form: int = _lib.FORM_PLAIN):
         n = job.clen

@@ -621,3 +621,58 @@ def test_flicker_clock_matches_oracle(backend):
         d = np.abs(g.astype(np.complex128) - o.astype(np.complex128)).max()
         _say(f"flicker, {label}, precision form: max|d|/peak = {d / np.abs(o).max():.3e}")
         assert d <= 2e-6 * np.abs(o).max()
+
+
+# ---------------------------------------------------------------- horizon gate
+
+def _sky_scene(n_leo=24, duration_s=0.5):
+    """A receiver with a 5 deg antenna cutoff under a sky of seeded LEO
+    sources (src/channel/horizon.rs's case: most of them below the cutoff at
+    any moment), 250 kHz."""
+    from lipika_amd.config import (CircularKeplerian, Cw, FrontendConfig, ReceiverBandConfig, SceneConfig, SceneMeta,
+                                   TransmitterConfig)
+    rng = np.random.default_rng(91)
+    txs = [TransmitterConfig(f"leo{k}", 437e6 + 2e3 * k, Cw(0.0),
+                             CircularKeplerian(float(rng.uniform(6.9e6, 7.5e6)), float(rng.uniform(0, 180)),
+                                               float(rng.uniform(0, 360)), float(rng.uniform(0, 360)), util.EPOCH), 0.0)
+           for k in range(n_leo)]
+    fe = FrontendConfig(antenna={"peak_dbi": 3.0, "horizon_dbi": -3.0, "fb_db": 10.0, "cutoff_deg": 5.0})
+    band = ReceiverBandConfig("uhf", 437e6, 250e3, 290.0, noise_power_w=0.0, frontend=fe)
+    return SceneConfig(SceneMeta(util.EPOCH, duration_s), txs, [util.lab_receiver([band])])
+
+
+@pytest.mark.parametrize("form", [_lib.FORM_PLAIN, _lib.FORM_PRECISION, _lib.FORM_CONVERGED])
+def test_horizon_gate_is_byte_identical(backend, form):
+    """tests/horizon_gate.rs's contract on the GPU: with the gate on, the
+    (tile, transmitter) pairs below the cutoff are skipped, and the IQ and the
+    unoccluded flags are byte-identical to the ungated render; the plain form
+    also matches the oracle (which has no gate)."""
+    seed = 13
+    sc = prepare(_sky_scene())
+    backend.upload(sc, seed)
+    n, na = sc.receivers[0].bands[0].n, len(sc.receivers[0].bands[0].active)
+    out = {}
+    for gate in (True, False):
+        backend.set_horizon_gate(gate)
+        backend.solve_stats_reset()
+        g = np.empty(n, dtype=np.complex64)
+        flags = np.zeros(na, dtype=np.uint8)
+        for i0 in range(0, n, 40_000):
+            f = np.zeros(na, dtype=np.uint8)
+            j, keep = backend.job(0, 0, 0, seed, i0, min(40_000, n - i0), out_host=g[i0:i0 + 40_000], unoccluded=f)
+            backend.render([j], form)
+            flags |= f
+        backend.sync()
+        out[gate] = (g, flags, backend.horizon_gate_stats())
+    backend.set_horizon_gate(True)
+    (gon, fon, son), (goff, foff, soff) = out[True], out[False]
+    _say(f"horizon gate, form {form}: {son} with the gate on; {int(fon.sum())} of {na} sources in view")
+    assert son["tested"] > 0 and son["skipped"] > 0.3 * son["tested"]
+    assert soff["skipped"] == 0
+    assert 0 < fon.sum() < na
+    np.testing.assert_array_equal(fon, foff)
+    np.testing.assert_array_equal(gon.view(np.uint32), goff.view(np.uint32))
+    if form == _lib.FORM_PLAIN:
+        orc = OracleScene(sc, seed)
+        o, _, _ = orc.render_band(0)
+        compare_iq(gon, o, "gated plain form vs oracle")
