@@ -353,6 +353,25 @@ int lpk_set_horizon_gate(lpk_ctx *ctx, int on);
 /* (tile, transmitter) pairs skipped and tested since lpk_solve_stats_reset */
 int lpk_horizon_gate_stats(lpk_ctx *ctx, uint64_t stats[2]);
 
+/* ---- output ----
+ * Streams whole bands to raw little-endian complex64 files, the FileSink /
+ * StreamingC64Writer of src/pipeline.rs:398-406 and src/output/mod.rs:95-139
+ * (the JSON sidecar is the caller's): chunk k of every band renders on the
+ * device while the host writes chunk k - 1 from pinned, double-buffered
+ * memory. bands: whole-band descriptors (i0 = 0, clen = the band's length, no
+ * out_dev / out_host / adc_codes_host); paths[b]: band b's file, truncated.
+ * The bytes equal those of chunked lpk_render_chunk calls of any chunk size.
+ * Synchronous. */
+int lpk_stream_to_files(lpk_ctx *ctx, const lpk_band_chunk *bands, uint32_t n_bands, const char *const *paths,
+                        uint64_t chunk_samples, int form);
+/* export_truth's geometry (src/pipeline.rs:548-611) for receiver rx over the
+ * epoch-relative grid t_rel[n]: range[q * n + k] of every uploaded transmitter
+ * q from solve_emission_times with `iterations` iterations (the reference uses
+ * 4), and src_xyz[(q * 3 + c) * n + k] the source position at epoch + t_rel[k].
+ * Synchronous; host arrays. */
+int lpk_truth(lpk_ctx *ctx, uint32_t rx, const double *t_rel, uint64_t n, double epoch_unix, int iterations,
+              double *range, double *src_xyz);
+
 #ifdef __cplusplus
 }
 #endif

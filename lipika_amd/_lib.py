@@ -173,6 +173,8 @@ def load() -> C.CDLL:
         "lpk_solve_stats_reset": (_i32, [C.c_void_p]),
         "lpk_set_horizon_gate": (_i32, [C.c_void_p, C.c_int]),
         "lpk_horizon_gate_stats": (_i32, [C.c_void_p, P(_u64)]),
+        "lpk_stream_to_files": (_i32, [C.c_void_p, P(BandChunk), _u32, P(C.c_char_p), _u64, C.c_int]),
+        "lpk_truth": (_i32, [C.c_void_p, _u32, P(_d), _u64, _d, C.c_int, P(_d), P(_d)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -196,6 +198,7 @@ EXPORTED = (
     "lpk_lws_create", "lpk_lws_destroy", "lpk_lws_get_params", "lpk_lws_set_params", "lpk_lws_generate",
     "lpk_lws_infer", "lpk_lws_gradient", "lpk_lws_train", "lpk_set_warm_start", "lpk_solve_stats",
     "lpk_solve_stats_reset", "lpk_set_horizon_gate", "lpk_horizon_gate_stats",
+    "lpk_stream_to_files", "lpk_truth",
 )
 
 

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cerrno>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -32,6 +33,8 @@ hipError_t launch_lws_update(const float *partials, int n_rows, float *theta, fl
                              float lr, float beta1, float beta2, float eps, float c1, float c2, float *grad_out,
                              float *loss_out, hipStream_t s);
 hipError_t launch_lws_pack(const float *theta, void *packed, hipStream_t s);
+hipError_t launch_truth(const TxDev *txs, uint32_t n_tx, const double *obs, double epoch, const double *t, uint64_t n,
+                        int iterations, double *range, double *src, hipStream_t stream);
 hipError_t launch_walk_gen(const WalkSegDev *segs, int n_segs, uint32_t n_blocks, const uint32_t *n_live,
                            hipStream_t stream);
 hipError_t launch_walk_carry(const CarrySegDev *segs, int n_segs, hipStream_t stream);
@@ -1656,6 +1659,105 @@ int lpk_solve_stats_reset(lpk_ctx *ctx) {
     return guarded(ctx, [&] {
         ensure_stats(ctx);
         hip_check(hipMemsetAsync(ctx->stats, 0, kNumStats * sizeof(unsigned long long), ctx->stream), "counter reset");
+    });
+}
+
+// ---------------------------------------------------------------- output
+
+int lpk_stream_to_files(lpk_ctx *ctx, const lpk_band_chunk *bands, uint32_t n_bands, const char *const *paths,
+                        uint64_t chunk_samples, int form) {
+    if (!ctx || (n_bands && (!bands || !paths)) || chunk_samples == 0) return LPK_E_INVALID;
+    return guarded(ctx, [&] {
+        struct Files {
+            std::vector<FILE *> f;
+            ~Files() {
+                for (FILE *p : f)
+                    if (p) std::fclose(p);
+            }
+        } files;
+        struct Pinned {
+            void *p[2] = {nullptr, nullptr};
+            ~Pinned() {
+                for (void *q : p)
+                    if (q) (void)hipHostFree(q);
+            }
+        } pin;
+        uint64_t slot_samples = 0, n_chunks = 0;
+        for (uint32_t b = 0; b < n_bands; ++b) {
+            const lpk_band_chunk &B = bands[b];
+            if (B.i0 != 0 || B.out_dev || B.out_host || B.adc_codes_host || !paths[b])
+                throw Fail{LPK_E_INVALID, "band " + std::to_string(b) +
+                                              ": whole-band descriptor (i0 = 0, no outputs) and a path expected"};
+            slot_samples += std::min<uint64_t>(B.clen, chunk_samples);
+            n_chunks = std::max<uint64_t>(n_chunks, (B.clen + chunk_samples - 1) / chunk_samples);
+            FILE *f = std::fopen(paths[b], "wb");
+            if (!f) throw Fail{LPK_E_INVALID, std::string("cannot open ") + paths[b] + ": " + std::strerror(errno)};
+            files.f.push_back(f);
+        }
+        for (int i = 0; i < 2; ++i)
+            if (slot_samples && hipHostMalloc(&pin.p[i], slot_samples * 8, hipHostMallocDefault) != hipSuccess)
+                throw Fail{LPK_E_NOMEM, "pinned output buffers"};
+        // chunk k of band b lands at off[b] of slot k % 2; the host writes
+        // chunk k - 1 while the device renders chunk k
+        std::vector<uint64_t> off(n_bands, 0), len_prev(n_bands, 0), len_cur(n_bands, 0);
+        for (uint32_t b = 1; b < n_bands; ++b) off[b] = off[b - 1] + std::min<uint64_t>(bands[b - 1].clen, chunk_samples);
+        auto write_slot = [&](int slot, const std::vector<uint64_t> &len) {
+            const char *base = static_cast<const char *>(pin.p[slot]);
+            for (uint32_t b = 0; b < n_bands; ++b)
+                if (len[b] && std::fwrite(base + off[b] * 8, 8, len[b], files.f[b]) != len[b])
+                    throw Fail{LPK_E_INVALID, std::string("write failed: ") + paths[b]};
+        };
+        std::vector<lpk_band_chunk> jobs;
+        for (uint64_t k = 0; k < n_chunks; ++k) {
+            const int slot = int(k & 1);
+            jobs.clear();
+            for (uint32_t b = 0; b < n_bands; ++b) {
+                const uint64_t i0 = k * chunk_samples;
+                len_cur[b] = i0 < bands[b].clen ? std::min(chunk_samples, bands[b].clen - i0) : 0;
+                if (!len_cur[b]) continue;
+                lpk_band_chunk J = bands[b];
+                J.i0 = i0;
+                J.clen = len_cur[b];
+                J.out_host = static_cast<char *>(pin.p[slot]) + off[b] * 8;
+                if (J.delta_t) J.delta_t += i0;
+                jobs.push_back(J);
+            }
+            render(ctx, jobs.data(), uint32_t(jobs.size()), form, nullptr);
+            if (k) write_slot(slot ^ 1, len_prev);
+            hip_check(hipStreamSynchronize(ctx->stream), "chunk render");
+            std::swap(len_prev, len_cur);
+        }
+        if (n_chunks) write_slot(int((n_chunks - 1) & 1), len_prev);
+        for (uint32_t b = 0; b < n_bands; ++b)
+            if (std::fflush(files.f[b]) != 0) throw Fail{LPK_E_INVALID, std::string("write failed: ") + paths[b]};
+    });
+}
+
+int lpk_truth(lpk_ctx *ctx, uint32_t rx, const double *t_rel, uint64_t n, double epoch_unix, int iterations,
+              double *range, double *src_xyz) {
+    if (!ctx || (n && (!t_rel || !range || !src_xyz)) || iterations < 0) return LPK_E_INVALID;
+    return guarded(ctx, [&] {
+        if (!ctx->have_scene) throw Fail{LPK_E_STATE, "no scene uploaded"};
+        if (rx >= ctx->rx.size()) throw Fail{LPK_E_INVALID, "receiver index out of range"};
+        const uint64_t nt = ctx->n_tx;
+        if (!n || !nt) return;
+        DevBuf tb, rb, sb;
+        struct Rel {
+            DevBuf *b[3];
+            ~Rel() {
+                for (DevBuf *q : b) q->release();
+            }
+        } rel{{&tb, &rb, &sb}};
+        double *td = static_cast<double *>(tb.ensure(n * 8, ctx->stream));
+        double *rd = static_cast<double *>(rb.ensure(nt * n * 8, ctx->stream));
+        double *sd = static_cast<double *>(sb.ensure(nt * 3 * n * 8, ctx->stream));
+        hip_check(hipMemcpyAsync(td, t_rel, n * 8, hipMemcpyHostToDevice, ctx->stream), "truth grid upload");
+        hip_check(launch_truth(ctx->tx_dev, uint32_t(nt), ctx->rx[rx].d.pos_ecef, epoch_unix, td, n, iterations, rd, sd,
+                               ctx->stream),
+                  "truth launch");
+        hip_check(hipMemcpyAsync(range, rd, nt * n * 8, hipMemcpyDeviceToHost, ctx->stream), "truth download");
+        hip_check(hipMemcpyAsync(src_xyz, sd, nt * 3 * n * 8, hipMemcpyDeviceToHost, ctx->stream), "truth download");
+        hip_check(hipStreamSynchronize(ctx->stream), "truth");
     });
 }
 

@@ -1036,6 +1036,47 @@ __global__ void __launch_bounds__(256) lws_predict_kernel(const JobDev *__restri
     if (valid && lane < 32) pred[q] = T.kep_on ? tau_sl + static_cast<double>(y) * kLwsTargetScale : tau_sl;
 }
 
+// ---------------------------------------------------------------- truth export
+
+// export_truth's geometry (src/pipeline.rs:548-611) for one receiver: for
+// transmitter q and grid time t[k] (epoch-relative), the range of
+// solve_emission_times with `iterations` iterations (the reference uses 4 on
+// this coarse grid) and the source position at epoch + t[k]. One lane per
+// (transmitter, time).
+__global__ void __launch_bounds__(256) truth_kernel(const TxDev *__restrict__ txs, uint32_t n_tx, double ox, double oy,
+                                                    double oz, double epoch, const double *__restrict__ t, uint64_t n,
+                                                    int iterations, double *__restrict__ range,
+                                                    double *__restrict__ src) {
+    const uint64_t g = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= uint64_t(n_tx) * n) return;
+    const uint32_t q = uint32_t(g / n);
+    const uint64_t k = g - uint64_t(q) * n;
+    const TxDev &T = txs[q];
+    const double tu = t[k];
+    const double obs[3] = {ox, oy, oz};
+    double tt = tu - kEmissionInitialLag, r = 0.0;
+    for (int it = 0; it < iterations; ++it) {
+        double sx, sy, sz;
+        int64_t cell;
+        tx_position(T, epoch + tt, sx, sy, sz, cell);
+        const double wtau = kEarthRotation * (tu - tt);
+        const double sx_rot = sx + wtau * sy;
+        const double sy_rot = sy - wtau * sx;
+        const double dx = sx_rot - obs[0];
+        const double dy = sy_rot - obs[1];
+        const double dz = sz - obs[2];
+        r = sqrt(dx * dx + dy * dy + dz * dz);
+        tt = tu - r * kInvC;
+    }
+    range[g] = r;
+    double sx, sy, sz;
+    int64_t cell;
+    tx_position(T, epoch + tu, sx, sy, sz, cell);
+    src[(uint64_t(q) * 3 + 0) * n + k] = sx;
+    src[(uint64_t(q) * 3 + 1) * n + k] = sy;
+    src[(uint64_t(q) * 3 + 2) * n + k] = sz;
+}
+
 // ---------------------------------------------------------------- clock walk
 
 // One 65 536-index block of the walk per workgroup, over the segments of
@@ -1189,6 +1230,17 @@ hipError_t launch_lws_predict(const JobDev *jobs, int n_jobs, const LinkDev *lin
     if (blocks > 0x7FFFFFFFull) return hipErrorInvalidValue;
     hipLaunchKernelGGL(lws_predict_kernel, dim3(uint32_t(blocks)), dim3(256), 0, stream, jobs, n_jobs, links, txs,
                        net, tile_shift, n_queries, pred);
+    return hipGetLastError();
+}
+
+hipError_t launch_truth(const TxDev *txs, uint32_t n_tx, const double *obs, double epoch, const double *t, uint64_t n,
+                        int iterations, double *range, double *src, hipStream_t stream) {
+    const uint64_t total = uint64_t(n_tx) * n;
+    if (!total) return hipSuccess;
+    const uint64_t blocks = (total + 255) / 256;
+    if (blocks > 0x7FFFFFFFull) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(truth_kernel, dim3(uint32_t(blocks)), dim3(256), 0, stream, txs, n_tx, obs[0], obs[1], obs[2],
+                       epoch, t, n, iterations, range, src);
     return hipGetLastError();
 }
 

@@ -693,13 +693,7 @@ def run_scene(scene: SceneConfig, rng_seed_base: int, chunk_samples: int = DEFAU
         for w, r, b in units:
             rx = sc.receivers[r]
             pb = rx.bands[b]
-            desc = {
-                "receiver": rx.name, "band_id": pb.band.band_id, "duration_s": sc.meta.duration_s,
-                "band": {"lo_carrier_hz": pb.band.lo_carrier_hz, "sample_rate_hz": pb.band.sample_rate_hz,
-                         "bandwidth_hz": pb.band.effective_bandwidth_hz(),
-                         "noise_temperature_k": pb.band.noise_temperature_k, "noise_power_w": pb.noise_power_w},
-                "contributing_transmitters": [sc.transmitters[i].name for i in pb.active],
-            }
+            desc = band_descriptor(scene, rx.name, pb.band, list(pb.active), pb.noise_power_w)
             result.append(ObserverOutput(f"{rx.name}__{pb.band.band_id}", sc.meta.epoch_unix,
                                          pb.band.sample_rate_hz, pb.band.lo_carrier_hz, outs[w], desc))
         return result
@@ -769,3 +763,9 @@ def render_window(sky: SkyConfig, receiver: ReceiverConfig, band: ReceiverBandCo
     finally:
         if own:
             be.close()
+
+
+# The writers (src/output/mod.rs, src/pipeline.rs:494-611), re-exported here
+# beside run_scene.
+from .output import (OutputManifest, band_descriptor, export_truth, run_scene_to_dir,  # noqa: E402
+                     write_raw_with_sidecar)
