@@ -799,7 +799,7 @@ void render(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form,
         JobDev J = fill_job(ctx, B, B.i0, B.clen, form, lh + link_off, perm);
         J.flicker = B.delta_t ? nullptr : ensure_flicker(ctx, ctx->rx[B.rx], B.i0 + B.clen, B.fs_hz);
         J.tile0 = uint32_t(tiles);
-        J.n_tiles = tiles_of(B.i0, B.clen, ts);
+        J.n_tiles = tiles_of(B.i0, B.clen, span_shift(form, any_antenna, ts));
         tiles += J.n_tiles;
         if (tiles > 0xFFFFFFFFull) throw Fail{LPK_E_INVALID, "render call holds more than 2^32 tiles"};
         J.link0 = uint32_t(link_off);
@@ -977,7 +977,7 @@ void plan_fill(lpk_plan *P, uint64_t k, char *dst) {
         JobDev J = fill_job(ctx, B, i0, clen, P->form, lh + P->link0[b], perm);
         J.flicker = B.delta_t ? nullptr : static_cast<const double *>(ctx->rx[B.rx].flick.dev.p);
         J.tile0 = uint32_t(tiles);
-        J.n_tiles = tiles_of(i0, clen, P->tile_shift);
+        J.n_tiles = tiles_of(i0, clen, span_shift(P->form, P->antenna, P->tile_shift));
         tiles += J.n_tiles;
         J.link0 = uint32_t(P->link0[b]);
         if (B.has_antenna) J.unocc = flags_dev;
@@ -1075,7 +1075,7 @@ lpk_plan *plan_create(lpk_ctx *ctx, const lpk_band_chunk *bands, uint32_t n_band
         for (uint32_t b = 0; b < n_bands; ++b) {
             uint64_t i0, clen;
             plan_chunk(P.get(), b, k, i0, clen);
-            tiles += tiles_of(i0, clen, P->tile_shift);
+            tiles += tiles_of(i0, clen, span_shift(P->form, P->antenna, P->tile_shift));
         }
         if (blocks > 0xFFFFFFull || tiles > 0xFFFFFFFFull) throw Fail{LPK_E_INVALID, "plan step too large"};
         P->max_blocks = std::max<uint32_t>(P->max_blocks, uint32_t(blocks));

@@ -49,6 +49,14 @@ static_assert(kWalkThreads * kWalkPerThread == (1 << kWalkBlockShift), "walk blo
 // lane; tiles sit at absolute multiples of the tile size in a band's sample
 // index. The tile size is a tuning choice (128, 256 or 512 samples).
 constexpr int kDefaultTileShift = 8;
+// The precision form without an antenna works on super-tiles of
+// 2^kSuperShift tiles per workgroup: the anchor solves of all of them run as
+// one batch, so their latency is paid once per super-tile.
+constexpr int kSuperShift = 2;
+// log2 of the samples one workgroup covers (the unit of a job's tile count).
+__host__ __device__ constexpr int span_shift(int form, bool antenna, int tile_shift) {
+    return tile_shift + (form == 1 && !antenna ? kSuperShift : 0);
+}
 // A dispatch packet counts its grid in work-items with 32 bits, so one
 // launch covers at most 2^32 / 512 - 1 tiles of the largest size; a render
 // call is sliced into launches of kMaxLaunchTiles.

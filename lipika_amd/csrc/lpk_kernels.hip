@@ -413,31 +413,71 @@ struct TapOut {
 // a quadratic in tu through three anchor samples, plus the first-order effect
 // of the reference's rounding of the absolute query time epoch + t
 // (src/channel/mod.rs:79), which the lanes compute exactly per sample. The
-// first part of the record holds everything the sample loop reads on its
-// fast path, so the loop touches no transmitter table in global memory.
-constexpr int32_t kLinkSmooth = 1;
-constexpr int32_t kLinkGated = 2;   // below the antenna cutoff for the whole tile
+// carrier phase (src/channel/mod.rs:256-257) is linear in range, delta_t and
+// t_local, so it is carried in turns from its value at the middle sample,
+// reduced there once in double-double. The first part of the record holds
+// everything the sample loop reads on its fast path, so the loop touches no
+// transmitter table in global memory but the C/A chips.
 struct alignas(16) Anchor {
     // read by every lane of the sample loop
     double r0m, q0d1, q0c2, rd0;   // r0(tu) = r0m + x (q0d1 + q0c2 x) + rd0 del0, x = tu - tu_mid
     double r1m, q1d1, q1c2, rd1q;  // r1(tu) = r1m + x (q1d1 + q1c2 x) + rd1q del1 - k01 del0
     double k01;                    // (d r1 / d t1, position and Sagnac) * rd0 / c
     double r1_mid;                 // range at the middle sample (model, or exact when not smooth)
-    double neg_tau_src, tau_if;    // -TAU f_src, TAU (f_src - f_lo)
-    double epoch_rel, rate;        // signal epoch; symbol rate (framed) or omega (CW)
-    double hu;                     // symbol (framed) or NAV bit (C/A) index near the tile's first sample
-    const int8_t *code;            // C/A chips
+    union {
+        struct {                   // written by the fit, read by the sample loop
+            double ph0;            // carrier phase at the middle sample, turns in [-1/2, 1/2]
+            double ph_r, ph_d, ph_t;  // its derivatives in turns per metre of range, second of delta_t and of t_local
+            double epoch_rel, rate;   // signal epoch; symbol rate (framed) or omega (CW)
+            double hu, hu1;        // symbol (framed) or NAV bit (C/A) index near the tile's first sample, and the next
+            double cbase;          // C/A: a chip count just before the tile's first sample
+            const int8_t *code;    // C/A chips
+        };
+        struct {                   // the anchor pass's results, read by the fit
+            double r0s[3], r1s[3]; // smooth ranges of iterations 1 and 2 at the anchors
+            int64_t cell[2][2];    // grid nodes of both iterations at the first and last anchors
+        };
+    };
     float amp_m;                   // (float)(amp_factor / r1_mid)
-    float inv_rm;                  // 1 / r1_mid
+    float amp_k;                   // amp_m / r1_mid: amplitude slope per metre of range
     float h0re, h0im;              // value of index hu: symbol (framed) or NAV sign (C/A, h0re)
-    float h1re, h1im;              // value of index hu + 1
-    int32_t flags, _pad0;          // kLink* bits
-    uint32_t tx, _pad;             // transmitter (slow paths read its table)
-    // anchor pass only
-    double r0s[3], r1s[3];         // smooth ranges of iterations 1 and 2 at the anchors
-    int64_t cell[2][2];            // grid nodes of both iterations at the first and last anchors
+    float h1re, h1im;              // value of index hu1
+    uint32_t cmod;                 // C/A: cbase mod 1023
+    uint32_t tx;                   // transmitter (slow paths read its table)
+    double neg_tau_src, tau_if;    // -TAU f_src, TAU (f_src - f_lo), for taps
 };
+static_assert(sizeof(Anchor) == 208, "Anchor layout");
 constexpr int kAnchorGroup = 64;  // transmitters per anchor pass
+
+// A wave-uniform 64-bit value from two 32-bit halves, in scalar registers.
+__device__ __forceinline__ uint64_t uniform_u64(uint32_t lo, uint32_t hi) {
+    const uint32_t l = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(lo)));
+    const uint32_t h = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(hi)));
+    return (static_cast<uint64_t>(h) << 32) | l;
+}
+constexpr double kInvTau = 0.15915494309189535;
+
+// An exact Cody-Waite reduction of hi + lo radians (24 + 24 + 53-bit split of
+// 2 pi, exact for |k| < 2^29) to turns in [-1/2, 1/2].
+__device__ __forceinline__ double reduce_to_turns(double hi, double lo) {
+    const double kk = rint(hi * kInvTau);
+    const double rr =
+        fma(-kk, -6.8604979977715316e-15, fma(-kk, -1.7484555314695172e-07, fma(-kk, 6.2831854820251465, hi)));
+    return (rr + lo) * kInvTau;
+}
+
+// The carrier phase neg_tau_src (r / c + dlt) + tau_if t_local in turns,
+// evaluated in double-double (error products and sums exact) before the
+// reduction, so that it is the phase of the reference's f64 inputs to ~1e-16
+// turns.
+__device__ __forceinline__ double carrier_turns(double neg_tau_src, double tau_if, double r, double dlt, double t) {
+    const double p = r * kInvC, pe = fma(r, kInvC, -p);
+    const double d = p + dlt, dv = d - p, de = (p - (d - dv)) + (dlt - dv) + pe;
+    const double a = neg_tau_src * d, ae = fma(neg_tau_src, d, -a) + neg_tau_src * de;
+    const double b = tau_if * t, be = fma(tau_if, t, -b);
+    const double s = a + b, sv = s - a, se = (a - (s - sv)) + (b - sv);
+    return reduce_to_turns(s, se + ae + be);
+}
 
 // The two-iteration solve with the trajectory queried at unrounded times
 // ((epoch - t0) + t instead of epoch + t); also returns the node cells and
@@ -502,37 +542,28 @@ __device__ __forceinline__ void solve_smooth(const TxDev &T, const double *obs, 
 
 // One (sample, transmitter) contribution of the precision form, for the
 // transmitter group G (kGroup*): the fitted range with the per-sample rounding
-// corrections, s(t_emit), the carrier phase in the reference's operation
-// order reduced exactly, the hardware sine and cosine, the f32 mix, and the
-// exact f64 sum. Slow paths (a tile crossing a grid node, a symbol or NAV bit
-// other than the tile's hints, gates) read the transmitter's table.
+// corrections (`smooth`, wave-uniform) or the exact solve, s(t_emit), the
+// carrier phase in turns from the middle sample's, the hardware sine and
+// cosine, the f32 mix, and the exact f64 sum. Slow paths (a tile crossing a
+// grid node, a symbol, NAV bit or chip outside the tile's hints, gates) read
+// the transmitter's table. ddl and dtl are delta_t and t_local less their
+// values at the middle sample.
 template <int G, bool ANT>
-__device__ __forceinline__ void precision_link(const Anchor &A, const TxDev *__restrict__ txs, const JobDev &J,
-                                               uint32_t link, const double *obs, double epoch, double tu,
-                                               double t_local, double dlt, double x, double del0, double &sum_re,
-                                               double &sum_im, bool tap_here, double &tap_te, double &tap_r,
-                                               double &tap_ph) {
-    if (ANT && (A.flags & kLinkGated)) return;   // below the cutoff for the whole tile
+__device__ __forceinline__ void precision_link(const Anchor &A, bool smooth, const TxDev *__restrict__ txs,
+                                               const JobDev &J, uint32_t link, double tu, double x, double del0,
+                                               double ddl, double dtl, double &sum_re, double &sum_im) {
     double r;
-    if (A.flags & kLinkSmooth) {
+    if (smooth) {
         const double r0 = fma(A.rd0, del0, fma(x, fma(A.q0c2, x, A.q0d1), A.r0m));
         const double t1 = fma(-r0, kInvC, tu);
-        const double del1 = ((epoch + t1) - epoch) - t1;
+        const double del1 = ((J.epoch + t1) - J.epoch) - t1;
         r = fma(A.rd1q, del1, fma(-A.k01, del0, fma(x, fma(A.q1c2, x, A.q1d1), A.r1m)));
     } else {
         double te;
         int64_t c0, c1;
-        solve_emission(txs[A.tx], obs, tu, epoch, te, r, c0, c1);
+        solve_emission(txs[A.tx], J.obs, tu, J.epoch, te, r, c0, c1);
     }
     const double t_emit = fma(-r, kInvC, tu);
-    // src/channel/mod.rs:256-258 in the reference's operation order, so its
-    // f64 rounding is followed
-    const double phase = A.neg_tau_src * (r * kInvC + dlt) + A.tau_if * t_local;
-    if (tap_here) {
-        tap_te = t_emit;
-        tap_r = r;
-        tap_ph = phase;
-    }
     // antenna gain; an occluded sample adds nothing (src/channel/mod.rs:253-255)
     float g = 1.0f;
     if (ANT && J.ant_on) {
@@ -541,26 +572,35 @@ __device__ __forceinline__ void precision_link(const Anchor &A, const TxDev *__r
         if (g == 0.0f) return;
     }
     // s(t_emit): the exact index decides; the tile's hints supply the value
-    // when the index is one they hold. CW's own phase joins the carrier phase.
+    // when the index is one they hold (read here, before the phase work, so
+    // that their latency is covered), and the rare other indices are
+    // evaluated after it. CW's own phase joins the carrier phase.
+    typedef const __attribute__((address_space(1))) int8_t gchip;
     float bre = 0.0f, bim = 0.0f;
     double ph_cw = 0.0;
+    bool hinted = true;
+    double sig_b = 0.0;
+    int chip = 0;
     if (G == kGroupCa) {
         const double dt = t_emit - A.epoch_rel;
-        const double b = floor(dt * kNavBitRate);
-        const float pm = b == A.hu ? A.h0re : (b == A.hu + 1.0 ? A.h1re : float(nav_pm(txs[A.tx], b)));
-        const int64_t ci = rem_euclid(floor_to_i64(floor(dt * kCaChipRate)), kCaCodeLength);
-        bre = static_cast<float>(A.code[ci]) * pm;
+        sig_b = floor(dt * kNavBitRate);
+        const double sig_k = floor(dt * kCaChipRate);
+        // the chip index mod 1023 from the tile's base when the count is
+        // within one code period of it (NaN fails the test)
+        const double d = sig_k - A.cbase;
+        uint32_t ix = A.cmod + static_cast<uint32_t>(static_cast<int32_t>(d));
+        ix = min(ix, ix - static_cast<uint32_t>(kCaCodeLength));
+        if (!(d >= 0.0 && d < static_cast<double>(kCaCodeLength)))
+            ix = static_cast<uint32_t>(rem_euclid(floor_to_i64(sig_k), kCaCodeLength));
+        chip = ((gchip *)A.code)[ix];
+        bre = sig_b == A.hu ? A.h0re : A.h1re;
+        hinted = sig_b == A.hu || sig_b == A.hu1;
     } else if (G == kGroupFramed) {
-        const double u = floor((t_emit - A.epoch_rel) * A.rate);
-        if (u == A.hu) {
-            bre = A.h0re;
-            bim = A.h0im;
-        } else if (u == A.hu + 1.0) {
-            bre = A.h1re;
-            bim = A.h1im;
-        } else {
-            framed_value(txs[A.tx], u, bre, bim);
-        }
+        sig_b = floor((t_emit - A.epoch_rel) * A.rate);
+        const bool h0 = sig_b == A.hu;
+        bre = h0 ? A.h0re : A.h1re;
+        bim = h0 ? A.h0im : A.h1im;
+        hinted = h0 || sig_b == A.hu1;
     } else if (G == kGroupCw) {
         ph_cw = static_cast<double>(static_cast<float>(A.rate * t_emit));  // rate holds omega
         bre = 1.0f;
@@ -575,16 +615,22 @@ __device__ __forceinline__ void precision_link(const Anchor &A, const TxDev *__r
             }
         }
     }
-    // an exact Cody-Waite reduction of the phase (24 + 24 + 53-bit split of
-    // 2 pi, exact for |k| < 2^29) and the hardware sine and cosine of the
-    // reduced turn fraction
-    const double pt = phase + ph_cw;
-    const double kk = rint(pt * 0.15915494309189535);
-    const double rr =
-        fma(-kk, -6.8604979977715316e-15, fma(-kk, -1.7484555314695172e-07, fma(-kk, 6.2831854820251465, pt)));
-    const float fr = static_cast<float>(rr * 0.15915494309189535);
+    // the carrier phase in turns (linear in its three inputs), its fraction,
+    // and the hardware sine and cosine of that fraction
+    const double dr = r - A.r1_mid;
+    double turns = fma(A.ph_r, dr, fma(A.ph_d, ddl, fma(A.ph_t, dtl, A.ph0)));
+    if (G == kGroupCw || G == kGroupGeneric) turns = fma(ph_cw, kInvTau, turns);
+    const float fr = static_cast<float>(__builtin_amdgcn_fract(turns));
     const float spf = __builtin_amdgcn_sinf(fr), cpf = __builtin_amdgcn_cosf(fr);
-    const float amplitude = A.amp_m * (1.0f - static_cast<float>(r - A.r1_mid) * A.inv_rm) * g;
+    // amp_factor / r to first order about the middle sample's range
+    float amplitude = fmaf(-A.amp_k, static_cast<float>(dr), A.amp_m);
+    if (ANT) amplitude *= g;
+    if (G == kGroupCa) {
+        if (!hinted) bre = static_cast<float>(nav_pm(txs[A.tx], sig_b));
+        bre *= static_cast<float>(chip);
+    } else if (G == kGroupFramed) {
+        if (!hinted) framed_value(txs[A.tx], sig_b, bre, bim);
+    }
     float cre, cim;
     if (G == kGroupCa || G == kGroupCw) {
         const float ab = amplitude * bre;
@@ -597,6 +643,311 @@ __device__ __forceinline__ void precision_link(const Anchor &A, const TxDev *__r
     // the f32 contributions summed exactly in f64
     sum_re += static_cast<double>(cre);
     sum_im += static_cast<double>(cim);
+}
+
+// The tapped link's emission time, range and carrier phase in the precision
+// form: the range as the sample loop forms it, the phase in the reference's
+// f64 operation order (src/channel/mod.rs:255-258) from that range.
+__device__ __forceinline__ void precision_tap(const Anchor &A, bool smooth, const TxDev *__restrict__ txs,
+                                              const JobDev &J, double tu, double t_local, double dlt, double x,
+                                              double del0, double &te, double &r, double &ph) {
+    if (smooth) {
+        const double r0 = fma(A.rd0, del0, fma(x, fma(A.q0c2, x, A.q0d1), A.r0m));
+        const double t1 = fma(-r0, kInvC, tu);
+        const double del1 = ((J.epoch + t1) - J.epoch) - t1;
+        r = fma(A.rd1q, del1, fma(-A.k01, del0, fma(x, fma(A.q1c2, x, A.q1d1), A.r1m)));
+    } else {
+        int64_t c0, c1;
+        solve_emission(txs[A.tx], J.obs, tu, J.epoch, te, r, c0, c1);
+    }
+    te = fma(-r, kInvC, tu);
+    ph = A.neg_tau_src * (r * kInvC + dlt) + A.tau_if * t_local;
+}
+
+// Anchor pass of one (tile, transmitter) record: the smooth solve at anchor a
+// (0 first, 1 middle, 2 last valid sample of the tile) of reception time tu.
+__device__ __forceinline__ void anchor_solve(Anchor &A, const TxDev &T, const JobDev &J, double tu, uint32_t a) {
+    const double eoff = T.traj_kind ? J.epoch - T.grid_t0 : 0.0;
+    double r0, r1, rd0, rd1q, rd1t;
+    int64_t c0, c1;
+    solve_smooth(T, J.obs, tu, eoff, r0, r1, c0, c1, rd0, rd1q, rd1t);
+    A.r0s[a] = r0;
+    A.r1s[a] = r1;
+    if (a != 1) {
+        A.cell[a >> 1][0] = c0;
+        A.cell[a >> 1][1] = c1;
+    } else {
+        A.rd0 = rd0;
+        A.rd1q = rd1q;
+        A.k01 = rd1t * rd0 * kInvC;
+    }
+}
+
+// The fit of one (tile, transmitter) record from its anchor pass; at[a] holds
+// (tu, t_local, delta_t) of anchor a. Returns whether the tile is smooth for
+// the transmitter (the lanes then use the fitted range, else the exact solve).
+__device__ __forceinline__ bool anchor_fit(Anchor &A, const LinkDev &L, const TxDev &T, const JobDev &J,
+                                           const double (*at)[3], bool fit_ok) {
+    // the anchor pass's results share storage with fields written below
+    const double r0s[3] = {A.r0s[0], A.r0s[1], A.r0s[2]};
+    const double r1s[3] = {A.r1s[0], A.r1s[1], A.r1s[2]};
+    const bool same = A.cell[0][0] == A.cell[1][0] && A.cell[0][1] == A.cell[1][1];
+    const bool smooth = fit_ok && (T.traj_kind == 0 || same);
+    double r1m;
+    A.r0m = r0s[1];
+    A.r1m = r1s[1];
+    if (smooth) {
+        const double xa = at[0][0] - at[1][0], xb = at[2][0] - at[1][0];
+        const double a0 = (r0s[1] - r0s[0]) / (-xa), b0 = (r0s[2] - r0s[1]) / xb;
+        const double a1 = (r1s[1] - r1s[0]) / (-xa), b1 = (r1s[2] - r1s[1]) / xb;
+        // Newton form r_m + x (d1 + c2 (x - xa)), with the -c2 xa
+        // term folded into the linear coefficient
+        A.q0c2 = (b0 - a0) / (xb - xa);
+        A.q0d1 = fma(-A.q0c2, xa, a0);
+        A.q1c2 = (b1 - a1) / (xb - xa);
+        A.q1d1 = fma(-A.q1c2, xa, a1);
+        // the model at the middle sample (x = 0)
+        const double t0m = at[1][0] - kEmissionInitialLag;
+        const double d0m = ((J.epoch + t0m) - J.epoch) - t0m;
+        const double r0m = fma(A.rd0, d0m, r0s[1]);
+        const double t1m = fma(-r0m, kInvC, at[1][0]);
+        const double d1m = ((J.epoch + t1m) - J.epoch) - t1m;
+        r1m = fma(A.rd1q, d1m, fma(-A.k01, d0m, r1s[1]));
+    } else {
+        // the exact solve at the middle sample (the lanes solve exactly too)
+        double te;
+        int64_t c0, c1;
+        solve_emission(T, J.obs, at[1][0], J.epoch, te, r1m, c0, c1);
+    }
+    A.r1_mid = r1m;
+    const float amp_m = static_cast<float>(T.amp_factor / r1m);
+    A.amp_m = amp_m;
+    A.amp_k = amp_m * static_cast<float>(1.0 / r1m);
+    A.neg_tau_src = T.neg_tau_src;
+    A.tau_if = L.tau_if;
+    A.ph0 = carrier_turns(T.neg_tau_src, L.tau_if, r1m, at[1][2], at[1][1]);
+    A.ph_r = T.neg_tau_src * kInvTau * kInvC;
+    A.ph_d = T.neg_tau_src * kInvTau;
+    A.ph_t = L.tau_if * kInvTau;
+    A.epoch_rel = T.epoch_rel;
+    A.rate = T.signal_kind == 0 ? T.omega : T.rate;
+    A.code = T.ca_code;
+    A.tx = L.tx;
+    // Values of the symbol (or NAV bit) the tile starts in and of the next
+    // one; lanes take them when their own exact index matches and evaluate
+    // the signal otherwise. For C/A, a chip count one before the first
+    // sample's and its residue.
+    const double te0 = at[0][0] - r1s[0] * kInvC;
+    A.hu = 0.0;
+    A.hu1 = 1.0;
+    A.cbase = 0.0;
+    A.cmod = 0u;
+    A.h0re = A.h0im = A.h1re = A.h1im = 0.0f;
+    if (T.signal_kind >= 1 && T.signal_kind <= 3) {
+        A.hu = floor((te0 - T.epoch_rel) * T.rate);
+        A.hu1 = A.hu + 1.0;
+        framed_value(T, A.hu, A.h0re, A.h0im);
+        framed_value(T, A.hu1, A.h1re, A.h1im);
+    } else if (T.signal_kind == 4) {
+        A.hu = floor((te0 - T.epoch_rel) * kNavBitRate);
+        A.hu1 = A.hu + 1.0;
+        A.h0re = static_cast<float>(nav_pm(T, A.hu));
+        A.h1re = static_cast<float>(nav_pm(T, A.hu1));
+        const double cb = floor((te0 - T.epoch_rel) * kCaChipRate) - 1.0;
+        if (fabs(cb) < 4503599627370496.0) {   // 2^52: d = k - cbase stays exact
+            A.cbase = cb;
+            A.cmod = static_cast<uint32_t>(rem_euclid(floor_to_i64(cb), kCaCodeLength));
+        } else {
+            A.cbase = __builtin_nan("");      // every lane takes the general index
+        }
+    }
+    return smooth;
+}
+
+// One sample's contributions from the links [g0, g0 + ng) of its job, whose
+// records are R[0..ng): the transmitter groups in order, links whose bit is
+// set in gmask skipped (antenna kernels), the fitted range for those whose
+// bit is set in smask.
+template <bool ANT>
+__device__ __forceinline__ void precision_links(const Anchor *R, uint64_t smask, uint64_t gmask,
+                                                const TxDev *__restrict__ txs, const JobDev &J, uint32_t g0,
+                                                uint32_t ng, double tu, double x, double del0, double ddl, double dtl,
+                                                double &sum_re, double &sum_im) {
+    const uint32_t e_ca = J.group_end[kGroupCa] > g0 ? min(J.group_end[kGroupCa] - g0, ng) : 0u;
+    const uint32_t e_fr = J.group_end[kGroupFramed] > g0 ? min(J.group_end[kGroupFramed] - g0, ng) : 0u;
+    const uint32_t e_cw = J.group_end[kGroupCw] > g0 ? min(J.group_end[kGroupCw] - g0, ng) : 0u;
+    uint32_t s = 0;
+    for (; s < e_ca; ++s)
+        if (!ANT || !((gmask >> s) & 1u))
+            precision_link<kGroupCa, ANT>(R[s], (smask >> s) & 1u, txs, J, g0 + s, tu, x, del0, ddl, dtl, sum_re,
+                                          sum_im);
+    for (; s < e_fr; ++s)
+        if (!ANT || !((gmask >> s) & 1u))
+            precision_link<kGroupFramed, ANT>(R[s], (smask >> s) & 1u, txs, J, g0 + s, tu, x, del0, ddl, dtl, sum_re,
+                                              sum_im);
+    for (; s < e_cw; ++s)
+        if (!ANT || !((gmask >> s) & 1u))
+            precision_link<kGroupCw, ANT>(R[s], (smask >> s) & 1u, txs, J, g0 + s, tu, x, del0, ddl, dtl, sum_re,
+                                          sum_im);
+    for (; s < ng; ++s)
+        if (!ANT || !((gmask >> s) & 1u))
+            precision_link<kGroupGeneric, ANT>(R[s], (smask >> s) & 1u, txs, J, g0 + s, tu, x, del0, ddl, dtl, sum_re,
+                                               sum_im);
+}
+
+// RxChain::apply_post_sum (src/rx/mod.rs:142-146 and :83-101) and the
+// stores of output sample j.
+__device__ __forceinline__ void finish_sample(const JobDev &J, float acc_re, float acc_im, uint64_t j) {
+    int16_t code_re = 0, code_im = 0;
+    for (int p = 0; p < J.n_post; ++p) {
+        const PostDev &P = J.post[p];
+        if (P.kind == 0) {
+            acc_re *= P.a;
+            acc_im *= P.a;
+        } else if (P.kind == 1) {
+            const float f = 1.0f - P.a * (acc_re * acc_re + acc_im * acc_im);
+            acc_re *= f;
+            acc_im *= f;
+        } else {
+            const float fs = P.a, lsb = P.b;
+            const float xr = acc_re < -fs ? -fs : (acc_re > fs ? fs : acc_re);
+            const float xi = acc_im < -fs ? -fs : (acc_im > fs ? fs : acc_im);
+            const float qr = roundf(xr / lsb), qi = roundf(xi / lsb);
+            acc_re = qr * lsb;
+            acc_im = qi * lsb;
+            code_re = static_cast<int16_t>(qr);
+            code_im = static_cast<int16_t>(qi);
+        }
+    }
+    reinterpret_cast<float2 *>(J.out)[j] = make_float2(acc_re, acc_im);
+    if (J.codes) reinterpret_cast<short2 *>(J.codes)[j] = make_short2(code_re, code_im);
+}
+
+// The precision form without an antenna over one span of 2^kSuperShift
+// tiles (`span` counts spans from the job's first). The anchor solves and fits
+// of several tiles run as one batch (as many (tile, transmitter) records as
+// kAnchorGroup holds), so the latency of their dependent global reads is
+// paid once per batch; the tiles of the batch then run their sample loops in
+// turn. Per tile the arithmetic is that of a single-tile workgroup.
+template <int TS>
+__device__ __forceinline__ void precision_span(const JobDev &J, uint32_t span, const LinkDev *__restrict__ links,
+                                               const TxDev *__restrict__ txs, const TapOut &tap) {
+    constexpr int kTile = 1 << TS, kSub = 1 << kSuperShift, kWaves = kTile / 64;
+    __shared__ uint64_t s_noise[2][kTile];       // double-buffered Philox words, one per sample
+    __shared__ Anchor s_rec[kAnchorGroup];        // record kk * ng + s: tile kk of the batch, link g0 + s
+    __shared__ double s_at[kSub][3][3];           // (tu, t_local, delta_t) at each tile's anchors
+    __shared__ uint32_t s_mask[kSub][2];          // smooth links of each tile of the batch
+    const uint64_t span0 = ((J.i0 >> (TS + kSuperShift)) + span) << (TS + kSuperShift);
+    const uint64_t jend = J.i0 + J.clen;
+    const uint32_t na = J.n_active;
+    const int kb_max = na == 0 ? kSub : (na >= uint32_t(kAnchorGroup) ? 1 : min(kSub, kAnchorGroup / int(na)));
+    const bool noisy = J.sigma > 0.0f;
+    int nbuf = 0;
+    for (int k0 = 0; k0 < kSub; k0 += kb_max) {
+        const int kb = min(kb_max, kSub - k0);
+        double sum_re = 0.0, sum_im = 0.0;
+        for (uint32_t g0 = 0; g0 == 0 || g0 < na; g0 += kAnchorGroup) {
+            const uint32_t ng = na - g0 < uint32_t(kAnchorGroup) ? na - g0 : uint32_t(kAnchorGroup);
+            const bool last_group = g0 + ng >= na;
+            // the anchor samples of each tile of the batch (src/pipeline.rs:918-926)
+            if (threadIdx.x < 3 * kb) {
+                const int kk = threadIdx.x / 3, a = threadIdx.x - 3 * kk;
+                const uint64_t t0 = span0 + uint64_t(k0 + kk) * kTile;
+                const uint64_t first = J.i0 > t0 ? J.i0 : t0, end = jend < t0 + kTile ? jend : t0 + kTile;
+                if (first < end) {
+                    const uint64_t ia = a == 0 ? first : (a == 1 ? (first + end - 1) >> 1 : end - 1);
+                    const double tl = static_cast<double>(ia) / J.fs;
+                    const double dl = J.delta_t ? J.delta_t[ia - J.i0] : clock_delta(J, tl);
+                    s_at[kk][a][0] = tl - dl;
+                    s_at[kk][a][1] = tl;
+                    s_at[kk][a][2] = dl;
+                }
+            }
+            __syncthreads();
+            // smooth solves at the three anchors of every record
+            for (uint32_t q = threadIdx.x; q < 3 * kb * ng; q += kTile) {
+                const uint32_t rec = q / 3, a = q - 3 * rec, kk = rec / ng;
+                const uint64_t t0 = span0 + uint64_t(k0 + kk) * kTile;
+                if ((J.i0 > t0 ? J.i0 : t0) >= (jend < t0 + kTile ? jend : t0 + kTile)) continue;
+                const LinkDev L = links[J.link0 + g0 + (rec - kk * ng)];
+                anchor_solve(s_rec[rec], txs[L.tx], J, s_at[kk][a][0], a);
+            }
+            __syncthreads();
+            // the fits: wave w takes tiles w, w + kWaves, ... of the batch, one link a lane
+            for (int kr = 0; kr < kb; kr += kWaves) {
+                const int kk = kr + static_cast<int>(threadIdx.x >> 6);
+                const uint32_t s = threadIdx.x & 63u;
+                bool smooth = false;
+                if (kk < kb && s < ng) {
+                    const uint64_t t0 = span0 + uint64_t(k0 + kk) * kTile;
+                    const uint64_t first = J.i0 > t0 ? J.i0 : t0, end = jend < t0 + kTile ? jend : t0 + kTile;
+                    if (first < end) {
+                        const LinkDev L = links[J.link0 + g0 + s];
+                        smooth = anchor_fit(s_rec[kk * ng + s], L, txs[L.tx], J, s_at[kk], end - 1 - first >= 2);
+                    }
+                }
+                const uint64_t m = __ballot(smooth);
+                if (kk < kb && s == 0) {
+                    s_mask[kk][0] = static_cast<uint32_t>(m);
+                    s_mask[kk][1] = static_cast<uint32_t>(m >> 32);
+                }
+            }
+            __syncthreads();
+            for (int kk = 0; kk < kb; ++kk) {
+                const uint64_t t0 = span0 + uint64_t(k0 + kk) * kTile;
+                const uint64_t first = J.i0 > t0 ? J.i0 : t0, end = jend < t0 + kTile ? jend : t0 + kTile;
+                if (first >= end) continue;   // a tile with no sample of its job (uniform)
+                // AWGN draws of the tile: kTile/4 Philox blocks, consumed after the loops
+                if (last_group && noisy && threadIdx.x < kTile / 4) {
+                    uint64_t w[4] = {(t0 >> 2) + threadIdx.x, 0, 0, 0};
+                    philox4x64(w, J.noise_seed, kAwgnTag);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) s_noise[nbuf][4 * threadIdx.x + q] = w[q];
+                }
+                const uint64_t i = t0 + threadIdx.x;
+                const bool valid = i >= J.i0 && i < jend;
+                const uint64_t j = i - J.i0;
+                const uint64_t ic = valid ? i : first;
+                const double t_local = static_cast<double>(ic) / J.fs;
+                const double dlt = J.delta_t ? J.delta_t[ic - J.i0] : clock_delta(J, t_local);
+                const double tu = t_local - dlt;
+                const double x = tu - s_at[kk][1][0];
+                // rounding of the iteration-1 query time, exact (Sterbenz) per sample
+                const double t0q = tu - kEmissionInitialLag;
+                const double del0 = ((J.epoch + t0q) - J.epoch) - t0q;
+                // the phase's per-sample inputs less the middle sample's
+                const double ddl = dlt - s_at[kk][1][2], dtl = t_local - s_at[kk][1][1];
+                const uint64_t smask = uniform_u64(s_mask[kk][0], s_mask[kk][1]);
+                const Anchor *R = s_rec + kk * ng;
+                precision_links<false>(R, smask, 0ull, txs, J, g0, ng, tu, x, del0, ddl, dtl, sum_re, sum_im);
+                if (tap.on && valid && tap.slot >= g0 && tap.slot < g0 + ng) {
+                    const uint32_t ts = tap.slot - g0;
+                    double te, r, ph;
+                    precision_tap(R[ts], (smask >> ts) & 1u, txs, J, tu, t_local, dlt, x, del0, te, r, ph);
+                    tap.t_emit[j] = te;
+                    tap.range[j] = r;
+                    tap.phase[j] = ph;
+                }
+                if (last_group) {
+                    if (tap.on && valid) tap.delta_t[j] = dlt;
+                    __syncthreads();   // the tile's Philox words are in LDS
+                    float acc_re = static_cast<float>(sum_re), acc_im = static_cast<float>(sum_im);
+                    // AWGN, src/channel/mod.rs:328-333 (sigma = sqrt(P/2) as f32)
+                    if (noisy) {
+                        float zr, zi;
+                        box_muller(s_noise[nbuf][threadIdx.x], zr, zi);
+                        acc_re += J.sigma * zr;
+                        acc_im += J.sigma * zi;
+                    }
+                    nbuf ^= 1;
+                    if (valid) finish_sample(J, acc_re, acc_im, j);
+                    sum_re = 0.0;
+                    sum_im = 0.0;
+                }
+            }
+            if (!last_group) __syncthreads();   // the next link group rewrites the records
+        }
+    }
 }
 
 // Workgroups are dealt round-robin to the 8 XCDs; give each XCD a contiguous
@@ -622,9 +973,10 @@ __global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM == 1 ? 5 : 6))
     render_kernel(const JobDev *__restrict__ jobs, int n_jobs, const LinkDev *__restrict__ links,
                   const TxDev *__restrict__ txs, RenderLaunch lx, TapOut tap) {
     constexpr int kTile = 1 << TS;
-    __shared__ uint64_t s_noise[kTile];              // one Philox word per sample
-    __shared__ double s_anc_t[3][3];                  // tu, t_local, delta_t at the anchors
-    __shared__ Anchor s_anc[FORM == 1 ? kAnchorGroup : 1];
+    constexpr bool kSpan = FORM == 1 && !ANT;          // the super-tile path has LDS of its own
+    __shared__ uint64_t s_noise[kSpan ? 1 : kTile];    // one Philox word per sample
+    __shared__ double s_anc_t[3][3];                   // tu, t_local, delta_t at the anchors
+    __shared__ Anchor s_anc[FORM == 1 && ANT ? kAnchorGroup : 1];
     __shared__ unsigned long long s_stats[4];
 
     const uint32_t tile = lx.tile_base + (lx.xcd_map ? xcd_tile(blockIdx.x, lx.n_tiles) : blockIdx.x);
@@ -636,6 +988,10 @@ __global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM == 1 ? 5 : 6))
         else hi = mid - 1;
     }
     const JobDev &J = jobs[lo];
+    if constexpr (kSpan) {
+        precision_span<TS>(J, tile - J.tile0, links, txs, tap);
+        return;
+    }
     const uint64_t tile_start = ((J.i0 >> TS) + (tile - J.tile0)) << TS;
     const uint64_t i = tile_start + threadIdx.x;
     const bool valid = i >= J.i0 && i < J.i0 + J.clen;
@@ -831,118 +1187,53 @@ __global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM == 1 ? 5 : 6))
         // rounding of the iteration-1 query time, exact (Sterbenz) per sample
         const double t0q = tu - kEmissionInitialLag;
         const double del0 = ((J.epoch + t0q) - J.epoch) - t0q;
+        // the phase's per-sample inputs less the middle sample's
+        const double ddl = dlt - s_anc_t[1][2], dtl = t_local - s_anc_t[1][1];
         const bool fit_ok = a_last - a_first >= 2;
+        // the group's smooth and gated links as bits (wave 0 holds the group)
+        __shared__ uint32_t s_mask[4];
         for (uint32_t g0 = 0; g0 < J.n_active; g0 += kAnchorGroup) {
             const uint32_t ng = J.n_active - g0 < kAnchorGroup ? J.n_active - g0 : kAnchorGroup;
             // smooth solves at the three anchors of every transmitter of the group
             for (uint32_t q = threadIdx.x; q < 3 * ng; q += kTile) {
                 const uint32_t s = q / 3, a = q - 3 * s;
                 const LinkDev L = links[J.link0 + g0 + s];
-                const TxDev &T = txs[L.tx];
-                Anchor &A = s_anc[s];
-                const double eoff = T.traj_kind ? J.epoch - T.grid_t0 : 0.0;
-                double r0, r1, rd0, rd1q, rd1t;
-                int64_t c0, c1;
-                solve_smooth(T, J.obs, s_anc_t[a][0], eoff, r0, r1, c0, c1, rd0, rd1q, rd1t);
-                A.r0s[a] = r0;
-                A.r1s[a] = r1;
-                if (a != 1) {
-                    A.cell[a >> 1][0] = c0;
-                    A.cell[a >> 1][1] = c1;
-                } else {
-                    A.rd0 = rd0;
-                    A.rd1q = rd1q;
-                    A.k01 = rd1t * rd0 * kInvC;
-                }
+                anchor_solve(s_anc[s], txs[L.tx], J, s_anc_t[a][0], a);
             }
             __syncthreads();
+            bool smooth = false, gated = false;
             for (uint32_t s = threadIdx.x; s < ng; s += kTile) {
-                Anchor &A = s_anc[s];
                 const LinkDev L = links[J.link0 + g0 + s];
                 const TxDev &T = txs[L.tx];
-                const bool same = A.cell[0][0] == A.cell[1][0] && A.cell[0][1] == A.cell[1][1];
-                const bool smooth = fit_ok && (T.traj_kind == 0 || same);
-                double r1m;
-                A.r0m = A.r0s[1];
-                A.r1m = A.r1s[1];
-                if (smooth) {
-                    const double xa = s_anc_t[0][0] - s_anc_t[1][0], xb = s_anc_t[2][0] - s_anc_t[1][0];
-                    const double a0 = (A.r0s[1] - A.r0s[0]) / (-xa), b0 = (A.r0s[2] - A.r0s[1]) / xb;
-                    const double a1 = (A.r1s[1] - A.r1s[0]) / (-xa), b1 = (A.r1s[2] - A.r1s[1]) / xb;
-                    // Newton form r_m + x (d1 + c2 (x - xa)), with the -c2 xa
-                    // term folded into the linear coefficient
-                    A.q0c2 = (b0 - a0) / (xb - xa);
-                    A.q0d1 = fma(-A.q0c2, xa, a0);
-                    A.q1c2 = (b1 - a1) / (xb - xa);
-                    A.q1d1 = fma(-A.q1c2, xa, a1);
-                    // the model at the middle sample (x = 0)
-                    const double t0m = s_anc_t[1][0] - kEmissionInitialLag;
-                    const double d0m = ((J.epoch + t0m) - J.epoch) - t0m;
-                    const double r0m = fma(A.rd0, d0m, A.r0s[1]);
-                    const double t1m = fma(-r0m, kInvC, s_anc_t[1][0]);
-                    const double d1m = ((J.epoch + t1m) - J.epoch) - t1m;
-                    r1m = fma(A.rd1q, d1m, fma(-A.k01, d0m, A.r1s[1]));
-                } else {
-                    // the exact solve at the middle sample (the lanes solve exactly too)
-                    double te;
-                    int64_t c0, c1;
-                    solve_emission(T, J.obs, s_anc_t[1][0], J.epoch, te, r1m, c0, c1);
+                smooth = anchor_fit(s_anc[s], L, T, J, s_anc_t, fit_ok);
+                gated = ANT && gating && g_ok && gate_below_cutoff(J, T, g_tmin, g_tmax);
+                if (ANT && gating && lx.gate_stats) {
+                    if (gated) atomicAdd(&lx.gate_stats[0], 1ull);
+                    atomicAdd(&lx.gate_stats[1], 1ull);
                 }
-                A.r1_mid = r1m;
-                A.inv_rm = static_cast<float>(1.0 / r1m);
-                A.amp_m = static_cast<float>(T.amp_factor / r1m);
-                A.neg_tau_src = T.neg_tau_src;
-                A.tau_if = L.tau_if;
-                A.epoch_rel = T.epoch_rel;
-                A.rate = T.signal_kind == 0 ? T.omega : T.rate;
-                A.code = T.ca_code;
-                A.flags = smooth ? kLinkSmooth : 0;
-                if (ANT && gating && g_ok && gate_below_cutoff(J, T, g_tmin, g_tmax)) {
-                    A.flags |= kLinkGated;
-                    if (lx.gate_stats) atomicAdd(&lx.gate_stats[0], 1ull);
-                }
-                if (ANT && gating && lx.gate_stats) atomicAdd(&lx.gate_stats[1], 1ull);
-                A.tx = L.tx;
-                // Values of the symbol (or NAV bit) the tile starts in and of
-                // the next one; lanes take them when their own exact index
-                // matches and evaluate the signal otherwise.
-                const double te0 = s_anc_t[0][0] - A.r1s[0] * kInvC;
-                A.hu = 0.0;
-                A.h0re = A.h0im = A.h1re = A.h1im = 0.0f;
-                if (T.signal_kind >= 1 && T.signal_kind <= 3) {
-                    A.hu = floor((te0 - T.epoch_rel) * T.rate);
-                    framed_value(T, A.hu, A.h0re, A.h0im);
-                    framed_value(T, A.hu + 1.0, A.h1re, A.h1im);
-                } else if (T.signal_kind == 4) {
-                    A.hu = floor((te0 - T.epoch_rel) * kNavBitRate);
-                    A.h0re = static_cast<float>(nav_pm(T, A.hu));
-                    A.h1re = static_cast<float>(nav_pm(T, A.hu + 1.0));
+            }
+            // wave 0 holds threads 0..63, hence every link of the group
+            if (threadIdx.x < 64) {
+                const uint64_t ms = __ballot(threadIdx.x < ng && smooth);
+                const uint64_t mg = __ballot(threadIdx.x < ng && gated);
+                if (threadIdx.x == 0) {
+                    s_mask[0] = static_cast<uint32_t>(ms);
+                    s_mask[1] = static_cast<uint32_t>(ms >> 32);
+                    s_mask[2] = static_cast<uint32_t>(mg);
+                    s_mask[3] = static_cast<uint32_t>(mg >> 32);
                 }
             }
             __syncthreads();
-            // the sample loops, one per transmitter group
-            const uint32_t e_ca = J.group_end[kGroupCa] > g0 ? min(J.group_end[kGroupCa] - g0, ng) : 0u;
-            const uint32_t e_fr = J.group_end[kGroupFramed] > g0 ? min(J.group_end[kGroupFramed] - g0, ng) : 0u;
-            const uint32_t e_cw = J.group_end[kGroupCw] > g0 ? min(J.group_end[kGroupCw] - g0, ng) : 0u;
-            const bool tapping = tap.on && valid && tap.slot >= g0 && tap.slot < g0 + ng;
-            double tv0 = 0.0, tv1 = 0.0, tv2 = 0.0;
-            uint32_t s = 0;
-            for (; s < e_ca; ++s)
-                precision_link<kGroupCa, ANT>(s_anc[s], txs, J, g0 + s, J.obs, J.epoch, tu, t_local, dlt, x, del0, sum_re, sum_im,
-                                         tapping && g0 + s == tap.slot, tv0, tv1, tv2);
-            for (; s < e_fr; ++s)
-                precision_link<kGroupFramed, ANT>(s_anc[s], txs, J, g0 + s, J.obs, J.epoch, tu, t_local, dlt, x, del0, sum_re,
-                                             sum_im, tapping && g0 + s == tap.slot, tv0, tv1, tv2);
-            for (; s < e_cw; ++s)
-                precision_link<kGroupCw, ANT>(s_anc[s], txs, J, g0 + s, J.obs, J.epoch, tu, t_local, dlt, x, del0, sum_re, sum_im,
-                                         tapping && g0 + s == tap.slot, tv0, tv1, tv2);
-            for (; s < ng; ++s)
-                precision_link<kGroupGeneric, ANT>(s_anc[s], txs, J, g0 + s, J.obs, J.epoch, tu, t_local, dlt, x, del0, sum_re,
-                                              sum_im, tapping && g0 + s == tap.slot, tv0, tv1, tv2);
-            if (tapping) {
-                tap.t_emit[j] = tv0;
-                tap.range[j] = tv1;
-                tap.phase[j] = tv2;
+            const uint64_t smask = uniform_u64(s_mask[0], s_mask[1]);
+            const uint64_t gmask = uniform_u64(s_mask[2], s_mask[3]);
+            precision_links<ANT>(s_anc, smask, gmask, txs, J, g0, ng, tu, x, del0, ddl, dtl, sum_re, sum_im);
+            if (tap.on && valid && tap.slot >= g0 && tap.slot < g0 + ng) {
+                const uint32_t ts = tap.slot - g0;
+                double te, r, ph;
+                precision_tap(s_anc[ts], (smask >> ts) & 1u, txs, J, tu, t_local, dlt, x, del0, te, r, ph);
+                tap.t_emit[j] = te;
+                tap.range[j] = r;
+                tap.phase[j] = ph;
             }
             __syncthreads();
         }
@@ -960,30 +1251,7 @@ __global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM == 1 ? 5 : 6))
         acc_im += J.sigma * zi;
     }
     if (!valid) return;
-    // RxChain::apply_post_sum, src/rx/mod.rs:142-146 and :83-101
-    int16_t code_re = 0, code_im = 0;
-    for (int p = 0; p < J.n_post; ++p) {
-        const PostDev &P = J.post[p];
-        if (P.kind == 0) {
-            acc_re *= P.a;
-            acc_im *= P.a;
-        } else if (P.kind == 1) {
-            const float f = 1.0f - P.a * (acc_re * acc_re + acc_im * acc_im);
-            acc_re *= f;
-            acc_im *= f;
-        } else {
-            const float fs = P.a, lsb = P.b;
-            const float xr = acc_re < -fs ? -fs : (acc_re > fs ? fs : acc_re);
-            const float xi = acc_im < -fs ? -fs : (acc_im > fs ? fs : acc_im);
-            const float qr = roundf(xr / lsb), qi = roundf(xi / lsb);
-            acc_re = qr * lsb;
-            acc_im = qi * lsb;
-            code_re = static_cast<int16_t>(qr);
-            code_im = static_cast<int16_t>(qi);
-        }
-    }
-    reinterpret_cast<float2 *>(J.out)[j] = make_float2(acc_re, acc_im);
-    if (J.codes) reinterpret_cast<short2 *>(J.codes)[j] = make_short2(code_re, code_im);
+    finish_sample(J, acc_re, acc_im, j);
 }
 
 // ---------------------------------------------------------------- learned start

@@ -231,6 +231,14 @@ def test_precision_form_error_vs_fp64_oracle(backend, name):
             _say(f"precision form, {name}, band {b}: n={len(o)} max|d|/peak={d.max() / peak:.3e}")
             assert np.all(np.isfinite(g))
             assert d.max() <= 2e-6 * peak
+            if len(pb.active) == 1:
+                # one transmitter: the output's own carrier phase against the
+                # oracle's, sample by sample (both rounded to complex64)
+                on = np.abs(o) >= 0.1 * peak
+                dphi = np.abs(np.angle(g[on].astype(np.complex128) * np.conj(o[on].astype(np.complex128))))
+                _say(f"precision form, {name}, band {b}: output phase max|dphi|={dphi.max():.3e} rad "
+                     f"over {int(on.sum())} samples")
+                assert on.sum() > 0.5 * len(o) and dphi.max() <= 1e-6
             n = min(pb.n, 120_000)
             for slot in range(len(pb.active)):
                 compare_taps(backend, orc, sc, w, r, b, seed, slot, n, form=_lib.FORM_PRECISION,
