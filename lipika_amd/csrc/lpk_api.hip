@@ -412,6 +412,13 @@ TxDev make_tx(lpk_ctx *ctx, const lpk_tx_desc &d, uint32_t idx) {
         T.nav_mode = d.nav_mode;
         if (d.nav_mode < LPK_NAV_OFF || d.nav_mode > LPK_NAV_FULL) throw bad("unknown nav mode");
         T.ca_code = static_cast<const int8_t *>(upload(ctx, d.ca_code, 1023));
+        {
+            // the chips as f32, twice over: the precision form indexes it from
+            // a chip residue plus an offset below one code period, unwrapped
+            float chips2[2 * 1023];
+            for (int c = 0; c < 2 * 1023; ++c) chips2[c] = static_cast<float>(d.ca_code[c % 1023]);
+            T.ca_chips2 = static_cast<const float *>(upload(ctx, chips2, sizeof(chips2)));
+        }
         T.nav_seed = d.nav_seed;
         if (d.nav_mode == LPK_NAV_FULL && d.n_nav_bits) {
             if (!d.nav_bits) throw bad("full nav needs its bit stream");

@@ -52,7 +52,7 @@ constexpr int kDefaultTileShift = 8;
 // The precision form without an antenna works on super-tiles of
 // 2^kSuperShift tiles per workgroup: the anchor solves of all of them run as
 // one batch, so their latency is paid once per super-tile.
-constexpr int kSuperShift = 2;
+constexpr int kSuperShift = 3;
 // log2 of the samples one workgroup covers (the unit of a job's tile count).
 __host__ __device__ constexpr int span_shift(int form, bool antenna, int tile_shift) {
     return tile_shift + (form == 1 && !antenna ? kSuperShift : 0);
@@ -109,6 +109,7 @@ struct TxDev {
     int32_t nav_mode;
     int32_t gated;
     const int8_t *ca_code;
+    const float *ca_chips2;  // the chips as f32, the code twice over (2046 values)
     uint64_t nav_seed;
     const int8_t *nav_bits;
     int64_t n_nav_bits;

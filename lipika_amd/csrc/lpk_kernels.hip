@@ -431,7 +431,7 @@ struct alignas(16) Anchor {
             double epoch_rel, rate;   // signal epoch; symbol rate (framed) or omega (CW)
             double hu, hu1;        // symbol (framed) or NAV bit (C/A) index near the tile's first sample, and the next
             double cbase;          // C/A: a chip count just before the tile's first sample
-            const int8_t *code;    // C/A chips
+            const float *chip0;    // C/A: the doubled f32 chip table from residue cmod on
         };
         struct {                   // the anchor pass's results, read by the fit
             double r0s[3], r1s[3]; // smooth ranges of iterations 1 and 2 at the anchors
@@ -448,6 +448,7 @@ struct alignas(16) Anchor {
 };
 static_assert(sizeof(Anchor) == 208, "Anchor layout");
 constexpr int kAnchorGroup = 64;  // transmitters per anchor pass
+constexpr int kSpanRecords = 128; // (tile, transmitter) records of one anchor batch of a span
 
 // A wave-uniform 64-bit value from two 32-bit halves, in scalar registers.
 __device__ __forceinline__ uint64_t uniform_u64(uint32_t lo, uint32_t hi) {
@@ -575,24 +576,24 @@ __device__ __forceinline__ void precision_link(const Anchor &A, bool smooth, con
     // when the index is one they hold (read here, before the phase work, so
     // that their latency is covered), and the rare other indices are
     // evaluated after it. CW's own phase joins the carrier phase.
-    typedef const __attribute__((address_space(1))) int8_t gchip;
+    typedef const __attribute__((address_space(1))) float gchip;
     float bre = 0.0f, bim = 0.0f;
     double ph_cw = 0.0;
     bool hinted = true;
     double sig_b = 0.0;
-    int chip = 0;
+    float chip = 0.0f;
     if (G == kGroupCa) {
         const double dt = t_emit - A.epoch_rel;
         sig_b = floor(dt * kNavBitRate);
         const double sig_k = floor(dt * kCaChipRate);
-        // the chip index mod 1023 from the tile's base when the count is
-        // within one code period of it (NaN fails the test)
+        // the chip at residue cmod + d of the doubled table when the count is
+        // within one code period of the tile's base (NaN fails the test), else
+        // at the count's own residue
         const double d = sig_k - A.cbase;
-        uint32_t ix = A.cmod + static_cast<uint32_t>(static_cast<int32_t>(d));
-        ix = min(ix, ix - static_cast<uint32_t>(kCaCodeLength));
+        int32_t off = static_cast<int32_t>(d);
         if (!(d >= 0.0 && d < static_cast<double>(kCaCodeLength)))
-            ix = static_cast<uint32_t>(rem_euclid(floor_to_i64(sig_k), kCaCodeLength));
-        chip = ((gchip *)A.code)[ix];
+            off = static_cast<int32_t>(rem_euclid(floor_to_i64(sig_k), kCaCodeLength)) - static_cast<int32_t>(A.cmod);
+        chip = ((gchip *)A.chip0)[off];
         bre = sig_b == A.hu ? A.h0re : A.h1re;
         hinted = sig_b == A.hu || sig_b == A.hu1;
     } else if (G == kGroupFramed) {
@@ -627,7 +628,7 @@ __device__ __forceinline__ void precision_link(const Anchor &A, bool smooth, con
     if (ANT) amplitude *= g;
     if (G == kGroupCa) {
         if (!hinted) bre = static_cast<float>(nav_pm(txs[A.tx], sig_b));
-        bre *= static_cast<float>(chip);
+        bre *= chip;
     } else if (G == kGroupFramed) {
         if (!hinted) framed_value(txs[A.tx], sig_b, bre, bim);
     }
@@ -731,7 +732,7 @@ __device__ __forceinline__ bool anchor_fit(Anchor &A, const LinkDev &L, const Tx
     A.ph_t = L.tau_if * kInvTau;
     A.epoch_rel = T.epoch_rel;
     A.rate = T.signal_kind == 0 ? T.omega : T.rate;
-    A.code = T.ca_code;
+    A.chip0 = T.ca_chips2;
     A.tx = L.tx;
     // Values of the symbol (or NAV bit) the tile starts in and of the next
     // one; lanes take them when their own exact index matches and evaluate
@@ -757,6 +758,7 @@ __device__ __forceinline__ bool anchor_fit(Anchor &A, const LinkDev &L, const Tx
         if (fabs(cb) < 4503599627370496.0) {   // 2^52: d = k - cbase stays exact
             A.cbase = cb;
             A.cmod = static_cast<uint32_t>(rem_euclid(floor_to_i64(cb), kCaCodeLength));
+            A.chip0 = T.ca_chips2 + A.cmod;
         } else {
             A.cbase = __builtin_nan("");      // every lane takes the general index
         }
@@ -826,7 +828,7 @@ __device__ __forceinline__ void finish_sample(const JobDev &J, float acc_re, flo
 // The precision form without an antenna over one span of 2^kSuperShift
 // tiles (`span` counts spans from the job's first). The anchor solves and fits
 // of several tiles run as one batch (as many (tile, transmitter) records as
-// kAnchorGroup holds), so the latency of their dependent global reads is
+// kSpanRecords holds), so the latency of their dependent global reads is
 // paid once per batch; the tiles of the batch then run their sample loops in
 // turn. Per tile the arithmetic is that of a single-tile workgroup.
 template <int TS>
@@ -834,13 +836,13 @@ __device__ __forceinline__ void precision_span(const JobDev &J, uint32_t span, c
                                                const TxDev *__restrict__ txs, const TapOut &tap) {
     constexpr int kTile = 1 << TS, kSub = 1 << kSuperShift, kWaves = kTile / 64;
     __shared__ uint64_t s_noise[2][kTile];       // double-buffered Philox words, one per sample
-    __shared__ Anchor s_rec[kAnchorGroup];        // record kk * ng + s: tile kk of the batch, link g0 + s
+    __shared__ Anchor s_rec[kSpanRecords];        // record kk * ng + s: tile kk of the batch, link g0 + s
     __shared__ double s_at[kSub][3][3];           // (tu, t_local, delta_t) at each tile's anchors
     __shared__ uint32_t s_mask[kSub][2];          // smooth links of each tile of the batch
     const uint64_t span0 = ((J.i0 >> (TS + kSuperShift)) + span) << (TS + kSuperShift);
     const uint64_t jend = J.i0 + J.clen;
     const uint32_t na = J.n_active;
-    const int kb_max = na == 0 ? kSub : (na >= uint32_t(kAnchorGroup) ? 1 : min(kSub, kAnchorGroup / int(na)));
+    const int kb_max = na == 0 ? kSub : (na >= uint32_t(kAnchorGroup) ? 1 : min(kSub, kSpanRecords / int(na)));
     const bool noisy = J.sigma > 0.0f;
     int nbuf = 0;
     for (int k0 = 0; k0 < kSub; k0 += kb_max) {
