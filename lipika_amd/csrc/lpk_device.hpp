@@ -224,6 +224,38 @@ __host__ __device__ __forceinline__ int clz32(uint32_t x) {
 #endif
 }
 
+// Philox4x32-10 (Salmon, Moraes, Dror and Shaw, SC'11; the multipliers and
+// Weyl key increments of their Random123 library) on counter c with key
+// (k0, k1): 32x32 -> 64-bit products, which the vector ALU forms directly.
+__host__ __device__ __forceinline__ void philox4x32(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        const uint64_t p0 = static_cast<uint64_t>(0xD2511F53u) * c[0];
+        const uint64_t p1 = static_cast<uint64_t>(0xCD9E8D57u) * c[2];
+        const uint32_t n0 = static_cast<uint32_t>(p1 >> 32) ^ c[1] ^ k0;
+        const uint32_t n2 = static_cast<uint32_t>(p0 >> 32) ^ c[3] ^ k1;
+        c[0] = n0;
+        c[1] = static_cast<uint32_t>(p1);
+        c[2] = n2;
+        c[3] = static_cast<uint32_t>(p0);
+    }
+}
+
+// Block `block` of the (seed, tag) stream of the clock walk and the AWGN:
+// Philox4x32-10 on counter (block, tag) with key `seed`, as two 64-bit words
+// (outputs 1:0 and 3:2, high half first).
+__host__ __device__ __forceinline__ void philox_words(uint64_t block, uint64_t seed, uint64_t tag, uint64_t w[2]) {
+    uint32_t c[4] = {static_cast<uint32_t>(block), static_cast<uint32_t>(block >> 32), static_cast<uint32_t>(tag),
+                     static_cast<uint32_t>(tag >> 32)};
+    philox4x32(c, static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32));
+    w[0] = (static_cast<uint64_t>(c[1]) << 32) | c[0];
+    w[1] = (static_cast<uint64_t>(c[3]) << 32) | c[2];
+}
+
 // Philox4x64-10 (Salmon, Moraes, Dror, Shaw, SC'11).
 __host__ __device__ __forceinline__ void philox4x64(uint64_t c[4], uint64_t k0, uint64_t k1) {
 #pragma unroll

@@ -899,12 +899,13 @@ __device__ __forceinline__ void precision_span(const JobDev &J, uint32_t span, c
                 const uint64_t t0 = span0 + uint64_t(k0 + kk) * kTile;
                 const uint64_t first = J.i0 > t0 ? J.i0 : t0, end = jend < t0 + kTile ? jend : t0 + kTile;
                 if (first >= end) continue;   // a tile with no sample of its job (uniform)
-                // AWGN draws of the tile: kTile/4 Philox blocks, consumed after the loops
-                if (last_group && noisy && threadIdx.x < kTile / 4) {
-                    uint64_t w[4] = {(t0 >> 2) + threadIdx.x, 0, 0, 0};
-                    philox4x64(w, J.noise_seed, kAwgnTag);
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) s_noise[nbuf][4 * threadIdx.x + q] = w[q];
+                // AWGN draws of the tile: kTile/2 Philox blocks (sample i: block
+                // i >> 1, word i & 1), consumed after the loops
+                if (last_group && noisy && threadIdx.x < kTile / 2) {
+                    uint64_t w[2];
+                    philox_words((t0 >> 1) + threadIdx.x, J.noise_seed, kAwgnTag, w);
+                    s_noise[nbuf][2 * threadIdx.x] = w[0];
+                    s_noise[nbuf][2 * threadIdx.x + 1] = w[1];
                 }
                 const uint64_t i = t0 + threadIdx.x;
                 const bool valid = i >= J.i0 && i < jend;
@@ -999,14 +1000,15 @@ __global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM == 1 ? 5 : 6))
     const bool valid = i >= J.i0 && i < J.i0 + J.clen;
     const uint64_t j = i - J.i0;
 
-    // AWGN draws: the first wavefront computes the tile's kTile/4 Philox
-    // blocks (4 samples per block) into LDS; consumed after the transmitter loop.
+    // AWGN draws: lanes below kTile/2 compute the tile's Philox blocks
+    // (sample i: block i >> 1, word i & 1) into LDS; consumed after the
+    // transmitter loop.
     const bool noisy = J.sigma > 0.0f;
-    if (noisy && threadIdx.x < kTile / 4) {
-        uint64_t w[4] = {(tile_start >> 2) + threadIdx.x, 0, 0, 0};
-        philox4x64(w, J.noise_seed, kAwgnTag);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) s_noise[4 * threadIdx.x + q] = w[q];
+    if (noisy && threadIdx.x < kTile / 2) {
+        uint64_t w[2];
+        philox_words((tile_start >> 1) + threadIdx.x, J.noise_seed, kAwgnTag, w);
+        s_noise[2 * threadIdx.x] = w[0];
+        s_noise[2 * threadIdx.x + 1] = w[1];
     }
 
     // src/pipeline.rs:918-926 (clamped to a valid sample on the tile's edge lanes)
@@ -1351,8 +1353,8 @@ __global__ void __launch_bounds__(256) truth_kernel(const TxDev *__restrict__ tx
 
 // One 65 536-index block of the walk per workgroup, over the segments of
 // every receiver of a render call. Each thread owns 256 consecutive
-// increments (64 Philox blocks, 8 normals per block: master index k uses
-// block k >> 3, word (k >> 1) & 3, and the cos (even k) or sin (odd k) branch
+// increments (64 Philox4x32 blocks, 4 normals per block: master index k uses
+// block k >> 2, word (k >> 1) & 1, and the cos (even k) or sin (odd k) branch
 // of its Box-Muller pair) and sums them sequentially in f64. In window blocks
 // it stores, per index, the prefix since the start of its 32-index sub-block
 // as f32 (a value a few sigma*sqrt(32) in size; staged through LDS so each
@@ -1390,11 +1392,11 @@ __global__ void __launch_bounds__(kWalkThreads) walk_gen_kernel(const WalkSegDev
     for (int m = 0; m < kSubsPerThread; ++m) {
         subs[m] = s;  // thread-local prefix before sub-block m
         const double sub0 = s;
-        for (int q = 0; q < kWalkSub; q += 8) {
-            uint64_t w[4] = {static_cast<uint64_t>((k0 + m * kWalkSub + q) >> 3), 0, 0, 0};
-            philox4x64(w, seed, kClockTag);
+        for (int q = 0; q < kWalkSub; q += 4) {
+            uint64_t w[2];
+            philox_words(static_cast<uint64_t>((k0 + m * kWalkSub + q) >> 2), seed, kClockTag, w);
 #pragma unroll
-            for (int p = 0; p < 4; ++p) {
+            for (int p = 0; p < 2; ++p) {
                 float zc, zs;
                 box_muller(w[p], zc, zs);
                 s = s + (0.0 + sigma * static_cast<double>(zc));

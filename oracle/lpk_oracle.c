@@ -515,6 +515,33 @@ void orc_philox4x64(const uint64_t ctr_in[4], const uint64_t key_in[2], uint64_t
     out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
 
+/* Philox4x32-10 (Salmon, Moraes, Dror and Shaw, SC'11, with the multipliers
+ * and key increments of their Random123 library). */
+void orc_philox4x32(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4]) {
+    uint32_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+    uint32_t k0 = key_in[0], k1 = key_in[1];
+    for (int r = 0; r < 10; ++r) {
+        if (r) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0, n1 = (uint32_t)p1;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1, n3 = (uint32_t)p0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* Word q (0 or 1) of block `block` of a (seed, tag) stream: Philox4x32-10 on
+ * counter (block, tag) with key seed, outputs 2q+1 : 2q as one 64-bit word. */
+static uint64_t stream_word(uint64_t block, uint64_t seed, uint64_t tag, unsigned q) {
+    uint32_t ctr[4] = {(uint32_t)block, (uint32_t)(block >> 32), (uint32_t)tag, (uint32_t)(tag >> 32)};
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)}, out[4];
+    orc_philox4x32(ctr, key, out);
+    return ((uint64_t)out[2 * q + 1] << 32) | out[2 * q];
+}
+
 /* Domain tags that separate the clock-walk and AWGN streams of one seed. */
 #define LPK_CLOCK_TAG 0x636C6F636B77616CULL   /* "clockwal" */
 #define LPK_AWGN_TAG 0x6177676E6E6F6973ULL    /* "awgnnois" */
@@ -581,24 +608,20 @@ void orc_box_muller(uint64_t w, float *zc, float *zs) {
     *zs = (float)(r * sv);
 }
 
-/* Standard normal for master-grid index k of a receiver's clock walk: block
- * k >> 3 of Philox4x64-10 (ctr = (k >> 3, 0, 0, 0), key = (seed, "clockwal")),
- * word (k >> 1) & 3, the cos branch for even k and the sin branch for odd k. */
+/* Standard normal for master-grid index k of a receiver's clock walk: word
+ * (k >> 1) & 1 of block k >> 2 of the (seed, "clockwal") stream, the cos
+ * branch of its Box-Muller pair for even k and the sin branch for odd k. */
 double orc_clock_normal(uint64_t clock_seed, uint64_t k) {
-    uint64_t ctr[4] = {k >> 3, 0, 0, 0}, key[2] = {clock_seed, LPK_CLOCK_TAG}, w[4];
-    orc_philox4x64(ctr, key, w);
     float zc, zs;
-    orc_box_muller(w[(k >> 1) & 3], &zc, &zs);
+    orc_box_muller(stream_word(k >> 2, clock_seed, LPK_CLOCK_TAG, (unsigned)((k >> 1) & 1)), &zc, &zs);
     return (double)((k & 1) ? zs : zc);
 }
 
 /* Two standard normals (re, im) for global sample i of a band's noise stream:
- * block i >> 2 of Philox4x64-10 (ctr = (i >> 2, 0, 0, 0), key = (seed,
- * "awgnnois")), word i & 3, both branches of its Box-Muller pair. */
+ * word i & 1 of block i >> 1 of the (seed, "awgnnois") stream, both branches
+ * of its Box-Muller pair. */
 void orc_awgn_normals(uint64_t noise_seed, uint64_t i, float *z_re, float *z_im) {
-    uint64_t ctr[4] = {i >> 2, 0, 0, 0}, key[2] = {noise_seed, LPK_AWGN_TAG}, w[4];
-    orc_philox4x64(ctr, key, w);
-    orc_box_muller(w[i & 3], z_re, z_im);
+    orc_box_muller(stream_word(i >> 1, noise_seed, LPK_AWGN_TAG, (unsigned)(i & 1)), z_re, z_im);
 }
 
 /* Normal n of a receiver's flicker stream: block n >> 3 of Philox4x64-10

@@ -23,12 +23,13 @@
  * rand_xoshiro 0.6 (restated here: xoshiro256++ seeded by SplitMix64), and the
  * clock walk and AWGN through rand_distr 0.4's ziggurat normals, which are not
  * under /root/reference and are not reproduced. The clock walk and AWGN use
- * Philox4x64-10 (Salmon et al., SC'11) with Box-Muller normals, the same
- * generator the product uses, and are verified statistically against the
- * reference's tests.
+ * Philox4x32-10 (Salmon et al., SC'11), the flicker stream Philox4x64-10, each
+ * with Box-Muller normals, the same generators the product uses, and are
+ * verified statistically against the reference's tests.
  *
- * Parity status of each function is stated in DESIGN.md; functions whose only
- * check is self-consistency are marked "parity unpinned" there.
+ * The checks that pin each function are in tests/ (test_oracle_pins.py and the
+ * per-subsystem tests); a function whose only check is self-consistency is
+ * called "parity unpinned" in the project's design notes.
  */
 #ifndef LPK_ORACLE_H
 #define LPK_ORACLE_H
@@ -145,8 +146,9 @@ void orc_payload_bpsk(const int8_t *sync, uint64_t n_sync, uint64_t n_payload, u
 int orc_payload_ook(const uint8_t *sync, uint64_t n_sync, uint64_t n_payload, uint64_t seed, int8_t *out);
 int orc_payload_qam16(const uint8_t *sync, uint64_t n_sync_bits, uint64_t n_payload, uint64_t seed, float *out_iq);
 
-/* ---- project RNG: Philox4x64-10 and the normals built on it ---- */
+/* ---- project RNG: Philox4x64-10, Philox4x32-10 and the normals built on them ---- */
 void orc_philox4x64(const uint64_t ctr[4], const uint64_t key[2], uint64_t out[4]);
+void orc_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 void orc_box_muller(uint64_t w, float *zc, float *zs);
 double orc_clock_normal(uint64_t clock_seed, uint64_t k);
 void orc_awgn_normals(uint64_t noise_seed, uint64_t i, float *z_re, float *z_im);

@@ -101,6 +101,7 @@ def load() -> C.CDLL:
         "orc_payload_ook": (C.c_int, [P(C.c_uint8), _u64, _u64, _u64, P(C.c_int8)]),
         "orc_payload_qam16": (C.c_int, [P(C.c_uint8), _u64, _u64, _u64, P(C.c_float)]),
         "orc_philox4x64": (None, [P(_u64), P(_u64), P(_u64)]),
+        "orc_philox4x32": (None, [P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]),
         "orc_box_muller": (None, [_u64, P(C.c_float), P(C.c_float)]),
         "orc_clock_normal": (_d, [_u64, _u64]),
         "orc_flicker_normal": (_d, [_u64, _u64]),

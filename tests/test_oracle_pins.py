@@ -418,6 +418,41 @@ def test_philox_matches_numpy(orc):
         assert list(out) == [int(x) for x in ref]
 
 
+def test_philox4x32_known_answers(orc):
+    """Philox4x32-10 against the known-answer vectors published with the
+    Random123 library (kat_vectors: zero, all-ones and pi-digit counters and
+    keys)."""
+    kats = [((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+            ((0xFFFFFFFF,) * 4, (0xFFFFFFFF,) * 2, (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+            ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+             (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1))]
+    for ctr, key, want in kats:
+        out = (C.c_uint32 * 4)()
+        orc.orc_philox4x32((C.c_uint32 * 4)(*ctr), (C.c_uint32 * 2)(*key), out)
+        assert tuple(out) == want
+
+
+def test_stream_words_follow_the_stated_layout(orc):
+    """AWGN sample i is word i & 1 of block i >> 1, clock index k the branch
+    k & 1 of word (k >> 1) & 1 of block k >> 2: counter (block, tag), key seed,
+    word q = outputs 2q+1 : 2q."""
+    awgn_tag, clock_tag = 0x6177676E6E6F6973, 0x636C6F636B77616C
+    zc, zs, zr, zi = C.c_float(), C.c_float(), C.c_float(), C.c_float()
+
+    def word(block, seed, tag, q):
+        ctr = (C.c_uint32 * 4)(block & 0xFFFFFFFF, block >> 32, tag & 0xFFFFFFFF, tag >> 32)
+        out = (C.c_uint32 * 4)()
+        orc.orc_philox4x32(ctr, (C.c_uint32 * 2)(seed & 0xFFFFFFFF, seed >> 32), out)
+        return (out[2 * q + 1] << 32) | out[2 * q]
+
+    for seed, i in ((5, 0), (5, 1), (2**63 + 11, 12345), (1, 2**33 + 7)):
+        orc.orc_awgn_normals(seed, i, C.byref(zr), C.byref(zi))
+        orc.orc_box_muller(word(i >> 1, seed, awgn_tag, i & 1), C.byref(zc), C.byref(zs))
+        assert (zr.value, zi.value) == (zc.value, zs.value)
+        orc.orc_box_muller(word(i >> 2, seed, clock_tag, (i >> 1) & 1), C.byref(zc), C.byref(zs))
+        assert orc.orc_clock_normal(seed, i) == (zs.value if i & 1 else zc.value)
+
+
 def test_box_muller_against_libm(orc):
     """The project's libm-free Box-Muller pair against numpy's f64 log, sqrt,
     cos and sin of the same uniforms, rounded to f32: within one f32 ulp
