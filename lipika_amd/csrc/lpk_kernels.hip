@@ -839,6 +839,7 @@ __device__ __forceinline__ void precision_span(const JobDev &J, uint32_t span, c
     __shared__ Anchor s_rec[kSpanRecords];        // record kk * ng + s: tile kk of the batch, link g0 + s
     __shared__ double s_at[kSub][3][3];           // (tu, t_local, delta_t) at each tile's anchors
     __shared__ uint32_t s_mask[kSub][2];          // smooth links of each tile of the batch
+    __shared__ uint8_t s_smooth[kSpanRecords];     // per record: the tile is smooth for the link
     const uint64_t span0 = ((J.i0 >> (TS + kSuperShift)) + span) << (TS + kSuperShift);
     const uint64_t jend = J.i0 + J.clen;
     const uint32_t na = J.n_active;
@@ -875,20 +876,24 @@ __device__ __forceinline__ void precision_span(const JobDev &J, uint32_t span, c
                 anchor_solve(s_rec[rec], txs[L.tx], J, s_at[kk][a][0], a);
             }
             __syncthreads();
-            // the fits: wave w takes tiles w, w + kWaves, ... of the batch, one link a lane
+            // the fits, one record a lane
+            for (uint32_t rec = threadIdx.x; rec < kb * ng; rec += kTile) {
+                const uint32_t kk = rec / ng;
+                const uint64_t t0 = span0 + uint64_t(k0 + kk) * kTile;
+                const uint64_t first = J.i0 > t0 ? J.i0 : t0, end = jend < t0 + kTile ? jend : t0 + kTile;
+                bool smooth = false;
+                if (first < end) {
+                    const LinkDev L = links[J.link0 + g0 + (rec - kk * ng)];
+                    smooth = anchor_fit(s_rec[rec], L, txs[L.tx], J, s_at[kk], end - 1 - first >= 2);
+                }
+                s_smooth[rec] = smooth ? 1u : 0u;
+            }
+            __syncthreads();
+            // each tile's smooth links as a mask: wave w takes tiles w, w + kWaves, ...
             for (int kr = 0; kr < kb; kr += kWaves) {
                 const int kk = kr + static_cast<int>(threadIdx.x >> 6);
                 const uint32_t s = threadIdx.x & 63u;
-                bool smooth = false;
-                if (kk < kb && s < ng) {
-                    const uint64_t t0 = span0 + uint64_t(k0 + kk) * kTile;
-                    const uint64_t first = J.i0 > t0 ? J.i0 : t0, end = jend < t0 + kTile ? jend : t0 + kTile;
-                    if (first < end) {
-                        const LinkDev L = links[J.link0 + g0 + s];
-                        smooth = anchor_fit(s_rec[kk * ng + s], L, txs[L.tx], J, s_at[kk], end - 1 - first >= 2);
-                    }
-                }
-                const uint64_t m = __ballot(smooth);
+                const uint64_t m = __ballot(kk < kb && s < ng && s_smooth[kk * ng + s]);
                 if (kk < kb && s == 0) {
                     s_mask[kk][0] = static_cast<uint32_t>(m);
                     s_mask[kk][1] = static_cast<uint32_t>(m >> 32);
