@@ -294,6 +294,45 @@ int lpk_bands_overlap(double tx_carrier_hz, double tx_bw_hz, double band_lo_hz, 
 /* src/channel/mod.rs:310-312 */
 double lpk_thermal_noise_power_w(double temperature_k, double bandwidth_hz);
 
+/* ---- GPS LNAV and broadcast ephemeris (host builders, src/signal/gps_nav.rs) ----
+ * They produce the nav_bits table (LPK_NAV_FULL) and the position grid of an
+ * lpk_tx_desc. Angles in radians, times in GPS seconds of week. */
+typedef struct lpk_ephemeris {       /* Ephemeris, src/signal/gps_nav.rs:56-85 */
+    uint32_t prn, week;
+    double toe_s, toc_s, sqrt_a, e, m0_rad, delta_n_rad_s, i0_rad, idot_rad_s;
+    double omega0_rad, omega_dot_rad_s, argp_rad;
+    double cuc, cus, crc, crs, cic, cis;
+    double af0, af1, af2, tgd;
+    uint32_t iode, iodc, health, ura_index;
+} lpk_ephemeris;
+/* GpsTime::from_unix (src/signal/gps_nav.rs:31-41) */
+void lpk_gps_time(double unix_s, double utc_offset_s, uint32_t *week, double *sow);
+/* The 16 s toe grid of Ephemeris::from_trajectory (src/signal/gps_nav.rs:140-144):
+ * the UTC time whose GPS seconds of week are round16(sow), that toe and its week. */
+int lpk_toe_snap(double unix_s, double utc_offset_s, double *unix_at_toe, uint32_t *week, double *toe_sow);
+/* Ephemeris::from_circular_keplerian (src/signal/gps_nav.rs:87-129) */
+int lpk_ephemeris_from_kepler(double a_m, double inc_deg, double raan_deg, double nu0_deg, double orbit_epoch_unix,
+                              uint32_t prn, double scene_epoch_unix, double utc_offset_s, lpk_ephemeris *out);
+/* Ephemeris::from_trajectory (src/signal/gps_nav.rs:131-180) from the ECEF
+ * state r (m), v (m/s) at the snapped toe (lpk_toe_snap) */
+int lpk_ephemeris_from_state(const double r[3], const double v[3], uint32_t prn, uint32_t week, double toe_sow,
+                             lpk_ephemeris *out);
+/* lnav_bits (src/signal/gps_nav.rs:554-585): +-1 LNAV bits for duration_s from
+ * GPS seconds of week sow; n must be lpk_lnav_length(duration_s). */
+uint64_t lpk_lnav_length(double duration_s);
+int lpk_lnav_bits(const lpk_ephemeris *eph, double sow, double duration_s, int8_t *out, uint64_t n);
+/* sv_position_ecef (src/signal/gps_nav.rs:587-628) */
+void lpk_sv_position(const lpk_ephemeris *eph, double t_sow, double out[3]);
+/* PrecomputedTrajectory::sample (src/geometry/precomputed.rs:47-70) over a
+ * GpsBroadcastTrajectory (src/signal/gps_nav.rs:630-654); xyz holds 3*n doubles,
+ * n = lpk_grid_size(duration_s, grid_dt_s). */
+int lpk_grid_fill_broadcast(const lpk_ephemeris *eph, double utc_offset_s, double scene_epoch_unix,
+                            double duration_s, double grid_dt_s, double *t0_out, double *xyz, uint64_t n);
+/* parse_rinex3_gps_nav (src/signal/gps_nav.rs:656-728): the GPS record of prn
+ * whose toe is nearest near_unix; LPK_E_INVALID when the text holds none. */
+int lpk_rinex3_gps_ephemeris(const char *text, uint32_t prn, double near_unix, double utc_offset_s,
+                             lpk_ephemeris *out);
+
 
 /* ---- learned warm start (no counterpart in the reference: its solve always
  * starts at t - 0.07 s, src/channel/mod.rs:75-96) ----

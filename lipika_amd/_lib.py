@@ -32,6 +32,16 @@ _i32 = C.c_int32
 _u32 = C.c_uint32
 
 
+class Ephemeris(C.Structure):
+    """lpk_ephemeris (include/lpk.h), Ephemeris of src/signal/gps_nav.rs:56-85."""
+    _fields_ = [("prn", _u32), ("week", _u32),
+                ("toe_s", _d), ("toc_s", _d), ("sqrt_a", _d), ("e", _d), ("m0_rad", _d), ("delta_n_rad_s", _d),
+                ("i0_rad", _d), ("idot_rad_s", _d), ("omega0_rad", _d), ("omega_dot_rad_s", _d), ("argp_rad", _d),
+                ("cuc", _d), ("cus", _d), ("crc", _d), ("crs", _d), ("cic", _d), ("cis", _d),
+                ("af0", _d), ("af1", _d), ("af2", _d), ("tgd", _d),
+                ("iode", _u32), ("iodc", _u32), ("health", _u32), ("ura_index", _u32)]
+
+
 class TxDesc(C.Structure):
     _fields_ = [
         ("traj_kind", _i32), ("signal_kind", _i32),
@@ -175,6 +185,15 @@ def load() -> C.CDLL:
         "lpk_horizon_gate_stats": (_i32, [C.c_void_p, P(_u64)]),
         "lpk_stream_to_files": (_i32, [C.c_void_p, P(BandChunk), _u32, P(C.c_char_p), _u64, C.c_int]),
         "lpk_truth": (_i32, [C.c_void_p, _u32, P(_d), _u64, _d, C.c_int, P(_d), P(_d)]),
+        "lpk_gps_time": (None, [_d, _d, P(_u32), P(_d)]),
+        "lpk_toe_snap": (_i32, [_d, _d, P(_d), P(_u32), P(_d)]),
+        "lpk_ephemeris_from_kepler": (_i32, [_d, _d, _d, _d, _d, _u32, _d, _d, P(Ephemeris)]),
+        "lpk_ephemeris_from_state": (_i32, [P(_d), P(_d), _u32, _u32, _d, P(Ephemeris)]),
+        "lpk_lnav_length": (_u64, [_d]),
+        "lpk_lnav_bits": (_i32, [P(Ephemeris), _d, _d, P(C.c_int8), _u64]),
+        "lpk_sv_position": (None, [P(Ephemeris), _d, P(_d)]),
+        "lpk_grid_fill_broadcast": (_i32, [P(Ephemeris), _d, _d, _d, _d, P(_d), P(_d), _u64]),
+        "lpk_rinex3_gps_ephemeris": (_i32, [C.c_char_p, _u32, _d, _d, P(Ephemeris)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -199,6 +218,8 @@ EXPORTED = (
     "lpk_lws_infer", "lpk_lws_gradient", "lpk_lws_train", "lpk_set_warm_start", "lpk_solve_stats",
     "lpk_solve_stats_reset", "lpk_set_horizon_gate", "lpk_horizon_gate_stats",
     "lpk_stream_to_files", "lpk_truth",
+    "lpk_gps_time", "lpk_toe_snap", "lpk_ephemeris_from_kepler", "lpk_ephemeris_from_state", "lpk_lnav_length",
+    "lpk_lnav_bits", "lpk_sv_position", "lpk_grid_fill_broadcast", "lpk_rinex3_gps_ephemeris",
 )
 
 

@@ -46,7 +46,25 @@ class CircularKeplerian:
     kind: str = "circular_keplerian"
 
 
-Trajectory = Union[StationaryGeodetic, StationaryEcef, CircularKeplerian]
+@dataclass
+class RinexNav:
+    """GPS broadcast ephemeris from a RINEX 3 nav file (src/config/mod.rs:440-444):
+    the record of `prn` whose toe is nearest the scene epoch."""
+    file: str
+    prn: int
+    kind: str = "rinex_nav"
+
+
+@dataclass
+class Tle:
+    """An SGP4 trajectory from a TLE pair (src/config/mod.rs:432). Parsed and
+    written back, but not propagated: SGP4 is the third-party sgp4 crate."""
+    line1: str
+    line2: str
+    kind: str = "tle"
+
+
+Trajectory = Union[StationaryGeodetic, StationaryEcef, CircularKeplerian, RinexNav, Tle]
 
 
 # ---------------------------------------------------------------- signals
@@ -216,7 +234,11 @@ def _trajectory(d: dict) -> Trajectory:
         return CircularKeplerian(float(d["semi_major_axis_m"]), float(d["inclination_deg"]),
                                  float(d["raan_deg"]), float(d["true_anomaly_deg_at_epoch"]),
                                  float(d["epoch_unix"]))
-    raise NotImplementedError(f"trajectory kind {kind!r} is not on the GPU path")
+    if kind == "rinex_nav":
+        return RinexNav(str(d["file"]), int(d["prn"]))
+    if kind == "tle":
+        return Tle(str(d["line1"]), str(d["line2"]))
+    raise ValueError(f"unknown trajectory kind {kind!r}")
 
 
 def _signal(d: dict) -> Signal:

@@ -25,8 +25,8 @@ from typing import List, Optional
 import numpy as np
 
 from . import _lib
-from .config import (Bpsk, CaCode, CircularKeplerian, Cw, Ook, Qam16, SceneConfig, StationaryEcef, StationaryGeodetic,
-                     TransmitterConfig, default_bandwidth_hz)
+from .config import (Bpsk, CaCode, CircularKeplerian, Cw, Ook, Qam16, RinexNav, SceneConfig, StationaryEcef,
+                     StationaryGeodetic, Tle, TransmitterConfig, default_bandwidth_hz)
 
 SPEED_OF_LIGHT_M_S = 299792458.0           # src/lib.rs:53
 TRUTH_ITERATIONS = 4                        # src/pipeline.rs:573 (NEWTON_ITERATIONS.max(4))
@@ -126,6 +126,10 @@ def _trajectory_value(t) -> dict:
         return {"kind": "circular_keplerian", "semi_major_axis_m": _f(t.semi_major_axis_m),
                 "inclination_deg": _f(t.inclination_deg), "raan_deg": _f(t.raan_deg),
                 "true_anomaly_deg_at_epoch": _f(t.true_anomaly_deg_at_epoch), "epoch_unix": _f(t.epoch_unix)}
+    if isinstance(t, RinexNav):
+        return {"kind": "rinex_nav", "file": t.file, "prn": int(t.prn)}
+    if isinstance(t, Tle):
+        return {"kind": "tle", "line1": t.line1, "line2": t.line2}
     raise TypeError(f"trajectory {t!r}")
 
 

@@ -12,6 +12,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import warnings
 from dataclasses import dataclass, field
 from enum import Enum
 from typing import Callable, List, Optional, Sequence
@@ -20,8 +21,8 @@ import numpy as np
 
 from . import _lib
 from .config import (
-    Bpsk, CaCode, CircularKeplerian, Cw, Ook, Qam16, ReceiverBandConfig, ReceiverConfig,
-    SceneConfig, SceneMeta, SkyConfig, StationaryEcef, StationaryGeodetic, TransmitterConfig,
+    Bpsk, CaCode, CircularKeplerian, Cw, Ook, Qam16, ReceiverBandConfig, ReceiverConfig, RinexNav,
+    SceneConfig, SceneMeta, SkyConfig, StationaryEcef, StationaryGeodetic, Tle, TransmitterConfig,
     CA_CHIP_RATE_HZ,
 )
 
@@ -91,6 +92,7 @@ class PreparedTx:
     bursts: Optional[np.ndarray] = None        # float64 (k, 2): (start, end)
     # (a m, i deg, raan deg, nu0 deg, epoch unix) of a CircularKeplerian trajectory
     kepler: Optional[tuple] = None
+    nav_bits: Optional[np.ndarray] = None      # int8 +-1 LNAV bits (nav = "full")
 
     @property
     def n_symbols(self) -> int:
@@ -145,6 +147,78 @@ def _check(rc: int) -> None:
     _lib.check(rc)
 
 
+def load_rinex_ephemeris(traj: RinexNav, meta: SceneMeta) -> "_lib.Ephemeris":
+    """load_rinex_ephemeris (src/pipeline.rs:109-121): the record of the PRN
+    whose toe is nearest the scene epoch."""
+    with open(traj.file, "r", encoding="utf-8") as f:
+        text = f.read()
+    eph = _lib.Ephemeris()
+    rc = _lib.load().lpk_rinex3_gps_ephemeris(text.encode(), int(traj.prn), meta.epoch_unix, meta.gps_utc_offset_s,
+                                              C.byref(eph))
+    if rc != _lib.LPK_OK:
+        raise ValueError(f"no GPS PRN {traj.prn} ephemeris found in {traj.file!r}")
+    return eph
+
+
+def trajectory_state(traj, t_unix: float):
+    """Trajectory::position_ecef and velocity_ecef (src/geometry/mod.rs:62-74,
+    src/geometry/kepler.rs:27-96) of the trajectories this mirror builds."""
+    if isinstance(traj, (StationaryGeodetic, StationaryEcef)):
+        return station_ecef(traj), np.zeros(3)
+    if isinstance(traj, CircularKeplerian):
+        # CircularKeplerian::eci_at + eci_to_ecef in the reference's operation order
+        a = traj.semi_major_axis_m
+        n = math.sqrt(3.986005e14 / (a * a * a))
+        nu = math.radians(traj.true_anomaly_deg_at_epoch) + n * (t_unix - traj.epoch_unix)
+        s_nu, c_nu = math.sin(nu), math.cos(nu)
+        s_i, c_i = math.sin(math.radians(traj.inclination_deg)), math.cos(math.radians(traj.inclination_deg))
+        s_o, c_o = math.sin(math.radians(traj.raan_deg)), math.cos(math.radians(traj.raan_deg))
+        xp, yp = a * c_nu, a * s_nu
+        y1, z1 = yp * c_i, yp * s_i
+        x, y = xp * c_o - y1 * s_o, xp * s_o + y1 * c_o
+        v_mag = a * n
+        vxp, vyp = -v_mag * s_nu, v_mag * c_nu
+        vy1, vz1 = vyp * c_i, vyp * s_i
+        vx, vy = vxp * c_o - vy1 * s_o, vxp * s_o + vy1 * c_o
+        w = 7.2921151467e-5
+        th = w * (t_unix - traj.epoch_unix)
+        s_t, c_t = math.sin(th), math.cos(th)
+        xe, ye = x * c_t + y * s_t, -x * s_t + y * c_t
+        vxe = (vx * c_t + vy * s_t) + w * ye
+        vye = (-vx * s_t + vy * c_t) - w * xe
+        return np.array([xe, ye, z1]), np.array([vxe, vye, vz1])
+    raise NotImplementedError(f"trajectory {traj!r} has no state here")
+
+
+def build_lnav_bits(cfg: TransmitterConfig, meta: SceneMeta, prn: int) -> np.ndarray:
+    """build_lnav_bits (src/pipeline.rs:255-295): the scene's LNAV stream from
+    the ephemeris of the transmitter's own orbit."""
+    lib = _lib.load()
+    week, sow = C.c_uint32(), C.c_double()
+    lib.lpk_gps_time(meta.epoch_unix, meta.gps_utc_offset_s, C.byref(week), C.byref(sow))
+    if math.fmod(sow.value / 6.0, 1.0) != 0.0:
+        warnings.warn(f"scene epoch is not on a 6 s GPS boundary (sow={sow.value}); full-NAV subframe edges won't "
+                      "align to the TOW Z-count and SV-position time will be biased. Use an epoch_unix divisible by 6.")
+    traj = cfg.trajectory
+    eph = _lib.Ephemeris()
+    if isinstance(traj, CircularKeplerian):
+        _check(lib.lpk_ephemeris_from_kepler(traj.semi_major_axis_m, traj.inclination_deg, traj.raan_deg,
+                                             traj.true_anomaly_deg_at_epoch, traj.epoch_unix, prn, meta.epoch_unix,
+                                             meta.gps_utc_offset_s, C.byref(eph)))
+    elif isinstance(traj, RinexNav):
+        eph = load_rinex_ephemeris(traj, meta)
+    else:
+        t_toe, wk, toe = C.c_double(), C.c_uint32(), C.c_double()
+        _check(lib.lpk_toe_snap(meta.epoch_unix, meta.gps_utc_offset_s, C.byref(t_toe), C.byref(wk), C.byref(toe)))
+        r, v = trajectory_state(traj, t_toe.value)
+        _check(lib.lpk_ephemeris_from_state((C.c_double * 3)(*r), (C.c_double * 3)(*v), prn, wk.value, toe.value,
+                                            C.byref(eph)))
+    n = int(lib.lpk_lnav_length(meta.duration_s))
+    bits = np.empty(n, dtype=np.int8)
+    _check(lib.lpk_lnav_bits(C.byref(eph), sow.value, meta.duration_s, bits.ctypes.data_as(C.POINTER(C.c_int8)), n))
+    return bits
+
+
 def build_transmitter(cfg: TransmitterConfig, meta: SceneMeta, time_base: TimeBase) -> PreparedTx:
     """Transmitter::from_config + build_signal (src/pipeline.rs:142-253, :312-341)."""
     lib = _lib.load()
@@ -170,6 +244,19 @@ def build_transmitter(cfg: TransmitterConfig, meta: SceneMeta, time_base: TimeBa
         tx.grid, tx.grid_t0, tx.grid_dt = grid, t0.value, TX_TRAJECTORY_GRID_DT_S
         tx.kepler = (traj.semi_major_axis_m, traj.inclination_deg, traj.raan_deg, traj.true_anomaly_deg_at_epoch,
                      traj.epoch_unix)
+    elif isinstance(traj, RinexNav):
+        eph = load_rinex_ephemeris(traj, meta)
+        n = int(lib.lpk_grid_size(meta.duration_s, TX_TRAJECTORY_GRID_DT_S))
+        grid = np.empty((3, n), dtype=np.float64)
+        t0 = C.c_double()
+        _check(lib.lpk_grid_fill_broadcast(C.byref(eph), meta.gps_utc_offset_s, meta.epoch_unix, meta.duration_s,
+                                           TX_TRAJECTORY_GRID_DT_S, C.byref(t0),
+                                           grid.ctypes.data_as(C.POINTER(C.c_double)), n))
+        tx.traj_kind = _lib.TRAJ_GRID
+        tx.grid, tx.grid_t0, tx.grid_dt = grid, t0.value, TX_TRAJECTORY_GRID_DT_S
+    elif isinstance(traj, Tle):
+        raise NotImplementedError(f"transmitter {cfg.name!r}: tle trajectories need SGP4 propagation (the sgp4 "
+                                  "crate of the reference), which this backend does not implement")
     else:
         raise NotImplementedError(f"trajectory {traj!r} is not on the GPU path")
 
@@ -235,8 +322,11 @@ def build_transmitter(cfg: TransmitterConfig, meta: SceneMeta, time_base: TimeBa
             elif sig.nav == "dummy":
                 tx.nav_mode = _lib.NAV_DUMMY
                 tx.nav_seed = int(lib.lpk_nav_fold_seed(sig.nav_seed & U64, sig.prn))
+            elif sig.nav == "full":
+                tx.nav_mode = _lib.NAV_FULL
+                tx.nav_bits = build_lnav_bits(cfg, meta, sig.prn)
             else:
-                raise NotImplementedError("nav = \"full\" (LNAV from ephemeris) is not on the GPU path")
+                raise ValueError(f"transmitter {cfg.name!r}: unknown nav mode {sig.nav!r}")
         else:
             if sig.nav != "off":
                 raise ValueError(f"transmitter '{cfg.name}': ca_code with nav = \"{sig.nav}\" is not supported "
@@ -378,6 +468,10 @@ class Backend:
                 keep.append(s)
                 d.symbols = _ptr(s, C.c_float)
             d.nav_mode, d.prn, d.nav_seed = t.nav_mode, t.prn, t.nav_seed & U64
+            if t.nav_bits is not None:
+                nb = np.ascontiguousarray(t.nav_bits, dtype=np.int8)
+                keep.append(nb)
+                d.nav_bits, d.n_nav_bits = _ptr(nb, C.c_int8), len(nb)
             if t.ca_code is not None:
                 c = np.ascontiguousarray(t.ca_code, dtype=np.int8)
                 keep.append(c)

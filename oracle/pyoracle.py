@@ -161,6 +161,10 @@ class OracleScene:
                     self._keep.append(a)
                     setattr(d, attr, _ptr(a, ct))
             d.nav_mode, d.prn, d.nav_seed = t.nav_mode, t.prn, t.nav_seed
+            if getattr(t, "nav_bits", None) is not None:
+                nb = np.ascontiguousarray(t.nav_bits, dtype=np.int8)
+                self._keep.append(nb)
+                d.nav_bits, d.n_nav_bits = _ptr(nb, C.c_int8), len(nb)
             if t.gated:
                 b = np.ascontiguousarray(t.bursts, dtype=np.float64)
                 self._keep.append(b)

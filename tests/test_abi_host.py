@@ -246,8 +246,12 @@ def test_invalid_scenes_raise():
     tx = TransmitterConfig("g", 1e9, Cw(0.0), StationaryEcef(7e6, 0, 0), gate=GateConfig(15.0, [(14.8, 0.5)]))
     with pytest.raises(ValueError):
         prepare(SceneConfig(SceneMeta(0.0, 1.0), [tx], []))
-    with pytest.raises(NotImplementedError):
-        prepare(SceneConfig(SceneMeta(0.0, 1.0), [TransmitterConfig("f", 1e9, CaCode(1, 0.0, "full"),
+    # full LNAV is on the path (its builders are tested in test_gpsnav.py); an unknown mode is not
+    sc = prepare(SceneConfig(SceneMeta(0.0, 1.0), [TransmitterConfig("f", 1e9, CaCode(1, 0.0, "full"),
+                                                                      StationaryEcef(7e6, 0, 0))], []))
+    assert sc.transmitters[0].nav_mode == _lib.NAV_FULL and len(sc.transmitters[0].nav_bits) == 600
+    with pytest.raises(ValueError):
+        prepare(SceneConfig(SceneMeta(0.0, 1.0), [TransmitterConfig("f", 1e9, CaCode(1, 0.0, "lnav"),
                                                                      StationaryEcef(7e6, 0, 0))], []))
     # flicker FM is on the path: a receiver with a flicker floor prepares
     sc = prepare(SceneConfig(SceneMeta(0.0, 1.0), [], [ReceiverConfig(

@@ -684,3 +684,40 @@ def test_horizon_gate_is_byte_identical(backend, form):
         orc = OracleScene(sc, seed)
         o, _, _ = orc.render_band(0)
         compare_iq(gon, o, "gated plain form vs oracle")
+
+
+# ---------------------------------------------------------------- full NAV and RINEX sources
+
+def test_full_nav_and_rinex_sources_match_oracle(backend):
+    """C/A with fully formed LNAV (src/signal/gps_nav.rs:554-585) on a
+    circular orbit and on the reference's RINEX record (its grid from the
+    broadcast ephemeris): the plain form bit-identical to the oracle with the
+    oracle's clock offsets, the precision form within its bounds, across the
+    NAV bit edges of 0.3 s at 2.048 MHz."""
+    import math
+    import os
+    from lipika_amd.config import CaCode, CircularKeplerian, ReceiverBandConfig, RinexNav, SceneConfig, SceneMeta, \
+        TransmitterConfig
+    rinex = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "rinex3_gps_prn01.rnx")
+    toe_unix = 315964800.0 + 2317.0 * 604800.0 + 518400.0 - 18.0
+    epoch = math.floor(toe_unix / 6.0) * 6.0 + 12.0
+    txs = [TransmitterConfig("kep", 1575.42e6, CaCode(5, epoch, "full"),
+                             CircularKeplerian(26.56e6, 55.0, 40.0, 120.0, epoch), 27.0),
+           TransmitterConfig("brdc", 1575.42e6, CaCode(1, epoch, "full"), RinexNav(rinex, 1), 27.0)]
+    band = ReceiverBandConfig("l1", 1575.42e6, 2.048e6, noise_power_w=0.0)
+    scene = SceneConfig(SceneMeta(epoch, 0.3), txs,
+                        [util.lab_receiver([band], NodeClockSpec(allan_deviation_1s=1e-11, frequency_offset_y0=2e-7))])
+    sc = prepare(scene)
+    assert all(t.nav_mode == _lib.NAV_FULL for t in sc.transmitters)
+    seed = 31
+    backend.upload(sc, seed)
+    orc = OracleScene(sc, seed)
+    check_band_both_clocks(backend, orc, sc, 0, 0, 0, seed, "full NAV + RINEX, 2.048 MHz", chunk=150_000)
+    o, _, _ = orc.render_band(0)
+    g, _ = gpu_band(backend, sc, 0, 0, 0, seed, form=_lib.FORM_PRECISION, chunk=150_000)
+    peak = float(np.max(np.abs(o)))
+    d = np.abs(g.astype(np.complex128) - o.astype(np.complex128))
+    _say(f"full NAV + RINEX, precision form: max|d|/peak={d.max() / peak:.3e}")
+    assert d.max() <= 2e-6 * peak
+    for slot in range(2):
+        compare_taps(backend, orc, sc, 0, 0, 0, seed, slot, 200_000, form=_lib.FORM_PRECISION, label="full NAV")
