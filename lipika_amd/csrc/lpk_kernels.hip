@@ -449,6 +449,7 @@ struct alignas(16) Anchor {
 static_assert(sizeof(Anchor) == 208, "Anchor layout");
 constexpr int kAnchorGroup = 64;  // transmitters per anchor pass
 constexpr int kSpanRecords = 128; // (tile, transmitter) records of one anchor batch of a span
+constexpr int kSpanSamples = 2;    // adjacent samples per lane of the span kernels
 
 // A wave-uniform 64-bit value from two 32-bit halves, in scalar registers.
 __device__ __forceinline__ uint64_t uniform_u64(uint32_t lo, uint32_t hi) {
@@ -541,34 +542,45 @@ __device__ __forceinline__ void solve_smooth(const TxDev &T, const double *obs, 
     rd1t = rt[1];
 }
 
-// One (sample, transmitter) contribution of the precision form, for the
+// N samples' contributions of one transmitter in the precision form, for the
 // transmitter group G (kGroup*): the fitted range with the per-sample rounding
 // corrections (`smooth`, wave-uniform) or the exact solve, s(t_emit), the
 // carrier phase in turns from the middle sample's, the hardware sine and
 // cosine, the f32 mix, and the exact f64 sum. Slow paths (a tile crossing a
 // grid node, a symbol, NAV bit or chip outside the tile's hints, gates) read
 // the transmitter's table. ddl and dtl are delta_t and t_local less their
-// values at the middle sample.
-template <int G, bool ANT>
+// values at the middle sample. The N samples are independent chains that
+// share the record's reads.
+template <int G, bool ANT, int N>
 __device__ __forceinline__ void precision_link(const Anchor &A, bool smooth, const TxDev *__restrict__ txs,
-                                               const JobDev &J, uint32_t link, double tu, double x, double del0,
-                                               double ddl, double dtl, double &sum_re, double &sum_im) {
-    double r;
+                                               const JobDev &J, uint32_t link, const double *tu, const double *x,
+                                               const double *del0, const double *ddl, const double *dtl,
+                                               double *sum_re, double *sum_im) {
+    static_assert(!ANT || N == 1, "antenna kernels take one sample per lane");
+    double r[N];
     if (smooth) {
-        const double r0 = fma(A.rd0, del0, fma(x, fma(A.q0c2, x, A.q0d1), A.r0m));
-        const double t1 = fma(-r0, kInvC, tu);
-        const double del1 = ((J.epoch + t1) - J.epoch) - t1;
-        r = fma(A.rd1q, del1, fma(-A.k01, del0, fma(x, fma(A.q1c2, x, A.q1d1), A.r1m)));
+#pragma unroll
+        for (int q = 0; q < N; ++q) {
+            const double r0 = fma(A.rd0, del0[q], fma(x[q], fma(A.q0c2, x[q], A.q0d1), A.r0m));
+            const double t1 = fma(-r0, kInvC, tu[q]);
+            const double del1 = ((J.epoch + t1) - J.epoch) - t1;
+            r[q] = fma(A.rd1q, del1, fma(-A.k01, del0[q], fma(x[q], fma(A.q1c2, x[q], A.q1d1), A.r1m)));
+        }
     } else {
-        double te;
-        int64_t c0, c1;
-        solve_emission(txs[A.tx], J.obs, tu, J.epoch, te, r, c0, c1);
+#pragma unroll
+        for (int q = 0; q < N; ++q) {
+            double te;
+            int64_t c0, c1;
+            solve_emission(txs[A.tx], J.obs, tu[q], J.epoch, te, r[q], c0, c1);
+        }
     }
-    const double t_emit = fma(-r, kInvC, tu);
+    double t_emit[N];
+#pragma unroll
+    for (int q = 0; q < N; ++q) t_emit[q] = fma(-r[q], kInvC, tu[q]);
     // antenna gain; an occluded sample adds nothing (src/channel/mod.rs:253-255)
     float g = 1.0f;
     if (ANT && J.ant_on) {
-        g = antenna_gain(J, txs[A.tx], t_emit);
+        g = antenna_gain(J, txs[A.tx], t_emit[0]);
         if (g != 0.0f && J.unocc) J.unocc[J.link0 + link] = 1u;
         if (g == 0.0f) return;
     }
@@ -577,73 +589,81 @@ __device__ __forceinline__ void precision_link(const Anchor &A, bool smooth, con
     // that their latency is covered), and the rare other indices are
     // evaluated after it. CW's own phase joins the carrier phase.
     typedef const __attribute__((address_space(1))) float gchip;
-    float bre = 0.0f, bim = 0.0f;
-    double ph_cw = 0.0;
-    bool hinted = true;
-    double sig_b = 0.0;
-    float chip = 0.0f;
-    if (G == kGroupCa) {
-        const double dt = t_emit - A.epoch_rel;
-        sig_b = floor(dt * kNavBitRate);
-        const double sig_k = floor(dt * kCaChipRate);
-        // the chip at residue cmod + d of the doubled table when the count is
-        // within one code period of the tile's base (NaN fails the test), else
-        // at the count's own residue
-        const double d = sig_k - A.cbase;
-        int32_t off = static_cast<int32_t>(d);
-        if (!(d >= 0.0 && d < static_cast<double>(kCaCodeLength)))
-            off = static_cast<int32_t>(rem_euclid(floor_to_i64(sig_k), kCaCodeLength)) - static_cast<int32_t>(A.cmod);
-        chip = ((gchip *)A.chip0)[off];
-        bre = sig_b == A.hu ? A.h0re : A.h1re;
-        hinted = sig_b == A.hu || sig_b == A.hu1;
-    } else if (G == kGroupFramed) {
-        sig_b = floor((t_emit - A.epoch_rel) * A.rate);
-        const bool h0 = sig_b == A.hu;
-        bre = h0 ? A.h0re : A.h1re;
-        bim = h0 ? A.h0im : A.h1im;
-        hinted = h0 || sig_b == A.hu1;
-    } else if (G == kGroupCw) {
-        ph_cw = static_cast<double>(static_cast<float>(A.rate * t_emit));  // rate holds omega
-        bre = 1.0f;
-    } else {
-        const TxDev &T = txs[A.tx];
-        if (!T.gated || gate_on(T, t_emit)) {
-            if (T.signal_kind == 0) {
-                ph_cw = static_cast<double>(cw_phase(T, t_emit));
-                bre = 1.0f;
-            } else {
-                baseband(T, t_emit, bre, bim);
+    float bre[N], bim[N], chip[N];
+    double ph_cw[N], sig_b[N];
+    bool hinted[N];
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+        bre[q] = bim[q] = chip[q] = 0.0f;
+        ph_cw[q] = sig_b[q] = 0.0;
+        hinted[q] = true;
+        if (G == kGroupCa) {
+            const double dt = t_emit[q] - A.epoch_rel;
+            sig_b[q] = floor(dt * kNavBitRate);
+            const double sig_k = floor(dt * kCaChipRate);
+            // the chip at residue cmod + d of the doubled table when the count is
+            // within one code period of the tile's base (NaN fails the test), else
+            // at the count's own residue
+            const double d = sig_k - A.cbase;
+            int32_t off = static_cast<int32_t>(d);
+            if (!(d >= 0.0 && d < static_cast<double>(kCaCodeLength)))
+                off = static_cast<int32_t>(rem_euclid(floor_to_i64(sig_k), kCaCodeLength)) -
+                      static_cast<int32_t>(A.cmod);
+            chip[q] = ((gchip *)A.chip0)[off];
+            bre[q] = sig_b[q] == A.hu ? A.h0re : A.h1re;
+            hinted[q] = sig_b[q] == A.hu || sig_b[q] == A.hu1;
+        } else if (G == kGroupFramed) {
+            sig_b[q] = floor((t_emit[q] - A.epoch_rel) * A.rate);
+            const bool h0 = sig_b[q] == A.hu;
+            bre[q] = h0 ? A.h0re : A.h1re;
+            bim[q] = h0 ? A.h0im : A.h1im;
+            hinted[q] = h0 || sig_b[q] == A.hu1;
+        } else if (G == kGroupCw) {
+            ph_cw[q] = static_cast<double>(static_cast<float>(A.rate * t_emit[q]));  // rate holds omega
+            bre[q] = 1.0f;
+        } else {
+            const TxDev &T = txs[A.tx];
+            if (!T.gated || gate_on(T, t_emit[q])) {
+                if (T.signal_kind == 0) {
+                    ph_cw[q] = static_cast<double>(cw_phase(T, t_emit[q]));
+                    bre[q] = 1.0f;
+                } else {
+                    baseband(T, t_emit[q], bre[q], bim[q]);
+                }
             }
         }
     }
-    // the carrier phase in turns (linear in its three inputs), its fraction,
-    // and the hardware sine and cosine of that fraction
-    const double dr = r - A.r1_mid;
-    double turns = fma(A.ph_r, dr, fma(A.ph_d, ddl, fma(A.ph_t, dtl, A.ph0)));
-    if (G == kGroupCw || G == kGroupGeneric) turns = fma(ph_cw, kInvTau, turns);
-    const float fr = static_cast<float>(__builtin_amdgcn_fract(turns));
-    const float spf = __builtin_amdgcn_sinf(fr), cpf = __builtin_amdgcn_cosf(fr);
-    // amp_factor / r to first order about the middle sample's range
-    float amplitude = fmaf(-A.amp_k, static_cast<float>(dr), A.amp_m);
-    if (ANT) amplitude *= g;
-    if (G == kGroupCa) {
-        if (!hinted) bre = static_cast<float>(nav_pm(txs[A.tx], sig_b));
-        bre *= chip;
-    } else if (G == kGroupFramed) {
-        if (!hinted) framed_value(txs[A.tx], sig_b, bre, bim);
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+        // the carrier phase in turns (linear in its three inputs), its fraction,
+        // and the hardware sine and cosine of that fraction
+        const double dr = r[q] - A.r1_mid;
+        double turns = fma(A.ph_r, dr, fma(A.ph_d, ddl[q], fma(A.ph_t, dtl[q], A.ph0)));
+        if (G == kGroupCw || G == kGroupGeneric) turns = fma(ph_cw[q], kInvTau, turns);
+        const float fr = static_cast<float>(__builtin_amdgcn_fract(turns));
+        const float spf = __builtin_amdgcn_sinf(fr), cpf = __builtin_amdgcn_cosf(fr);
+        // amp_factor / r to first order about the middle sample's range
+        float amplitude = fmaf(-A.amp_k, static_cast<float>(dr), A.amp_m);
+        if (ANT) amplitude *= g;
+        if (G == kGroupCa) {
+            if (!hinted[q]) bre[q] = static_cast<float>(nav_pm(txs[A.tx], sig_b[q]));
+            bre[q] *= chip[q];
+        } else if (G == kGroupFramed) {
+            if (!hinted[q]) framed_value(txs[A.tx], sig_b[q], bre[q], bim[q]);
+        }
+        float cre, cim;
+        if (G == kGroupCa || G == kGroupCw) {
+            const float ab = amplitude * bre[q];
+            cre = ab * cpf;
+            cim = ab * spf;
+        } else {
+            cre = amplitude * (bre[q] * cpf - bim[q] * spf);
+            cim = amplitude * (bre[q] * spf + bim[q] * cpf);
+        }
+        // the f32 contributions summed exactly in f64
+        sum_re[q] += static_cast<double>(cre);
+        sum_im[q] += static_cast<double>(cim);
     }
-    float cre, cim;
-    if (G == kGroupCa || G == kGroupCw) {
-        const float ab = amplitude * bre;
-        cre = ab * cpf;
-        cim = ab * spf;
-    } else {
-        cre = amplitude * (bre * cpf - bim * spf);
-        cim = amplitude * (bre * spf + bim * cpf);
-    }
-    // the f32 contributions summed exactly in f64
-    sum_re += static_cast<double>(cre);
-    sum_im += static_cast<double>(cim);
 }
 
 // The tapped link's emission time, range and carrier phase in the precision
@@ -766,35 +786,36 @@ __device__ __forceinline__ bool anchor_fit(Anchor &A, const LinkDev &L, const Tx
     return smooth;
 }
 
-// One sample's contributions from the links [g0, g0 + ng) of its job, whose
+// N samples' contributions from the links [g0, g0 + ng) of their job, whose
 // records are R[0..ng): the transmitter groups in order, links whose bit is
 // set in gmask skipped (antenna kernels), the fitted range for those whose
 // bit is set in smask.
-template <bool ANT>
+template <bool ANT, int N>
 __device__ __forceinline__ void precision_links(const Anchor *R, uint64_t smask, uint64_t gmask,
                                                 const TxDev *__restrict__ txs, const JobDev &J, uint32_t g0,
-                                                uint32_t ng, double tu, double x, double del0, double ddl, double dtl,
-                                                double &sum_re, double &sum_im) {
+                                                uint32_t ng, const double *tu, const double *x, const double *del0,
+                                                const double *ddl, const double *dtl, double *sum_re,
+                                                double *sum_im) {
     const uint32_t e_ca = J.group_end[kGroupCa] > g0 ? min(J.group_end[kGroupCa] - g0, ng) : 0u;
     const uint32_t e_fr = J.group_end[kGroupFramed] > g0 ? min(J.group_end[kGroupFramed] - g0, ng) : 0u;
     const uint32_t e_cw = J.group_end[kGroupCw] > g0 ? min(J.group_end[kGroupCw] - g0, ng) : 0u;
     uint32_t s = 0;
     for (; s < e_ca; ++s)
         if (!ANT || !((gmask >> s) & 1u))
-            precision_link<kGroupCa, ANT>(R[s], (smask >> s) & 1u, txs, J, g0 + s, tu, x, del0, ddl, dtl, sum_re,
-                                          sum_im);
+            precision_link<kGroupCa, ANT, N>(R[s], (smask >> s) & 1u, txs, J, g0 + s, tu, x, del0, ddl, dtl, sum_re,
+                                             sum_im);
     for (; s < e_fr; ++s)
         if (!ANT || !((gmask >> s) & 1u))
-            precision_link<kGroupFramed, ANT>(R[s], (smask >> s) & 1u, txs, J, g0 + s, tu, x, del0, ddl, dtl, sum_re,
-                                              sum_im);
+            precision_link<kGroupFramed, ANT, N>(R[s], (smask >> s) & 1u, txs, J, g0 + s, tu, x, del0, ddl, dtl,
+                                                 sum_re, sum_im);
     for (; s < e_cw; ++s)
         if (!ANT || !((gmask >> s) & 1u))
-            precision_link<kGroupCw, ANT>(R[s], (smask >> s) & 1u, txs, J, g0 + s, tu, x, del0, ddl, dtl, sum_re,
-                                          sum_im);
+            precision_link<kGroupCw, ANT, N>(R[s], (smask >> s) & 1u, txs, J, g0 + s, tu, x, del0, ddl, dtl, sum_re,
+                                             sum_im);
     for (; s < ng; ++s)
         if (!ANT || !((gmask >> s) & 1u))
-            precision_link<kGroupGeneric, ANT>(R[s], (smask >> s) & 1u, txs, J, g0 + s, tu, x, del0, ddl, dtl, sum_re,
-                                               sum_im);
+            precision_link<kGroupGeneric, ANT, N>(R[s], (smask >> s) & 1u, txs, J, g0 + s, tu, x, del0, ddl, dtl,
+                                                  sum_re, sum_im);
 }
 
 // RxChain::apply_post_sum (src/rx/mod.rs:142-146 and :83-101) and the
@@ -826,16 +847,20 @@ __device__ __forceinline__ void finish_sample(const JobDev &J, float acc_re, flo
 }
 
 // The precision form without an antenna over one span of 2^kSuperShift
-// tiles (`span` counts spans from the job's first). The anchor solves and fits
-// of several tiles run as one batch (as many (tile, transmitter) records as
-// kSpanRecords holds), so the latency of their dependent global reads is
-// paid once per batch; the tiles of the batch then run their sample loops in
-// turn. Per tile the arithmetic is that of a single-tile workgroup.
+// tiles of 2^TS samples (`span` counts spans from the job's first), with
+// 2^(TS-1) lanes, each owning two adjacent samples of a tile. The anchor
+// solves and fits of several tiles run as one batch (as many (tile,
+// transmitter) records as kSpanRecords holds), so the latency of their
+// dependent global reads is paid once per batch; the tiles of the batch then
+// run their sample loops in turn, each lane's two samples as independent
+// chains that share every record read. Per tile the arithmetic is that of a
+// one-tile workgroup.
 template <int TS>
 __device__ __forceinline__ void precision_span(const JobDev &J, uint32_t span, const LinkDev *__restrict__ links,
                                                const TxDev *__restrict__ txs, const TapOut &tap) {
-    constexpr int kTile = 1 << TS, kSub = 1 << kSuperShift, kWaves = kTile / 64;
-    __shared__ uint64_t s_noise[2][kTile];       // double-buffered Philox words, one per sample
+    constexpr int kTile = 1 << TS, kLanes = kTile / kSpanSamples, kSub = 1 << kSuperShift;
+    constexpr int kWaves = kLanes / 64;
+    static_assert(kWaves >= 1, "at least one wavefront per span");
     __shared__ Anchor s_rec[kSpanRecords];        // record kk * ng + s: tile kk of the batch, link g0 + s
     __shared__ double s_at[kSub][3][3];           // (tu, t_local, delta_t) at each tile's anchors
     __shared__ uint32_t s_mask[kSub][2];          // smooth links of each tile of the batch
@@ -845,10 +870,9 @@ __device__ __forceinline__ void precision_span(const JobDev &J, uint32_t span, c
     const uint32_t na = J.n_active;
     const int kb_max = na == 0 ? kSub : (na >= uint32_t(kAnchorGroup) ? 1 : min(kSub, kSpanRecords / int(na)));
     const bool noisy = J.sigma > 0.0f;
-    int nbuf = 0;
     for (int k0 = 0; k0 < kSub; k0 += kb_max) {
         const int kb = min(kb_max, kSub - k0);
-        double sum_re = 0.0, sum_im = 0.0;
+        double sum_re[kSpanSamples] = {}, sum_im[kSpanSamples] = {};
         for (uint32_t g0 = 0; g0 == 0 || g0 < na; g0 += kAnchorGroup) {
             const uint32_t ng = na - g0 < uint32_t(kAnchorGroup) ? na - g0 : uint32_t(kAnchorGroup);
             const bool last_group = g0 + ng >= na;
@@ -868,7 +892,7 @@ __device__ __forceinline__ void precision_span(const JobDev &J, uint32_t span, c
             }
             __syncthreads();
             // smooth solves at the three anchors of every record
-            for (uint32_t q = threadIdx.x; q < 3 * kb * ng; q += kTile) {
+            for (uint32_t q = threadIdx.x; q < 3 * kb * ng; q += kLanes) {
                 const uint32_t rec = q / 3, a = q - 3 * rec, kk = rec / ng;
                 const uint64_t t0 = span0 + uint64_t(k0 + kk) * kTile;
                 if ((J.i0 > t0 ? J.i0 : t0) >= (jend < t0 + kTile ? jend : t0 + kTile)) continue;
@@ -877,7 +901,7 @@ __device__ __forceinline__ void precision_span(const JobDev &J, uint32_t span, c
             }
             __syncthreads();
             // the fits, one record a lane
-            for (uint32_t rec = threadIdx.x; rec < kb * ng; rec += kTile) {
+            for (uint32_t rec = threadIdx.x; rec < kb * ng; rec += kLanes) {
                 const uint32_t kk = rec / ng;
                 const uint64_t t0 = span0 + uint64_t(k0 + kk) * kTile;
                 const uint64_t first = J.i0 > t0 ? J.i0 : t0, end = jend < t0 + kTile ? jend : t0 + kTile;
@@ -904,56 +928,66 @@ __device__ __forceinline__ void precision_span(const JobDev &J, uint32_t span, c
                 const uint64_t t0 = span0 + uint64_t(k0 + kk) * kTile;
                 const uint64_t first = J.i0 > t0 ? J.i0 : t0, end = jend < t0 + kTile ? jend : t0 + kTile;
                 if (first >= end) continue;   // a tile with no sample of its job (uniform)
-                // AWGN draws of the tile: kTile/2 Philox blocks (sample i: block
-                // i >> 1, word i & 1), consumed after the loops
-                if (last_group && noisy && threadIdx.x < kTile / 2) {
-                    uint64_t w[2];
-                    philox_words((t0 >> 1) + threadIdx.x, J.noise_seed, kAwgnTag, w);
-                    s_noise[nbuf][2 * threadIdx.x] = w[0];
-                    s_noise[nbuf][2 * threadIdx.x + 1] = w[1];
+                // the lane's two samples (src/pipeline.rs:918-926), clamped into the job
+                uint64_t i[kSpanSamples];
+                bool valid[kSpanSamples];
+                double t_local[kSpanSamples], dlt[kSpanSamples], tu[kSpanSamples], x[kSpanSamples];
+                double del0[kSpanSamples], ddl[kSpanSamples], dtl[kSpanSamples];
+#pragma unroll
+                for (int q = 0; q < kSpanSamples; ++q) {
+                    i[q] = t0 + kSpanSamples * threadIdx.x + q;
+                    valid[q] = i[q] >= J.i0 && i[q] < jend;
+                    const uint64_t ic = valid[q] ? i[q] : first;
+                    t_local[q] = static_cast<double>(ic) / J.fs;
+                    dlt[q] = J.delta_t ? J.delta_t[ic - J.i0] : clock_delta(J, t_local[q]);
+                    tu[q] = t_local[q] - dlt[q];
+                    x[q] = tu[q] - s_at[kk][1][0];
+                    // rounding of the iteration-1 query time, exact (Sterbenz) per sample
+                    const double t0q = tu[q] - kEmissionInitialLag;
+                    del0[q] = ((J.epoch + t0q) - J.epoch) - t0q;
+                    // the phase's per-sample inputs less the middle sample's
+                    ddl[q] = dlt[q] - s_at[kk][1][2];
+                    dtl[q] = t_local[q] - s_at[kk][1][1];
                 }
-                const uint64_t i = t0 + threadIdx.x;
-                const bool valid = i >= J.i0 && i < jend;
-                const uint64_t j = i - J.i0;
-                const uint64_t ic = valid ? i : first;
-                const double t_local = static_cast<double>(ic) / J.fs;
-                const double dlt = J.delta_t ? J.delta_t[ic - J.i0] : clock_delta(J, t_local);
-                const double tu = t_local - dlt;
-                const double x = tu - s_at[kk][1][0];
-                // rounding of the iteration-1 query time, exact (Sterbenz) per sample
-                const double t0q = tu - kEmissionInitialLag;
-                const double del0 = ((J.epoch + t0q) - J.epoch) - t0q;
-                // the phase's per-sample inputs less the middle sample's
-                const double ddl = dlt - s_at[kk][1][2], dtl = t_local - s_at[kk][1][1];
                 const uint64_t smask = uniform_u64(s_mask[kk][0], s_mask[kk][1]);
                 const Anchor *R = s_rec + kk * ng;
-                precision_links<false>(R, smask, 0ull, txs, J, g0, ng, tu, x, del0, ddl, dtl, sum_re, sum_im);
-                if (tap.on && valid && tap.slot >= g0 && tap.slot < g0 + ng) {
+                precision_links<false, kSpanSamples>(R, smask, 0ull, txs, J, g0, ng, tu, x, del0, ddl, dtl, sum_re,
+                                                     sum_im);
+                if (tap.on && tap.slot >= g0 && tap.slot < g0 + ng) {
                     const uint32_t ts = tap.slot - g0;
-                    double te, r, ph;
-                    precision_tap(R[ts], (smask >> ts) & 1u, txs, J, tu, t_local, dlt, x, del0, te, r, ph);
-                    tap.t_emit[j] = te;
-                    tap.range[j] = r;
-                    tap.phase[j] = ph;
+#pragma unroll
+                    for (int q = 0; q < kSpanSamples; ++q) {
+                        if (!valid[q]) continue;
+                        double te, r, ph;
+                        precision_tap(R[ts], (smask >> ts) & 1u, txs, J, tu[q], t_local[q], dlt[q], x[q], del0[q],
+                                      te, r, ph);
+                        tap.t_emit[i[q] - J.i0] = te;
+                        tap.range[i[q] - J.i0] = r;
+                        tap.phase[i[q] - J.i0] = ph;
+                    }
                 }
                 if (last_group) {
-                    if (tap.on && valid) tap.delta_t[j] = dlt;
-                    __syncthreads();   // the tile's Philox words are in LDS
-                    float acc_re = static_cast<float>(sum_re), acc_im = static_cast<float>(sum_im);
-                    // AWGN, src/channel/mod.rs:328-333 (sigma = sqrt(P/2) as f32)
-                    if (noisy) {
-                        float zr, zi;
-                        box_muller(s_noise[nbuf][threadIdx.x], zr, zi);
-                        acc_re += J.sigma * zr;
-                        acc_im += J.sigma * zi;
+                    // AWGN, src/channel/mod.rs:328-333 (sigma = sqrt(P/2) as f32): the
+                    // lane's samples are words 0 and 1 of Philox block i >> 1
+                    uint64_t w[2] = {0, 0};
+                    if (noisy) philox_words((t0 >> 1) + threadIdx.x, J.noise_seed, kAwgnTag, w);
+#pragma unroll
+                    for (int q = 0; q < kSpanSamples; ++q) {
+                        if (tap.on && valid[q]) tap.delta_t[i[q] - J.i0] = dlt[q];
+                        float acc_re = static_cast<float>(sum_re[q]), acc_im = static_cast<float>(sum_im[q]);
+                        if (noisy) {
+                            float zr, zi;
+                            box_muller(w[q], zr, zi);
+                            acc_re += J.sigma * zr;
+                            acc_im += J.sigma * zi;
+                        }
+                        if (valid[q]) finish_sample(J, acc_re, acc_im, i[q] - J.i0);
+                        sum_re[q] = 0.0;
+                        sum_im[q] = 0.0;
                     }
-                    nbuf ^= 1;
-                    if (valid) finish_sample(J, acc_re, acc_im, j);
-                    sum_re = 0.0;
-                    sum_im = 0.0;
                 }
             }
-            if (!last_group) __syncthreads();   // the next link group rewrites the records
+            __syncthreads();   // the next batch or link group rewrites the records and anchors
         }
     }
 }
@@ -977,7 +1011,7 @@ struct RenderLaunch {
 };
 
 template <int FORM, bool ANT, int TS>
-__global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM == 1 ? 5 : 6))
+__global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM == 1 ? 4 : 6))
     render_kernel(const JobDev *__restrict__ jobs, int n_jobs, const LinkDev *__restrict__ links,
                   const TxDev *__restrict__ txs, RenderLaunch lx, TapOut tap) {
     constexpr int kTile = 1 << TS;
@@ -1235,7 +1269,7 @@ __global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM == 1 ? 5 : 6))
             __syncthreads();
             const uint64_t smask = uniform_u64(s_mask[0], s_mask[1]);
             const uint64_t gmask = uniform_u64(s_mask[2], s_mask[3]);
-            precision_links<ANT>(s_anc, smask, gmask, txs, J, g0, ng, tu, x, del0, ddl, dtl, sum_re, sum_im);
+            precision_links<ANT, 1>(s_anc, smask, gmask, txs, J, g0, ng, &tu, &x, &del0, &ddl, &dtl, &sum_re, &sum_im);
             if (tap.on && valid && tap.slot >= g0 && tap.slot < g0 + ng) {
                 const uint32_t ts = tap.slot - g0;
                 double te, r, ph;
@@ -1492,7 +1526,8 @@ hipError_t launch_render(int form, bool antenna, int tile_shift, bool xcd_map, c
     for (uint32_t base = 0; base < n_tiles; base += kMaxLaunchTiles) {
         const uint32_t n = n_tiles - base < kMaxLaunchTiles ? n_tiles - base : kMaxLaunchTiles;
         const RenderLaunch lx{base, n, n_live, xcd_map ? 1u : 0u, pred, stats, gate ? 1u : 0u, gate_stats};
-        hipLaunchKernelGGL(kernel, dim3(n), dim3(1u << tile_shift), 0, stream, jobs, n_jobs, links, txs, lx, tap);
+        const uint32_t lanes = form == 1 && !antenna ? (1u << tile_shift) / kSpanSamples : (1u << tile_shift);
+        hipLaunchKernelGGL(kernel, dim3(n), dim3(lanes), 0, stream, jobs, n_jobs, links, txs, lx, tap);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
