@@ -36,7 +36,7 @@ PEAK_FP64_TFLOPS = 78.6            # MI355X published FP64 peak
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per configuration-2 render dispatch (reads + writes), from the rocprofv3 FETCH_SIZE and
 # WRITE_SIZE passes scaled by a copy kernel of known size: profiles/render_config2_hbm_traffic.md
-TRAFFIC_CONFIG2 = {"plain": 1.930e9, "precision": 1.969e9}
+TRAFFIC_CONFIG2 = {"plain": 2.063e9, "precision": 2.073e9}
 WORKLOADS = {
     2: "BPSK LEO 437 MHz + 6 C/A MEO 1575.42 MHz; bands 48 kHz and 2.048 MHz; clock walk and kTB noise on; 60 s",
     4: "40 transmitters (CW, OOK, BPSK, 16-QAM, C/A) x 64 receivers with 130 bands at 48 kHz to 8.184 MHz; "
