@@ -634,6 +634,14 @@ void validate_job(const lpk_ctx *ctx, const lpk_band_chunk &B, uint32_t q) {
         throw bad(LPK_E_NONFINITE, "antenna parameters");
 }
 
+// The precision form takes at most kPrecisionMaxLinks transmitters per job.
+void check_form_links(int form, const lpk_band_chunk &B, uint32_t q) {
+    if (form == LPK_FORM_PRECISION && B.n_active > kPrecisionMaxLinks)
+        throw Fail{LPK_E_INVALID, "job " + std::to_string(q) + ": the precision form takes at most " +
+                                      std::to_string(kPrecisionMaxLinks) + " transmitters per band-chunk (" +
+                                      std::to_string(B.n_active) + " given); use the plain form"};
+}
+
 // The job-table record of samples [i0, i0 + clen) of a band and its links
 // (written at `lh`), apart from the tile range, link offset, output, clock
 // window and flag pointers, which depend on the call. `perm` receives the
@@ -742,6 +750,7 @@ void render(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form,
     for (uint32_t q = 0; q < n_jobs; ++q) {
         const lpk_band_chunk &B = jobs[q];
         validate_job(ctx, B, q);
+        check_form_links(form, B, q);
         if (B.unoccluded && B.n_active) want_flags = true;
         if (B.has_antenna) any_antenna = true;
         total_links += B.n_active;
@@ -1037,6 +1046,7 @@ lpk_plan *plan_create(lpk_ctx *ctx, const lpk_band_chunk *bands, uint32_t n_band
     for (uint32_t b = 0; b < n_bands; ++b) {
         lpk_band_chunk &B = P->bands[b];
         validate_job(ctx, B, b);
+        check_form_links(form, B, b);
         if (B.i0 != 0 || B.delta_t || B.out_host || B.adc_codes_host || B.unoccluded)
             throw Fail{LPK_E_INVALID, "band " + std::to_string(b) +
                                           ": a plan band starts at sample 0, uses the device clock and has "

@@ -57,6 +57,12 @@ constexpr int kSuperShift = 3;
 __host__ __device__ constexpr int span_shift(int form, bool antenna, int tile_shift) {
     return tile_shift + (form == 1 && !antenna ? kSuperShift : 0);
 }
+// Transmitters per band-chunk the precision form takes. A 72-link band made
+// a device memory fault that no host-side check of the kernels' indexing has
+// explained, so the form is held to the largest count it has rendered
+// cleanly on the device (four tiles per anchor batch); the plain form has no
+// such limit.
+constexpr uint32_t kPrecisionMaxLinks = 32;
 // A dispatch packet counts its grid in work-items with 32 bits, so one
 // launch covers at most 2^32 / 512 - 1 tiles of the largest size; a render
 // call is sliced into launches of kMaxLaunchTiles.

@@ -277,6 +277,11 @@ def build_transmitter(cfg: TransmitterConfig, meta: SceneMeta, time_base: TimeBa
             _check(lib.lpk_cw_anchor_phase(render_epoch, anchor_epoch, sig.baseband_offset_hz, C.byref(ph)))
             tx.cw_anchored, tx.anchor_phase = True, ph.value
     elif isinstance(sig, (Bpsk, Ook, Qam16)):
+        # payload lengths are usize in the reference (src/config/mod.rs:200-208);
+        # the generators take them as uint32
+        n_payload = sig.payload_symbols if isinstance(sig, Qam16) else sig.payload_bits
+        if not 0 <= int(n_payload) < 2 ** 32:
+            raise ValueError(f"payload length {n_payload} out of range (transmitter {cfg.name!r})")
         if isinstance(sig, Qam16):
             sync = np.asarray(sig.sync_word, dtype=np.int64)
             if np.any((sync < 0) | (sync > 1)):
