@@ -334,6 +334,30 @@ int lpk_grid_fill_broadcast(const lpk_ephemeris *eph, double utc_offset_s, doubl
 int lpk_rinex3_gps_ephemeris(const char *text, uint32_t prn, double near_unix, double utc_offset_s,
                              lpk_ephemeris *out);
 
+/* ---- TLE trajectories (host builders, src/geometry/tle.rs:15-85). The
+ *      reference propagates with the third-party sgp4 crate (2.3, WGS72,
+ *      improved mode); this is SGP4/SDP4 restated from Spacetrack Report #3
+ *      as revised by Vallado et al. (AIAA 2006-6753). ---- */
+typedef struct lpk_tle {
+    uint32_t satnum;
+    int32_t epoch_year;           /* four digits (two-digit years 57-99 are 1957-1999) */
+    double epoch_day;             /* day of year, 1.0 = 1 January 0h UTC */
+    double epoch_unix;            /* UTC unix seconds of the epoch, whole microseconds (tle.rs:37-39) */
+    double jd_epoch;              /* Julian date of the epoch */
+    double ndot, nddot, bstar;    /* as in the TLE: rev/day^2 / 2, rev/day^3 / 6, 1 / earth radii */
+    double inclo_deg, nodeo_deg, ecco, argpo_deg, mo_deg, no_revday;
+} lpk_tle;
+/* Elements::from_tle: 69-column lines with valid checksums; LPK_E_INVALID otherwise */
+int lpk_tle_parse(const char *line1, const char *line2, lpk_tle *out);
+/* Constants::propagate: TEME position (km) and velocity (km/s) at `minutes` from epoch */
+int lpk_sgp4_teme(const lpk_tle *tle, double minutes, double r_km[3], double v_kms[3]);
+/* TleTrajectory::position_ecef / velocity_ecef (tle.rs:64-85); vel_ms may be NULL */
+int lpk_tle_state_ecef(const lpk_tle *tle, double t_unix, double pos_m[3], double vel_ms[3]);
+/* PrecomputedTrajectory::sample (src/geometry/precomputed.rs:47-70) over a
+ * TleTrajectory; xyz holds 3*n doubles, n = lpk_grid_size(duration_s, grid_dt_s). */
+int lpk_grid_fill_tle(const lpk_tle *tle, double scene_epoch_unix, double duration_s, double grid_dt_s,
+                      double *t0_out, double *xyz, uint64_t n);
+
 
 /* ---- learned warm start (no counterpart in the reference: its solve always
  * starts at t - 0.07 s, src/channel/mod.rs:75-96) ----

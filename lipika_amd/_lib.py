@@ -42,6 +42,13 @@ class Ephemeris(C.Structure):
                 ("iode", _u32), ("iodc", _u32), ("health", _u32), ("ura_index", _u32)]
 
 
+class Tle(C.Structure):
+    """lpk_tle (include/lpk.h): a parsed two-line element set."""
+    _fields_ = [("satnum", _u32), ("epoch_year", _i32), ("epoch_day", _d), ("epoch_unix", _d), ("jd_epoch", _d),
+                ("ndot", _d), ("nddot", _d), ("bstar", _d), ("inclo_deg", _d), ("nodeo_deg", _d), ("ecco", _d),
+                ("argpo_deg", _d), ("mo_deg", _d), ("no_revday", _d)]
+
+
 class TxDesc(C.Structure):
     _fields_ = [
         ("traj_kind", _i32), ("signal_kind", _i32),
@@ -194,6 +201,10 @@ def load() -> C.CDLL:
         "lpk_sv_position": (None, [P(Ephemeris), _d, P(_d)]),
         "lpk_grid_fill_broadcast": (_i32, [P(Ephemeris), _d, _d, _d, _d, P(_d), P(_d), _u64]),
         "lpk_rinex3_gps_ephemeris": (_i32, [C.c_char_p, _u32, _d, _d, P(Ephemeris)]),
+        "lpk_tle_parse": (_i32, [C.c_char_p, C.c_char_p, P(Tle)]),
+        "lpk_sgp4_teme": (_i32, [P(Tle), _d, P(_d), P(_d)]),
+        "lpk_tle_state_ecef": (_i32, [P(Tle), _d, P(_d), P(_d)]),
+        "lpk_grid_fill_tle": (_i32, [P(Tle), _d, _d, _d, P(_d), P(_d), _u64]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -220,6 +231,7 @@ EXPORTED = (
     "lpk_stream_to_files", "lpk_truth",
     "lpk_gps_time", "lpk_toe_snap", "lpk_ephemeris_from_kepler", "lpk_ephemeris_from_state", "lpk_lnav_length",
     "lpk_lnav_bits", "lpk_sv_position", "lpk_grid_fill_broadcast", "lpk_rinex3_gps_ephemeris",
+    "lpk_tle_parse", "lpk_sgp4_teme", "lpk_tle_state_ecef", "lpk_grid_fill_tle",
 )
 
 

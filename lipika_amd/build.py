@@ -15,7 +15,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "liblpk.so")
 
-SOURCES = ["lpk_api.hip", "lpk_kernels.hip", "lpk_lws.hip", "lpk_setup.cpp", "lpk_gpsnav.cpp"]
+SOURCES = ["lpk_api.hip", "lpk_kernels.hip", "lpk_lws.hip", "lpk_setup.cpp", "lpk_gpsnav.cpp", "lpk_sgp4.cpp"]
 
 # -ffp-contract=off: no FMA contraction anywhere, so the plain form rounds
 # exactly like the reference. Packed FP32 is switched off for the device code.

@@ -57,8 +57,8 @@ class RinexNav:
 
 @dataclass
 class Tle:
-    """An SGP4 trajectory from a TLE pair (src/config/mod.rs:432). Parsed and
-    written back, but not propagated: SGP4 is the third-party sgp4 crate."""
+    """An SGP4 trajectory from a TLE pair (src/config/mod.rs:432), propagated
+    on the host (include/lpk.h, lpk_tle_*) into the transmitter's grid."""
     line1: str
     line2: str
     kind: str = "tle"

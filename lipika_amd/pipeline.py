@@ -160,9 +160,19 @@ def load_rinex_ephemeris(traj: RinexNav, meta: SceneMeta) -> "_lib.Ephemeris":
     return eph
 
 
+def parse_tle(traj: Tle) -> "_lib.Tle":
+    """TleTrajectory::from_tle (src/geometry/tle.rs:33-44): the element set,
+    or ValueError as the reference's "TLE parse" error."""
+    e = _lib.Tle()
+    if _lib.load().lpk_tle_parse(traj.line1.encode(), traj.line2.encode(), C.byref(e)) != _lib.LPK_OK:
+        raise ValueError(f"TLE parse: {traj.line1!r} / {traj.line2!r} is not a valid element set")
+    return e
+
+
 def trajectory_state(traj, t_unix: float):
     """Trajectory::position_ecef and velocity_ecef (src/geometry/mod.rs:62-74,
-    src/geometry/kepler.rs:27-96) of the trajectories this mirror builds."""
+    src/geometry/kepler.rs:27-96, src/geometry/tle.rs:64-85) of the
+    trajectories this mirror builds."""
     if isinstance(traj, (StationaryGeodetic, StationaryEcef)):
         return station_ecef(traj), np.zeros(3)
     if isinstance(traj, CircularKeplerian):
@@ -187,6 +197,10 @@ def trajectory_state(traj, t_unix: float):
         vxe = (vx * c_t + vy * s_t) + w * ye
         vye = (-vx * s_t + vy * c_t) - w * xe
         return np.array([xe, ye, z1]), np.array([vxe, vye, vz1])
+    if isinstance(traj, Tle):
+        pos, vel = (C.c_double * 3)(), (C.c_double * 3)()
+        _check(_lib.load().lpk_tle_state_ecef(C.byref(parse_tle(traj)), t_unix, pos, vel))
+        return np.array(pos[:]), np.array(vel[:])
     raise NotImplementedError(f"trajectory {traj!r} has no state here")
 
 
@@ -255,8 +269,15 @@ def build_transmitter(cfg: TransmitterConfig, meta: SceneMeta, time_base: TimeBa
         tx.traj_kind = _lib.TRAJ_GRID
         tx.grid, tx.grid_t0, tx.grid_dt = grid, t0.value, TX_TRAJECTORY_GRID_DT_S
     elif isinstance(traj, Tle):
-        raise NotImplementedError(f"transmitter {cfg.name!r}: tle trajectories need SGP4 propagation (the sgp4 "
-                                  "crate of the reference), which this backend does not implement")
+        tle = parse_tle(traj)
+        n = int(lib.lpk_grid_size(meta.duration_s, TX_TRAJECTORY_GRID_DT_S))
+        grid = np.empty((3, n), dtype=np.float64)
+        t0 = C.c_double()
+        if lib.lpk_grid_fill_tle(C.byref(tle), meta.epoch_unix, meta.duration_s, TX_TRAJECTORY_GRID_DT_S,
+                                 C.byref(t0), grid.ctypes.data_as(C.POINTER(C.c_double)), n) != _lib.LPK_OK:
+            raise ValueError(f"transmitter {cfg.name!r}: sgp4 propagation failed over the scene")
+        tx.traj_kind = _lib.TRAJ_GRID
+        tx.grid, tx.grid_t0, tx.grid_dt = grid, t0.value, TX_TRAJECTORY_GRID_DT_S
     else:
         raise NotImplementedError(f"trajectory {traj!r} is not on the GPU path")
 

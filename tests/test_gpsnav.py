@@ -265,7 +265,7 @@ def test_scene_with_full_nav_and_rinex_prepares(tmp_path):
     assert r.traj_kind == _lib.TRAJ_GRID and r.grid.shape[0] == 3
     assert 26.2e6 < np.linalg.norm(r.grid[:, 0]) < 26.9e6
     bad = [TransmitterConfig("t", 1575.42e6, CaCode(3, epoch), Tle("1 26407U", "2 26407"), 27.0)]
-    with pytest.raises(NotImplementedError, match="SGP4"):
+    with pytest.raises(ValueError, match="TLE parse"):
         prepare(SceneConfig(SceneMeta(epoch, 1.0), bad, [lab_receiver([band])]))
     with warnings.catch_warnings(record=True) as w:
         warnings.simplefilter("always")

@@ -723,6 +723,37 @@ def test_full_nav_and_rinex_sources_match_oracle(backend):
         compare_taps(backend, orc, sc, 0, 0, 0, seed, slot, 200_000, form=_lib.FORM_PRECISION, label="full NAV")
 
 
+def test_tle_sources_match_oracle(backend):
+    """Two GPS satellites from TLEs (examples/gps_real_tle.toml), propagated
+    by SGP4 into their grids, with full LNAV from the ephemeris fitted to each
+    TLE orbit: the plain form bit-identical to the oracle with the oracle's
+    clock offsets, the precision form within its bounds."""
+    from lipika_amd.config import CaCode, ReceiverBandConfig, SceneConfig, SceneMeta, Tle, TransmitterConfig
+    epoch = 1780268400.0
+    tles = {24: ("1 38833U 12053A   26150.86639939  .00000051  00000+0  00000+0 0  9997",
+                 "2 38833  53.5827 144.3088 0179371  65.5174 296.3427  2.00567269 99098"),
+            22: ("1 26407U 00040A   26150.58242625  .00000046  00000+0  00000+0 0  9999",
+                 "2 26407  54.8538 215.8617 0121278 302.3883 230.2243  2.00557636189594")}
+    txs = [TransmitterConfig(f"gps_prn{p}", 1575.42e6, CaCode(p, epoch, "full"), Tle(*l), 27.0)
+           for p, l in tles.items()]
+    band = ReceiverBandConfig("l1", 1575.42e6, 2.048e6, noise_power_w=0.0)
+    sc = prepare(SceneConfig(SceneMeta(epoch, 0.3), txs,
+                             [util.lab_receiver([band], NodeClockSpec(allan_deviation_1s=1e-11))]))
+    assert all(t.traj_kind == _lib.TRAJ_GRID and t.nav_mode == _lib.NAV_FULL for t in sc.transmitters)
+    seed = 41
+    backend.upload(sc, seed)
+    orc = OracleScene(sc, seed)
+    check_band_both_clocks(backend, orc, sc, 0, 0, 0, seed, "GPS from TLEs, 2.048 MHz", chunk=150_000)
+    o, _, _ = orc.render_band(0)
+    g, _ = gpu_band(backend, sc, 0, 0, 0, seed, form=_lib.FORM_PRECISION, chunk=150_000)
+    peak = float(np.max(np.abs(o)))
+    d = np.abs(g.astype(np.complex128) - o.astype(np.complex128))
+    _say(f"GPS from TLEs, precision form: max|d|/peak={d.max() / peak:.3e}")
+    assert d.max() <= 2e-6 * peak
+    for slot in range(2):
+        compare_taps(backend, orc, sc, 0, 0, 0, seed, slot, 200_000, form=_lib.FORM_PRECISION, label="TLE")
+
+
 def _many_links_scene():
     """72 in-band transmitters: 30 C/A and 30 BPSK on circular orbits, 12 CW
     tones on the ground, one 2.048 MHz L1 band of 0.01 s."""
