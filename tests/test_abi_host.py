@@ -47,7 +47,8 @@ def test_ctypes_layout_matches_header(tmp_path):
         pytest.skip("gcc is not available")
     structs = {"lpk_tx_desc": _lib.TxDesc, "lpk_rx_desc": _lib.RxDesc, "lpk_post_op": _lib.PostOp,
                "lpk_band_chunk": _lib.BandChunk, "lpk_tuning": _lib.Tuning,
-               "lpk_lws_train_cfg": _lib.LwsTrainCfg, "lpk_lws_train_report": _lib.LwsTrainReport}
+               "lpk_lws_train_cfg": _lib.LwsTrainCfg, "lpk_lws_train_report": _lib.LwsTrainReport,
+               "lpk_ephemeris": _lib.Ephemeris, "lpk_tle": _lib.Tle}
     lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "lpk.h"', "int main(void) {"]
     for cname, ct in structs.items():
         lines.append(f'    printf("{cname} size %zu\\n", sizeof({cname}));')
