@@ -219,8 +219,27 @@ class SceneConfig:
 
 @dataclass
 class SkyConfig:
+    """src/config/mod.rs:150-185: transmitters without a scene or receivers."""
     transmitters: List[TransmitterConfig] = field(default_factory=list)
     gps_utc_offset_s: float = 18.0
+
+    @staticmethod
+    def from_toml_str(text: str) -> "SkyConfig":
+        d = _toml.loads(text)
+        unknown = set(d) - {"transmitter", "gps_utc_offset_s"}
+        if unknown:   # serde(deny_unknown_fields)
+            raise ValueError(f"unknown sky field(s): {sorted(unknown)}")
+        return SkyConfig([_transmitter(t) for t in d.get("transmitter", [])],
+                         float(d.get("gps_utc_offset_s", 18.0)))
+
+    @staticmethod
+    def from_toml_file(path: str) -> "SkyConfig":
+        with open(path, "r", encoding="utf-8") as f:
+            text = f.read()
+        try:
+            return SkyConfig.from_toml_str(text)
+        except (ValueError, KeyError) as e:
+            raise ValueError(f"parsing sky '{path}': {e}") from e
 
 
 # ---------------------------------------------------------------- TOML

@@ -867,19 +867,25 @@ def render_window(sky: SkyConfig, receiver: ReceiverConfig, band: ReceiverBandCo
         n = pb.n
         i0, stopped = 0, False
         buf = np.empty(min(chunk, max(n, 1)), dtype=np.complex64)
+        # per contributing transmitter: unoccluded for some emitted sample
+        # (src/pipeline.rs:972-974, the summary's visible_transmitters)
+        seen = np.zeros(len(pb.active), dtype=np.uint8)
+        flags = np.zeros(max(len(pb.active), 1), dtype=np.uint8)
         while i0 < n:
             clen = min(chunk, n - i0)
-            j, keep = be.job(0, 0, 0, options.seed, i0, clen, out_host=buf[:clen])
+            flags[:] = 0
+            j, keep = be.job(0, 0, 0, options.seed, i0, clen, out_host=buf[:clen], unoccluded=flags)
             be.render([j], form_code(form))
             be.sync()
+            seen |= flags[:len(pb.active)]
             flow = sink(buf[:clen].copy())
             i0 += clen
             if flow == BREAK and i0 < n:
                 stopped = True
                 break
         names = [sc.transmitters[i].name for i in pb.active]
-        return WindowSummary(rust_round(band.sample_rate_hz * duration_s), i0, stopped, names,
-                             list(names) if i0 > 0 else [])
+        visible = [nm for nm, f in zip(names, seen) if f]
+        return WindowSummary(rust_round(band.sample_rate_hz * duration_s), i0, stopped, names, visible)
     finally:
         if own:
             be.close()

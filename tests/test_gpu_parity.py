@@ -754,6 +754,69 @@ def test_tle_sources_match_oracle(backend):
         compare_taps(backend, orc, sc, 0, 0, 0, seed, slot, 200_000, form=_lib.FORM_PRECISION, label="TLE")
 
 
+def test_horizon_gate_tle_rise_is_byte_identical(backend):
+    """tests/horizon_gate.rs:378-440 on the GPU: the example sky
+    (examples/hiber_beacon_sky.toml: an ISS-like TLE carrying a burst-gated
+    BPSK beacon) plus a CW tone on the same orbit, over Glasgow with a 0 deg
+    antenna cutoff, 6 s at 48 kHz around the rise before the pass that
+    culminates at Unix 1717260990: silent, then never silent; both sources
+    visible; the gated render byte-identical to the ungated one at four chunk
+    sizes, one of them a single gate stretch and one odd."""
+    import math
+    import os
+    from lipika_amd.config import (Cw, FrontendConfig, ReceiverBandConfig, ReceiverConfig, SkyConfig,
+                                   StationaryGeodetic, TransmitterConfig)
+    from lipika_amd.pipeline import WindowOptions, parse_tle, render_window
+    sky = SkyConfig.from_toml_file(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                                "hiber_beacon_sky.toml"))
+    orbit = sky.transmitters[0].trajectory
+    tle = parse_tle(orbit)
+    lib = _lib.load()
+    o = (C.c_double * 3)()
+    lib.lpk_geodetic_to_ecef(55.86, -4.25, 20.0, o)
+    o = np.array(o[:])
+
+    def el(t):
+        p = (C.c_double * 3)()
+        assert lib.lpk_tle_state_ecef(C.byref(tle), t, p, None) == _lib.LPK_OK
+        d = np.array(p[:]) - o
+        return math.degrees(math.asin(np.dot(d, o / np.linalg.norm(o)) / np.linalg.norm(d)))
+
+    rise = 1_717_260_990.0
+    while el(rise) > 0.0:
+        rise -= 1.0
+        assert 1_717_260_990.0 - rise < 900.0
+    start = rise - 3.0
+    sky.transmitters.append(TransmitterConfig("tone", 401.15e6, Cw(1000.0), orbit, 0.0))
+    rx = ReceiverConfig("glasgow", StationaryGeodetic(55.86, -4.25, 20.0))
+    fe = FrontendConfig(antenna={"peak_dbi": 0.0, "horizon_dbi": 0.0, "fb_db": 0.0, "cutoff_deg": 0.0})
+    band = ReceiverBandConfig("uhf", 401.15e6, 48_000.0, noise_power_w=0.0, frontend=fe)
+
+    def collect(gate, chunk):
+        backend.set_horizon_gate(gate)
+        iq = []
+        summary = render_window(sky, rx, band, start, 6.0, WindowOptions(11, chunk, 1), iq.append, backend=backend)
+        iq = np.concatenate(iq)
+        assert summary.n_samples_emitted == len(iq)
+        return iq, summary
+
+    try:
+        ref, ref_summary = collect(False, 1 << 20)
+        assert len(ref) == 288_000
+        nz = np.flatnonzero(np.abs(ref) > 0.0)
+        first = int(nz[0])
+        _say(f"TLE rise: first signal at sample {first}")
+        assert 48_000 < first < 240_000
+        assert np.all(np.abs(ref[first:]) > 0.0)
+        assert ref_summary.visible_transmitters == ["hiber_beacon", "tone"]
+        for chunk in (1 << 16, 4800, 4799, 100_000):
+            g, summary = collect(True, chunk)
+            assert g.tobytes() == ref.tobytes(), chunk
+            assert summary.visible_transmitters == ["hiber_beacon", "tone"]
+    finally:
+        backend.set_horizon_gate(True)
+
+
 def _many_links_scene():
     """72 in-band transmitters: 30 C/A and 30 BPSK on circular orbits, 12 CW
     tones on the ground, one 2.048 MHz L1 band of 0.01 s."""

@@ -197,3 +197,30 @@ def test_grid_and_scene_from_tle(lib):
     assert t.traj_kind == _lib.TRAJ_GRID and np.array_equal(t.grid[:, 0], ecef(lib, e, t.grid_t0)[0])
     d = decode_ephemeris(decode_words(t.nav_bits))
     assert abs(d["sqrt_a"] ** 2 - 26_560_000.0) < 1_500_000.0 and abs(math.degrees(d["i0"]) - 54.85) < 0.5
+
+
+def test_example_sky_pass_over_glasgow(lib):
+    """tests/horizon_gate.rs:380-396: the example sky's ISS-like TLE,
+    propagated 152 days past its epoch, passes over Glasgow culminating at
+    Unix 1717260990 and rises less than 900 s before. SGP4's along-track
+    drift over that span would move the pass by minutes for a wrong drag or
+    secular term; here the elevation peaks within 10 s of that time."""
+    e = parse(lib, ISS)
+    o = (C.c_double * 3)()
+    lib.lpk_geodetic_to_ecef(55.86, -4.25, 20.0, o)
+    o = np.array(o[:])
+    up = o / np.linalg.norm(o)
+
+    def el(t):
+        p, _ = ecef(lib, e, t)
+        d = p - o
+        return math.degrees(math.asin(np.dot(d, up) / np.linalg.norm(d)))
+
+    cul = 1_717_260_990.0
+    assert el(cul) > 30.0
+    ts = np.arange(cul - 60.0, cul + 60.0, 1.0)
+    assert abs(ts[int(np.argmax([el(t) for t in ts]))] - cul) <= 10.0
+    rise = cul
+    while el(rise) > 0.0:
+        rise -= 1.0
+        assert cul - rise < 900.0
