@@ -270,6 +270,24 @@ def test_window_argument_validation():
             render_window(SkyConfig(), rx, b, start, dur, WindowOptions(), lambda c: None)
 
 
+def test_a_sky_file_holds_transmitters_only():
+    """tests/window_render.rs:475-484: scene-only tables are rejected, so a
+    scene file is not silently read as a sky; the example sky file loads; a
+    missing file is an error."""
+    import os
+    with pytest.raises(ValueError):
+        SkyConfig.from_toml_str("[scene]\nepoch_unix = 0.0\nduration_s = 1.0\n")
+    with pytest.raises(ValueError):
+        SkyConfig.from_toml_str('[[receiver]]\nname = "rx"\ntrajectory = { kind = "stationary_ecef", x_m = 1.0, '
+                                'y_m = 0.0, z_m = 0.0 }\nband = []\n')
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "hiber_beacon_sky.toml")
+    sky = SkyConfig.from_toml_file(path)
+    assert sky.transmitters[0].name == "hiber_beacon" and sky.transmitters[0].trajectory.kind == "tle"
+    assert SkyConfig.from_toml_str("").transmitters == []
+    with pytest.raises(OSError):
+        SkyConfig.from_toml_file("/nonexistent/sky.toml")
+
+
 def test_workload_totals():
     p1 = prepare(workloads.scene1())
     assert workloads.totals(p1) == (480_000, 480_000)
