@@ -812,7 +812,14 @@ def test_horizon_gate_tle_rise_is_byte_identical(backend):
         for chunk in (1 << 16, 4800, 4799, 100_000):
             g, summary = collect(True, chunk)
             assert g.tobytes() == ref.tobytes(), chunk
-            assert summary.visible_transmitters == ["hiber_beacon", "tone"]
+            assert summary == ref_summary
+        # a window that ends before the rise: in band, never visible, all zeros
+        backend.set_horizon_gate(True)
+        iq = []
+        summary = render_window(sky, rx, band, start - 30.0, 2.0, WindowOptions(11, 1 << 16, 1), iq.append,
+                                backend=backend)
+        assert not np.any(np.concatenate(iq).view(np.uint32))
+        assert summary.contributing_transmitters == ["hiber_beacon", "tone"] and summary.visible_transmitters == []
     finally:
         backend.set_horizon_gate(True)
 
