@@ -846,6 +846,45 @@ def _many_links_scene():
     return sc
 
 
+def test_precision_form_at_its_32_link_limit(backend):
+    """The precision form at its largest band-chunk: 32 in-band transmitters
+    of every transmitter group (10 C/A and 10 BPSK on circular orbits, 8 CW
+    and 4 burst-gated CW on the ground), against the oracle within its
+    bounds, with taps on the first and last link of each group."""
+    from lipika_amd.config import (Bpsk, CaCode, CircularKeplerian, Cw, GateConfig, ReceiverBandConfig, SceneConfig,
+                                   SceneMeta, StationaryGeodetic, TransmitterConfig)
+    txs = []
+    for k in range(10):
+        txs.append(TransmitterConfig(f"ca{k}", 1575.42e6 + 1e3 * k, CaCode(k + 1, util.EPOCH),
+                                     CircularKeplerian(26.56e6, 55.0, 36.0 * k, 17.0 * k, util.EPOCH), 27.0))
+    for k in range(10):
+        txs.append(TransmitterConfig(f"bp{k}", 1575.42e6 - 2e3 * k, Bpsk(9600.0, 512, 40 + k, [1, -1, 1, 1]),
+                                     CircularKeplerian(7.0e6 + 1e4 * k, 60.0, 33.0 * k, 13.0 * k, util.EPOCH), -10.0))
+    for k in range(8):
+        txs.append(TransmitterConfig(f"cw{k}", 1575.42e6 + 5e3 * k, Cw(200.0 * k),
+                                     StationaryGeodetic(50.0 + k, -4.0 + k, 100.0), -20.0))
+    for k in range(4):
+        txs.append(TransmitterConfig(f"gated{k}", 1575.42e6 - 7e3 * k, Cw(150.0 * k),
+                                     StationaryGeodetic(52.0 + k, -2.0 + k, 300.0), -20.0,
+                                     gate=GateConfig(0.004, [(0.0005 * k, 0.0015)], 0.0)))
+    band = ReceiverBandConfig("l1", 1575.42e6, 2.048e6, noise_power_w=0.0)
+    sc = prepare(SceneConfig(SceneMeta(util.EPOCH, 0.01), txs,
+                             [util.lab_receiver([band], NodeClockSpec(allan_deviation_1s=1e-11))]))
+    assert len(sc.receivers[0].bands[0].active) == 32
+    seed = 23
+    backend.upload(sc, seed)
+    orc = OracleScene(sc, seed)
+    o, _, _ = orc.render_band(0)
+    peak = float(np.max(np.abs(o)))
+    for chunk in (20_480, 7_000):
+        g, _ = gpu_band(backend, sc, 0, 0, 0, seed, form=_lib.FORM_PRECISION, chunk=chunk)
+        d = np.abs(g.astype(np.complex128) - o.astype(np.complex128))
+        _say(f"32 links, precision form, chunks of {chunk}: max|d|/peak={d.max() / peak:.3e}")
+        assert np.all(np.isfinite(g)) and d.max() <= 2e-6 * peak
+    for slot in (0, 9, 10, 19, 20, 27, 28, 31):
+        compare_taps(backend, orc, sc, 0, 0, 0, seed, slot, 20_480, form=_lib.FORM_PRECISION, label="32 links")
+
+
 def test_plain_form_with_72_links(backend):
     """72 in-band transmitters in the plain form, with the oracle's clock
     offsets, against the oracle."""
