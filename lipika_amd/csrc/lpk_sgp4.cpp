@@ -199,8 +199,10 @@ void dscom(double epoch, double ep, double argpp, double tc, double inclp, doubl
 }
 
 // Lunar-solar periodics (dpper). `init`: the call made while initialising,
-// which evaluates the epoch values (stored in peo..pho) and applies nothing.
-void dpper(Sat &s, double t, bool init, double &ep, double &inclp, double &nodep, double &argpp, double &mp) {
+// which evaluates the terms and applies nothing. Later calls subtract the
+// epoch values peo..pho, which dscom leaves at zero, so the full periodics
+// apply.
+void dpper(const Sat &s, double t, bool init, double &ep, double &inclp, double &nodep, double &argpp, double &mp) {
     const double zns = 1.19459e-5, zes = 0.01675, znl = 1.5835218e-4, zel = 0.05490;
     double zm = init ? s.zmos : s.zmos + zns * t;
     double zf = zm + 2.0 * zes * std::sin(zm);
@@ -637,8 +639,7 @@ int propagate(const Sat &s, double t, double r[3], double v[3]) {
     double ep = em, xincp = inclm, argpp = argpm, nodep = nodem, mp = mm, sinip = sinim, cosip = cosim;
     double aycof = s.aycof, xlcof = s.xlcof, con41 = s.con41, x1mth2 = s.x1mth2, x7thm1 = s.x7thm1;
     if (s.deep) {
-        Sat &ms = const_cast<Sat &>(s);  // dpper reads only, with init false
-        dpper(ms, t, false, ep, xincp, nodep, argpp, mp);
+        dpper(s, t, false, ep, xincp, nodep, argpp, mp);
         if (xincp < 0.0) {
             xincp = -xincp;
             nodep = nodep + kPi;
