@@ -31,6 +31,26 @@ GPS22 = ("1 26407U 00040A   26150.58242625  .00000046  00000+0  00000+0 0  9999"
 ISS = ("1 25544U 98067A   24001.50000000  .00010000  00000+0  18000-3 0  9994",
        "2 25544  51.6400 100.0000 0001000  90.0000 270.0000 15.50000000000009")
 
+# examples/gps_real_tle.toml: eight live GPS satellites (Celestrak, epoch ~2026-150)
+GPS_EXAMPLE = [
+    ("1 38833U 12053A   26150.86639939  .00000051  00000+0  00000+0 0  9997",
+     "2 38833  53.5827 144.3088 0179371  65.5174 296.3427  2.00567269 99098"),
+    ("1 32260U 07047A   26150.77386177 -.00000055  00000+0  00000+0 0  9990",
+     "2 32260  54.0827  80.7833 0166076  87.0458 274.8256  2.00556201136463"),
+    ("1 26407U 00040A   26150.58242625  .00000046  00000+0  00000+0 0  9999",
+     "2 26407  54.8538 215.8617 0121278 302.3883 230.2243  2.00557636189594"),
+    ("1 45854U 20041A   26151.52781884 -.00000085  00000+0  00000+0 0  9993",
+     "2 45854  56.6592  29.5897 0063281 204.6776 332.6203  2.00559374 43701"),
+    ("1 67588U 26017A   26150.24909249 -.00000043  00000+0  00000+0 0  9999",
+     "2 67588  55.0776  89.4749 0022608 256.9258 102.7871  2.00571628431254"),
+    ("1 41019U 15062A   26150.82698463 -.00000088  00000+0  00000+0 0  9994",
+     "2 41019  56.9480  31.4799 0111519 231.2324 127.8139  2.00567167 77488"),
+    ("1 46826U 20078A   26150.43676341  .00000051  00000+0  00000+0 0  9995",
+     "2 46826  53.9818 211.4986 0072702 206.7371 238.5468  2.00565060 41158"),
+    ("1 28874U 05038A   26150.41242926 -.00000056  00000+0  00000+0 0  9999",
+     "2 28874  54.8931 273.8947 0127762 296.0439  71.0693  2.00567025151481"),
+]
+
 
 @pytest.fixture(scope="module")
 def lib():
@@ -224,3 +244,19 @@ def test_example_sky_pass_over_glasgow(lib):
     while el(rise) > 0.0:
         rise -= 1.0
         assert cul - rise < 900.0
+
+
+def test_example_constellation_is_above_fifteen_degrees(lib):
+    """examples/gps_real_tle.toml:3-5: the eight GPS satellites of the example
+    are all above 15 deg elevation over Glasgow at 2026-05-31 23:00 UTC; each
+    propagated about half a day from its own epoch through the deep-space
+    model and rotated by that instant's GMST."""
+    o = (C.c_double * 3)()
+    lib.lpk_geodetic_to_ecef(55.86, -4.25, 10.0, o)
+    o = np.array(o[:])
+    t = 1780268400.0
+    for l1, l2 in GPS_EXAMPLE:
+        p, _ = ecef(lib, parse(lib, (l1, l2)), t)
+        d = p - o
+        el = math.degrees(math.asin(np.dot(d, o) / (np.linalg.norm(d) * np.linalg.norm(o))))
+        assert el > 15.0, (l1[2:7], el)
