@@ -50,10 +50,13 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def plan_leg(be, lib, sc, bufs, seed, form, steps, warmup, chunk=1 << 20):
+def plan_leg(be, lib, sc, bufs, seed, form, steps, warmup, chunk=1 << 24):
     """The batching form: a streaming plan renders chunk k of every band per
     step as one graph replay, with the context's (autotuned) tuning; a pass is
-    every step of the scene."""
+    every step of the scene. Chunks of 16 Mi samples per band: the replays
+    then hold enough tiles to fill the device (configuration 4: 15 replays;
+    1 Mi-sample chunks take 235 replays whose tails show, and 1.4x the time),
+    at about 4 GB of clock-walk windows for its 64 receivers."""
     from lipika_amd.pipeline import form_code
     units = [(r, b) for r, rx in enumerate(sc.receivers) for b, _ in enumerate(rx.bands)]
     tuning = be.tuning()
