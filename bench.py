@@ -338,7 +338,8 @@ def main():
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_leg(sc, 2026, args.cpu_seconds)
     if args.config4 and args.config != 4:
-        # the north-star target is stated on configuration 4 (uneven bands)
+        # the speed-up targets (against the plain form and the CPU path) are stated on
+        # configuration 4 (uneven bands)
         sc4, out4, contrib4, res4, plan4 = gpu_leg(args, 4, ["plain", "precision"], max(1, args.steps // 5), 1,
                                                     batching=True)
         c4 = {"workload": "config 4: " + WORKLOADS[4], "contributions_per_step": contrib4,
