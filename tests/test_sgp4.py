@@ -260,3 +260,24 @@ def test_example_constellation_is_above_fifteen_degrees(lib):
         d = p - o
         el = math.degrees(math.asin(np.dot(d, o) / (np.linalg.norm(d) * np.linalg.norm(o))))
         assert el > 15.0, (l1[2:7], el)
+
+
+def test_a_decayed_orbit_is_an_error(lib):
+    """The reference's propagate panics where SGP4 reports an error
+    (tle.rs:66-67, `expect`); here the state and the grid calls return
+    LPK_E_INVALID, and scene preparation raises ValueError. The example's
+    ISS-like orbit (B* 1.8e-4) has decayed 3 000 days after its epoch."""
+    from lipika_amd.config import Cw, ReceiverBandConfig, SceneConfig, SceneMeta, Tle, TransmitterConfig
+    from lipika_amd.pipeline import prepare
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from util import lab_receiver
+    e = parse(lib, ISS)
+    r, v = (C.c_double * 3)(), (C.c_double * 3)()
+    assert lib.lpk_sgp4_teme(C.byref(e), 100 * 1440.0, r, v) == _lib.LPK_OK
+    assert lib.lpk_sgp4_teme(C.byref(e), 3000 * 1440.0, r, v) == -1
+    late = e.epoch_unix + 3000 * 86400.0
+    assert lib.lpk_tle_state_ecef(C.byref(e), late, r, v) == -1
+    band = ReceiverBandConfig("uhf", 437e6, 48e3, noise_power_w=0.0)
+    txs = [TransmitterConfig("iss", 437e6, Cw(0.0), Tle(*ISS), 0.0)]
+    with pytest.raises(ValueError, match="sgp4"):
+        prepare(SceneConfig(SceneMeta(late, 1.0), txs, [lab_receiver([band])]))
