@@ -281,3 +281,22 @@ def test_a_decayed_orbit_is_an_error(lib):
     txs = [TransmitterConfig("iss", 437e6, Cw(0.0), Tle(*ISS), 0.0)]
     with pytest.raises(ValueError, match="sgp4"):
         prepare(SceneConfig(SceneMeta(late, 1.0), txs, [lab_receiver([band])]))
+
+
+def test_geostationary_orbit_stays_over_one_longitude(lib):
+    """The 24 h synchronous-resonance branch with Lyddane's low-inclination
+    periodics (inclination 0.02 deg): a geostationary element set stays at
+    the GEO radius and, Earth-fixed, within 0.2 deg of one longitude and of
+    the equator over two days (its mean motion of 1.0027 rev/day is one
+    sidereal day)."""
+    geo = (with_checksum("1 28626U 05004A   06176.43865743 -.00000282  00000-0  10000-3 0  122"),
+           with_checksum("2 28626   0.0177 257.7591 0001835  40.4209 214.8001  1.00271000  503"))
+    e = parse(lib, geo)
+    lons, lats = [], []
+    for h in range(0, 49, 3):
+        p, _ = ecef(lib, e, e.epoch_unix + h * 3600.0)
+        assert abs(np.linalg.norm(p) - 42_164e3) < 50e3, h
+        lons.append(math.degrees(math.atan2(p[1], p[0])))
+        lats.append(math.degrees(math.asin(p[2] / np.linalg.norm(p))))
+    lons = np.unwrap(np.radians(lons))
+    assert np.degrees(np.ptp(lons)) < 0.2 and max(abs(x) for x in lats) < 0.2
