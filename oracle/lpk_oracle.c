@@ -2,7 +2,7 @@
  * lpk_oracle.c -- CPU FP64 oracle for the per-chunk IQ path.
  *
  * TEST INFRASTRUCTURE ONLY (see lpk_oracle.h). Never linked into the product.
- * Built with -O2 -ffp-contract=off and without fast-math so that every f64
+ * Built with -O3 -ffp-contract=off and without fast-math so that every f64
  * and f32 operation rounds exactly as the reference's Rust (which never
  * contracts to FMA) does. Transcendentals (sin, cos, sqrt, pow, log) come from
  * glibc libm, which is also what the reference's f64::sin etc. call on Linux.
@@ -970,6 +970,57 @@ int orc_render_band(const orc_tx *txs, uint32_t n_tx, const orc_rx *rx, const or
         orc_post_ops(band->post, band->n_post, s.acc, clen, codes ? codes + 2 * i0 : NULL);
         memcpy(out_iq + 2 * i0, s.acc, 2 * clen * sizeof(float));
     }
+    scratch_free(&s);
+    return 0;
+}
+
+/* A window of one band, samples [i0, i0 + n), as one chunk of the chunk body
+ * (src/pipeline.rs:915-983) with the receiver clock offsets delta_t[0..n)
+ * supplied by the caller instead of walked from the band's start. Taps, when
+ * given, hold every active slot: tap_*[k * n + j]. Returns 0, or -1 on bad
+ * input. */
+int orc_render_window(const orc_tx *txs, uint32_t n_tx, const orc_band *band, const double obs[3],
+                      uint64_t i0, uint64_t n, const double *delta_t, float *out_iq, int16_t *codes,
+                      double *tap_t_emit, double *tap_range, double *tap_phase) {
+    for (uint32_t k = 0; k < band->n_active; ++k)
+        if (band->active[k] >= n_tx) return -1;
+    if (!delta_t || n == 0) return n == 0 ? 0 : -1;
+    chunk_scratch s;
+    if (!scratch_alloc(&s, n, band->n_active)) {
+        scratch_free(&s);
+        return -1;
+    }
+    const double fs = band->fs_hz;
+    for (uint64_t j = 0; j < n; ++j) s.t_local[j] = (double)(i0 + j) / fs;   /* src/pipeline.rs:918 */
+    for (uint64_t j = 0; j < n; ++j) s.delta_t[j] = delta_t[j];
+    for (uint64_t j = 0; j < n; ++j) s.t_utc[j] = s.t_local[j] - s.delta_t[j];   /* :926 */
+    const uint32_t na = band->n_active;
+    for (uint32_t k = 0; k < na; ++k) {
+        float *c = s.contrib + (uint64_t)k * 2 * n;
+        memset(c, 0, 2 * n * sizeof(float));
+        tx_contribution(&txs[band->active[k]], band, obs, s.t_local, s.t_utc, s.delta_t, n, band->lo_hz,
+                        band->epoch_unix, s.t_emit + (uint64_t)k * n, s.ranges + (uint64_t)k * n,
+                        s.bb + (uint64_t)k * 2 * n, c, band->unoccluded ? band->unoccluded + k : NULL,
+                        tap_t_emit ? tap_t_emit + (uint64_t)k * n : NULL, tap_range ? tap_range + (uint64_t)k * n : NULL,
+                        tap_phase ? tap_phase + (uint64_t)k * n : NULL);
+    }
+    /* src/pipeline.rs:965-976 */
+    memset(s.acc, 0, 2 * n * sizeof(float));
+    for (uint32_t k = 0; k < na; ++k) {
+        const float *c = s.contrib + (uint64_t)k * 2 * n;
+        for (uint64_t j = 0; j < 2 * n; ++j) s.acc[j] = s.acc[j] + c[j];
+    }
+    const float sigma = (float)sqrt(band->noise_power_w / 2.0);
+    if (sigma > 0.0f) {
+        for (uint64_t j = 0; j < n; ++j) {
+            float zr, zi;
+            orc_awgn_normals(band->noise_seed, i0 + j, &zr, &zi);
+            s.acc[2 * j] += 0.0f + sigma * zr;
+            s.acc[2 * j + 1] += 0.0f + sigma * zi;
+        }
+    }
+    orc_post_ops(band->post, band->n_post, s.acc, n, codes);
+    memcpy(out_iq, s.acc, 2 * n * sizeof(float));
     scratch_free(&s);
     return 0;
 }

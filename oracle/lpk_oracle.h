@@ -192,6 +192,13 @@ int orc_render_band(const orc_tx *txs, uint32_t n_tx, const orc_rx *rx, const or
                     uint32_t tap_slot, double *tap_t_emit, double *tap_range, double *tap_phase,
                     double *tap_delta_t);
 
+/* Samples [i0, i0 + n) of one band with caller-supplied clock offsets
+ * delta_t[0..n) (a window late in a long band without walking its clock);
+ * optional taps of every active slot k at tap_*[k * n + j]. */
+int orc_render_window(const orc_tx *txs, uint32_t n_tx, const orc_band *band, const double obs[3],
+                      uint64_t i0, uint64_t n, const double *delta_t, float *out_iq, int16_t *codes,
+                      double *tap_t_emit, double *tap_range, double *tap_phase);
+
 /* The reference CPU path as a timed baseline: renders every band over samples
  * [0, n_limit[b]) with the reference's two-level parallelism (bands in
  * parallel, transmitters of a chunk in parallel, fixed-order sum) on n_threads

@@ -117,6 +117,8 @@ def load() -> C.CDLL:
         "orc_post_ops": (None, [P(OrcPost), _u32, P(C.c_float), _u64, P(C.c_int16)]),
         "orc_render_band": (C.c_int, [P(OrcTx), _u32, P(OrcRx), P(OrcBand), _u64, _u64, P(C.c_float),
                                       P(C.c_int16), _u32, P(_d), P(_d), P(_d), P(_d)]),
+        "orc_render_window": (C.c_int, [P(OrcTx), _u32, P(OrcBand), P(_d), _u64, _u64, P(_d), P(C.c_float),
+                                         P(C.c_int16), P(_d), P(_d), P(_d)]),
         "orc_render_scene_hashed": (_u64, [P(OrcTx), _u32, P(OrcRx), P(OrcBand), _u32, P(_u64), _u64,
                                            C.c_int, P(_u64)]),
     }
@@ -227,6 +229,29 @@ class OracleScene:
         if rc != 0:
             raise ValueError("oracle rejected the band")
         return out, cd, taps
+
+    def render_window(self, work_idx: int, i0: int, delta_t: np.ndarray, taps: bool = False,
+                      converged: bool = False, codes: bool = False):
+        """Samples [i0, i0 + len(delta_t)) of unit `work_idx` with the given
+        receiver clock offsets (orc_render_window); returns (iq, codes, taps)
+        with taps = (t_emit, range, phase), each [n_active, n], or None."""
+        lib = load()
+        r_idx, pb = self.units[work_idx]
+        dt = np.ascontiguousarray(delta_t, dtype=np.float64)
+        n = len(dt)
+        band = self.bands[work_idx]
+        band.unoccluded = P(C.c_uint8)()
+        band.converged = 1 if converged else 0
+        obs = np.ascontiguousarray(self.rxs[r_idx].pos_ecef[:], dtype=np.float64)
+        out = np.empty(n, dtype=np.complex64)
+        cd = np.empty(2 * n, dtype=np.int16) if codes else None
+        tp = [np.empty((len(pb.active), n), dtype=np.float64) for _ in range(3)] if taps else [None] * 3
+        rc = lib.orc_render_window(self.txs, self.n_tx, C.byref(band), _ptr(obs, _d), i0, n, _ptr(dt, _d),
+                                   out.ctypes.data_as(P(C.c_float)), _ptr(cd, C.c_int16), *[_ptr(a, _d) for a in tp])
+        band.converged = 0
+        if rc != 0:
+            raise ValueError("oracle rejected the window")
+        return out, cd, (tuple(tp) if taps else None)
 
     def render_scene_hashed(self, n_limit, chunk_samples: int, n_threads: int):
         lib = load()

@@ -698,3 +698,26 @@ def test_oracle_chunk_invariance():
     ref = o.render_band(0, chunk_samples=1 << 20)[0]
     for chunk in (4096, 777):
         assert o.render_band(0, chunk_samples=chunk)[0].tobytes() == ref.tobytes()
+
+
+def test_oracle_window_equals_band_render():
+    """orc_render_window given the band render's own clock offsets is the
+    band render, bit for bit: the whole band, a window late in it (noise
+    included: the AWGN index is absolute), and the taps of every link at once
+    equal the single-slot taps of orc_render_band."""
+    from lipika_amd import workloads
+    sc = workloads.scene2(duration_s=0.05)
+    o = oracle_scene(sc, 9)
+    w = 1
+    n = o.units[w][1].n
+    ref, _, taps = o.render_band(w, tap_slot=0, chunk_samples=30_000)
+    dt = taps[3]
+    full, _, tp = o.render_window(w, 0, dt, taps=True)
+    assert full.tobytes() == ref.tobytes()
+    i0, m = n - 5000, 4000
+    win, _, _ = o.render_window(w, i0, dt[i0:i0 + m])
+    assert win.tobytes() == ref[i0:i0 + m].tobytes()
+    for slot in range(tp[0].shape[0]):
+        _, _, ts = o.render_band(w, tap_slot=slot, chunk_samples=30_000)
+        for k in range(3):
+            assert tp[k][slot].tobytes() == ts[k].tobytes(), (slot, k)
