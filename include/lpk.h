@@ -428,6 +428,30 @@ int lpk_horizon_gate_stats(lpk_ctx *ctx, uint64_t stats[2]);
  * Synchronous. */
 int lpk_stream_to_files(lpk_ctx *ctx, const lpk_band_chunk *bands, uint32_t n_bands, const char *const *paths,
                         uint64_t chunk_samples, int form);
+/* The IqSink of src/pipeline.rs:372-375 (sink.push at :984) as a C callback:
+ * called on the calling thread with chunk [i0, i0 + n) of band `band` (an
+ * index into the descriptors given), in chunk order per band; iq holds 2 * n
+ * floats (interleaved complex64) valid for the call only. Return 0 to
+ * continue, non-zero to stop that band (ControlFlow::Break). */
+typedef int (*lpk_iq_sink)(void *user, uint32_t band, uint64_t i0, const float *iq, uint64_t n);
+/* generate_band's chunk loop with its sink (src/pipeline.rs:905-990) for
+ * whole bands at once: chunk k of every band renders on the device into
+ * pinned, double-buffered host memory while the sink takes chunk k - 1 of
+ * each band. bands: whole-band descriptors as for lpk_stream_to_files; a
+ * band whose sink returned non-zero renders no further chunk. The bytes
+ * pushed equal those of chunked lpk_render_chunk calls. Synchronous. */
+int lpk_stream_to_sink(lpk_ctx *ctx, const lpk_band_chunk *bands, uint32_t n_bands, uint64_t chunk_samples, int form,
+                       lpk_iq_sink sink, void *user);
+/* The hashing sink of examples/sky_throughput.rs:73-82: user points to one
+ * uint64 FNV-1a state per band (start each at 0xCBF29CE484222325). */
+int lpk_sink_fnv1a(void *user, uint32_t band, uint64_t i0, const float *iq, uint64_t n);
+/* FNV-1a 64 of `bytes` host bytes, continuing from state h. */
+uint64_t lpk_fnv1a64(const void *data, size_t bytes, uint64_t h);
+/* Pinned host memory, for out_host buffers that the device fills
+ * asynchronously (no counterpart in the reference, which renders into
+ * ordinary heap arrays). */
+int lpk_host_alloc(lpk_ctx *ctx, size_t bytes, void **host);
+int lpk_host_free(lpk_ctx *ctx, void *host);
 /* export_truth's geometry (src/pipeline.rs:548-611) for receiver rx over the
  * epoch-relative grid t_rel[n]: range[q * n + k] of every uploaded transmitter
  * q from solve_emission_times with `iterations` iterations (the reference uses

@@ -129,6 +129,10 @@ class LpkError(RuntimeError):
 
 _LIB = None
 
+# lpk_iq_sink: int (*)(void *user, uint32_t band, uint64_t i0, const float *iq, uint64_t n)
+IQ_SINK = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint32, C.c_uint64, C.POINTER(C.c_float), C.c_uint64)
+FNV_OFFSET = 0xCBF29CE484222325
+
 
 def load() -> C.CDLL:
     """Load liblpk.so (raises if it has not been built)."""
@@ -191,6 +195,10 @@ def load() -> C.CDLL:
         "lpk_set_horizon_gate": (_i32, [C.c_void_p, C.c_int]),
         "lpk_horizon_gate_stats": (_i32, [C.c_void_p, P(_u64)]),
         "lpk_stream_to_files": (_i32, [C.c_void_p, P(BandChunk), _u32, P(C.c_char_p), _u64, C.c_int]),
+        "lpk_stream_to_sink": (_i32, [C.c_void_p, P(BandChunk), _u32, _u64, C.c_int, IQ_SINK, C.c_void_p]),
+        "lpk_fnv1a64": (_u64, [C.c_void_p, C.c_size_t, _u64]),
+        "lpk_host_alloc": (_i32, [C.c_void_p, C.c_size_t, P(C.c_void_p)]),
+        "lpk_host_free": (_i32, [C.c_void_p, C.c_void_p]),
         "lpk_truth": (_i32, [C.c_void_p, _u32, P(_d), _u64, _d, C.c_int, P(_d), P(_d)]),
         "lpk_gps_time": (None, [_d, _d, P(_u32), P(_d)]),
         "lpk_toe_snap": (_i32, [_d, _d, P(_d), P(_u32), P(_d)]),
@@ -228,7 +236,8 @@ EXPORTED = (
     "lpk_lws_create", "lpk_lws_destroy", "lpk_lws_get_params", "lpk_lws_set_params", "lpk_lws_generate",
     "lpk_lws_infer", "lpk_lws_gradient", "lpk_lws_train", "lpk_set_warm_start", "lpk_solve_stats",
     "lpk_solve_stats_reset", "lpk_set_horizon_gate", "lpk_horizon_gate_stats",
-    "lpk_stream_to_files", "lpk_truth",
+    "lpk_stream_to_files", "lpk_stream_to_sink", "lpk_sink_fnv1a", "lpk_fnv1a64", "lpk_host_alloc", "lpk_host_free",
+    "lpk_truth",
     "lpk_gps_time", "lpk_toe_snap", "lpk_ephemeris_from_kepler", "lpk_ephemeris_from_state", "lpk_lnav_length",
     "lpk_lnav_bits", "lpk_sv_position", "lpk_grid_fill_broadcast", "lpk_rinex3_gps_ephemeris",
     "lpk_tle_parse", "lpk_sgp4_teme", "lpk_tle_state_ecef", "lpk_grid_fill_tle",

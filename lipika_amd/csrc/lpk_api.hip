@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <new>
 #include <string>
@@ -1681,6 +1682,72 @@ int lpk_solve_stats_reset(lpk_ctx *ctx) {
 
 // ---------------------------------------------------------------- output
 
+}  // extern "C"
+
+namespace {
+
+// generate_band's chunk loop (src/pipeline.rs:905-990) for every band at
+// once, with the sink push of :984: chunk k of every band renders on the
+// device into one of two pinned buffers while the host pushes chunk k - 1 of
+// each band, in band order, from the other. push(b, i0, iq, n) returns false
+// to stop band b (ControlFlow::Break, src/pipeline.rs:372-375); its later
+// chunks are not rendered. Synchronous.
+void stream_bands(lpk_ctx *ctx, const lpk_band_chunk *bands, uint32_t n_bands, uint64_t chunk_samples, int form,
+                  const std::function<bool(uint32_t, uint64_t, const float *, uint64_t)> &push) {
+    struct Pinned {
+        void *p[2] = {nullptr, nullptr};
+        ~Pinned() {
+            for (void *q : p)
+                if (q) (void)hipHostFree(q);
+        }
+    } pin;
+    uint64_t slot_samples = 0, n_chunks = 0;
+    for (uint32_t b = 0; b < n_bands; ++b) {
+        const lpk_band_chunk &B = bands[b];
+        if (B.i0 != 0 || B.out_dev || B.out_host || B.adc_codes_host)
+            throw Fail{LPK_E_INVALID, "band " + std::to_string(b) + ": whole-band descriptor (i0 = 0, no outputs) expected"};
+        slot_samples += std::min<uint64_t>(B.clen, chunk_samples);
+        n_chunks = std::max<uint64_t>(n_chunks, (B.clen + chunk_samples - 1) / chunk_samples);
+    }
+    for (int i = 0; i < 2; ++i)
+        if (slot_samples && hipHostMalloc(&pin.p[i], slot_samples * 8, hipHostMallocDefault) != hipSuccess)
+            throw Fail{LPK_E_NOMEM, "pinned output buffers"};
+    // chunk k of band b lands at off[b] of slot k % 2
+    std::vector<uint64_t> off(n_bands, 0), len_prev(n_bands, 0), len_cur(n_bands, 0);
+    std::vector<char> live(n_bands, 1);
+    for (uint32_t b = 1; b < n_bands; ++b) off[b] = off[b - 1] + std::min<uint64_t>(bands[b - 1].clen, chunk_samples);
+    auto push_slot = [&](int slot, uint64_t k, const std::vector<uint64_t> &len) {
+        const float *base = static_cast<const float *>(pin.p[slot]);
+        for (uint32_t b = 0; b < n_bands; ++b)
+            if (len[b] && live[b] && !push(b, k * chunk_samples, base + 2 * off[b], len[b])) live[b] = 0;
+    };
+    std::vector<lpk_band_chunk> jobs;
+    for (uint64_t k = 0; k < n_chunks; ++k) {
+        const int slot = int(k & 1);
+        jobs.clear();
+        for (uint32_t b = 0; b < n_bands; ++b) {
+            const uint64_t i0 = k * chunk_samples;
+            len_cur[b] = live[b] && i0 < bands[b].clen ? std::min(chunk_samples, bands[b].clen - i0) : 0;
+            if (!len_cur[b]) continue;
+            lpk_band_chunk J = bands[b];
+            J.i0 = i0;
+            J.clen = len_cur[b];
+            J.out_host = static_cast<char *>(pin.p[slot]) + off[b] * 8;
+            if (J.delta_t) J.delta_t += i0;
+            jobs.push_back(J);
+        }
+        if (!jobs.empty()) render(ctx, jobs.data(), uint32_t(jobs.size()), form, nullptr);
+        if (k) push_slot(slot ^ 1, k - 1, len_prev);
+        hip_check(hipStreamSynchronize(ctx->stream), "chunk render");
+        std::swap(len_prev, len_cur);
+    }
+    if (n_chunks) push_slot(int((n_chunks - 1) & 1), n_chunks - 1, len_prev);
+}
+
+}  // namespace
+
+extern "C" {
+
 int lpk_stream_to_files(lpk_ctx *ctx, const lpk_band_chunk *bands, uint32_t n_bands, const char *const *paths,
                         uint64_t chunk_samples, int form) {
     if (!ctx || (n_bands && (!bands || !paths)) || chunk_samples == 0) return LPK_E_INVALID;
@@ -1692,61 +1759,58 @@ int lpk_stream_to_files(lpk_ctx *ctx, const lpk_band_chunk *bands, uint32_t n_ba
                     if (p) std::fclose(p);
             }
         } files;
-        struct Pinned {
-            void *p[2] = {nullptr, nullptr};
-            ~Pinned() {
-                for (void *q : p)
-                    if (q) (void)hipHostFree(q);
-            }
-        } pin;
-        uint64_t slot_samples = 0, n_chunks = 0;
         for (uint32_t b = 0; b < n_bands; ++b) {
-            const lpk_band_chunk &B = bands[b];
-            if (B.i0 != 0 || B.out_dev || B.out_host || B.adc_codes_host || !paths[b])
-                throw Fail{LPK_E_INVALID, "band " + std::to_string(b) +
-                                              ": whole-band descriptor (i0 = 0, no outputs) and a path expected"};
-            slot_samples += std::min<uint64_t>(B.clen, chunk_samples);
-            n_chunks = std::max<uint64_t>(n_chunks, (B.clen + chunk_samples - 1) / chunk_samples);
+            if (!paths[b]) throw Fail{LPK_E_INVALID, "band " + std::to_string(b) + ": no path"};
             FILE *f = std::fopen(paths[b], "wb");
             if (!f) throw Fail{LPK_E_INVALID, std::string("cannot open ") + paths[b] + ": " + std::strerror(errno)};
             files.f.push_back(f);
         }
-        for (int i = 0; i < 2; ++i)
-            if (slot_samples && hipHostMalloc(&pin.p[i], slot_samples * 8, hipHostMallocDefault) != hipSuccess)
-                throw Fail{LPK_E_NOMEM, "pinned output buffers"};
-        // chunk k of band b lands at off[b] of slot k % 2; the host writes
-        // chunk k - 1 while the device renders chunk k
-        std::vector<uint64_t> off(n_bands, 0), len_prev(n_bands, 0), len_cur(n_bands, 0);
-        for (uint32_t b = 1; b < n_bands; ++b) off[b] = off[b - 1] + std::min<uint64_t>(bands[b - 1].clen, chunk_samples);
-        auto write_slot = [&](int slot, const std::vector<uint64_t> &len) {
-            const char *base = static_cast<const char *>(pin.p[slot]);
-            for (uint32_t b = 0; b < n_bands; ++b)
-                if (len[b] && std::fwrite(base + off[b] * 8, 8, len[b], files.f[b]) != len[b])
-                    throw Fail{LPK_E_INVALID, std::string("write failed: ") + paths[b]};
-        };
-        std::vector<lpk_band_chunk> jobs;
-        for (uint64_t k = 0; k < n_chunks; ++k) {
-            const int slot = int(k & 1);
-            jobs.clear();
-            for (uint32_t b = 0; b < n_bands; ++b) {
-                const uint64_t i0 = k * chunk_samples;
-                len_cur[b] = i0 < bands[b].clen ? std::min(chunk_samples, bands[b].clen - i0) : 0;
-                if (!len_cur[b]) continue;
-                lpk_band_chunk J = bands[b];
-                J.i0 = i0;
-                J.clen = len_cur[b];
-                J.out_host = static_cast<char *>(pin.p[slot]) + off[b] * 8;
-                if (J.delta_t) J.delta_t += i0;
-                jobs.push_back(J);
-            }
-            render(ctx, jobs.data(), uint32_t(jobs.size()), form, nullptr);
-            if (k) write_slot(slot ^ 1, len_prev);
-            hip_check(hipStreamSynchronize(ctx->stream), "chunk render");
-            std::swap(len_prev, len_cur);
-        }
-        if (n_chunks) write_slot(int((n_chunks - 1) & 1), len_prev);
+        stream_bands(ctx, bands, n_bands, chunk_samples, form,
+                     [&](uint32_t b, uint64_t, const float *iq, uint64_t n) {
+                         if (std::fwrite(iq, 8, n, files.f[b]) != n)
+                             throw Fail{LPK_E_INVALID, std::string("write failed: ") + paths[b]};
+                         return true;
+                     });
         for (uint32_t b = 0; b < n_bands; ++b)
             if (std::fflush(files.f[b]) != 0) throw Fail{LPK_E_INVALID, std::string("write failed: ") + paths[b]};
+    });
+}
+
+int lpk_stream_to_sink(lpk_ctx *ctx, const lpk_band_chunk *bands, uint32_t n_bands, uint64_t chunk_samples, int form,
+                       lpk_iq_sink sink, void *user) {
+    if (!ctx || (n_bands && !bands) || chunk_samples == 0 || !sink) return LPK_E_INVALID;
+    return guarded(ctx, [&] {
+        stream_bands(ctx, bands, n_bands, chunk_samples, form,
+                     [&](uint32_t b, uint64_t i0, const float *iq, uint64_t n) { return sink(user, b, i0, iq, n) == 0; });
+    });
+}
+
+int lpk_sink_fnv1a(void *user, uint32_t band, uint64_t i0, const float *iq, uint64_t n) {
+    (void)i0;
+    uint64_t *h = static_cast<uint64_t *>(user) + band;
+    *h = lpk_fnv1a64(iq, n * 8, *h);
+    return 0;
+}
+
+uint64_t lpk_fnv1a64(const void *data, size_t bytes, uint64_t h) {
+    const unsigned char *p = static_cast<const unsigned char *>(data);
+    for (size_t i = 0; i < bytes; ++i) h = (h ^ p[i]) * 0x100000001B3ULL;
+    return h;
+}
+
+int lpk_host_alloc(lpk_ctx *ctx, size_t bytes, void **host) {
+    if (!ctx || !host) return LPK_E_INVALID;
+    return guarded(ctx, [&] {
+        *host = nullptr;
+        if (hipHostMalloc(host, std::max<size_t>(bytes, 1), hipHostMallocDefault) != hipSuccess)
+            throw Fail{LPK_E_NOMEM, "pinned host allocation of " + std::to_string(bytes) + " bytes failed"};
+    });
+}
+
+int lpk_host_free(lpk_ctx *ctx, void *host) {
+    if (!ctx) return LPK_E_INVALID;
+    return guarded(ctx, [&] {
+        if (host) hip_check(hipHostFree(host), "pinned host free");
     });
 }
 

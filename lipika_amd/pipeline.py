@@ -624,6 +624,36 @@ class Backend:
         self.check(self.lib.lpk_horizon_gate_stats(self.h, a))
         return dict(skipped=a[0], tested=a[1])
 
+    def stream_to_sink(self, bands: Sequence, chunk_samples: int, form: int = _lib.FORM_PLAIN,
+                       sink: Optional[Callable[[int, int, np.ndarray], object]] = None) -> list:
+        """generate_band's chunk loop with its sink (src/pipeline.rs:905-990,
+        :984) for whole-band descriptors, through lpk_stream_to_sink: chunk k
+        of every band renders on the device while the sink takes chunk k - 1.
+        `sink(band, i0, iq)` gets a complex64 view valid for the call only and
+        returns a truthy value to stop that band. With no sink, the library's
+        FNV-1a sink hashes each band; returns the per-band hashes."""
+        arr = (_lib.BandChunk * max(len(bands), 1))(*bands)
+        if sink is None:
+            hashes = (C.c_uint64 * max(len(bands), 1))(*([_lib.FNV_OFFSET] * len(bands)))
+            fn = _lib.IQ_SINK(C.cast(self.lib.lpk_sink_fnv1a, C.c_void_p).value)
+            self.check(self.lib.lpk_stream_to_sink(self.h, arr, len(bands), chunk_samples, form, fn, C.cast(hashes, C.c_void_p)))
+            return list(hashes)[:len(bands)]
+        err = []
+
+        def trampoline(user, band, i0, iq, n):
+            try:
+                view = np.ctypeslib.as_array(iq, shape=(2 * n,)).view(np.complex64)
+                return 1 if sink(int(band), int(i0), view) else 0
+            except BaseException as e:  # no exception crosses the C ABI
+                err.append(e)
+                return 1
+
+        fn = _lib.IQ_SINK(trampoline)
+        self.check(self.lib.lpk_stream_to_sink(self.h, arr, len(bands), chunk_samples, form, fn, None))
+        if err:
+            raise err[0]
+        return []
+
     def taps(self, job, slot: int, form: int = _lib.FORM_PLAIN):
         n = job.clen
         te, rg, ph, dt = (np.empty(n, dtype=np.float64) for _ in range(4))

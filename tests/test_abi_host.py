@@ -297,3 +297,15 @@ def test_workload_totals():
     p3 = prepare(workloads.scene3(duration_s=1.0))
     assert len(p3.transmitters) == 32 and len(p3.receivers) == 16
     assert all(len(b.active) == 32 for r in p3.receivers for b in r.bands)
+
+
+def test_fnv1a64_known_answers(lpk):
+    """FNV-1a 64 (Fowler, Noll, Vo; the hashing sink of
+    examples/sky_throughput.rs:73-82) on its published test strings."""
+    from lipika_amd import _lib
+    for text, want in ((b"", 0xCBF29CE484222325), (b"a", 0xAF63DC4C8601EC8C), (b"foobar", 0x85944171F73967E8)):
+        buf = C.create_string_buffer(text, len(text))
+        assert lpk.lpk_fnv1a64(buf, len(text), _lib.FNV_OFFSET) == want, text
+    # continuing from a state is hashing the concatenation
+    a, b = C.create_string_buffer(b"foo", 3), C.create_string_buffer(b"bar", 3)
+    assert lpk.lpk_fnv1a64(b, 3, lpk.lpk_fnv1a64(a, 3, _lib.FNV_OFFSET)) == 0x85944171F73967E8

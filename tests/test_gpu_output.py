@@ -74,3 +74,53 @@ def test_truth_export_is_the_four_iteration_solve(backend, tmp_path, orc):
                               epoch + t_rel[7], src)
         assert [tx["source_x_m"][7], tx["source_y_m"][7], tx["source_z_m"][7]] == list(src[:])
     _say(f"truth: {len(truth['transmitters'])} transmitters x 21 grid times equal the oracle's 4-iteration solve")
+
+
+def test_sink_stream_equals_chunked_renders_and_stops_per_band(backend):
+    """lpk_stream_to_sink (generate_band's chunk loop with IqSink::push,
+    src/pipeline.rs:905-990, :984): every band's chunks arrive in order and
+    equal eager renders of the same chunks; the library's FNV-1a sink hashes
+    exactly those bytes; a sink that returns Break stops its own band only."""
+    from lipika_amd import _lib
+    scene = workloads.scene2(duration_s=0.3)
+    sc = prepare(scene)
+    seed = 77
+    backend.upload(sc, seed)
+    units = [(r, b) for r, rx in enumerate(sc.receivers) for b, _ in enumerate(rx.bands)]
+    chunk = 100_000
+    whole, keep = [], []
+    for w, (r, b) in enumerate(units):
+        j, k = backend.job(r, b, w, seed, 0, sc.receivers[r].bands[b].n)
+        whole.append(j)
+        keep.append(k)
+    got = {w: [] for w in range(len(units))}
+
+    def sink(band, i0, iq):
+        got[band].append((i0, iq.copy()))
+        return False
+
+    backend.stream_to_sink(whole, chunk, _lib.FORM_PRECISION, sink)
+    lib = _lib.load()
+    for w, (r, b) in enumerate(units):
+        n = sc.receivers[r].bands[b].n
+        want = np.empty(n, dtype=np.complex64)
+        for i0 in range(0, n, chunk):
+            j, k = backend.job(r, b, w, seed, i0, min(chunk, n - i0), out_host=want[i0:])
+            backend.render([j], _lib.FORM_PRECISION)
+        backend.sync()
+        assert [i0 for i0, _ in got[w]] == list(range(0, n, chunk))
+        assert np.concatenate([c for _, c in got[w]]).tobytes() == want.tobytes()
+        got[w] = want
+    hashes = backend.stream_to_sink(whole, chunk, _lib.FORM_PRECISION)
+    for w in range(len(units)):
+        assert hashes[w] == lib.lpk_fnv1a64(got[w].ctypes.data, got[w].nbytes, _lib.FNV_OFFSET)
+    seen = {w: 0 for w in range(len(units))}
+
+    def stopper(band, i0, iq):
+        seen[band] += 1
+        return band == 1 and seen[band] == 2   # Break after the L1 band's second chunk
+
+    backend.stream_to_sink(whole, chunk, _lib.FORM_PLAIN, stopper)
+    assert seen[1] == 2
+    assert seen[0] == -(-sc.receivers[0].bands[0].n // chunk)
+    _say(f"sink stream: {[len(v) for v in got.values()]} samples per band; hashes {[hex(h) for h in hashes]}")
