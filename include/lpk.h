@@ -138,7 +138,7 @@ typedef struct {
 
 typedef struct {
     int32_t kind;                /* lpk_post_kind */
-    int32_t bits;                /* ADC bits */
+    int32_t bits;                /* ADC bits, 0..64 (src/rx/mod.rs:93-97; 2 below 2); at most 16 with adc_codes_host */
     double value;                /* GAIN: dB; NONLINEARITY: iip3 W; ADC: full-scale V */
 } lpk_post_op;
 

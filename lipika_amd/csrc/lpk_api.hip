@@ -602,8 +602,10 @@ bool job_window(const lpk_ctx *ctx, const lpk_band_chunk &B, uint64_t i0, uint64
     const double xa = (double(i0) / B.fs_hz) * M;
     const double xb = (double(i0 + clen - 1) / B.fs_hz) * M;
     if (!(xb < 4.0e15)) throw Fail{LPK_E_INVALID, "clock index range too large"};
-    klo = xa <= 0.0 ? 0 : int64_t(std::floor(xa));
-    khi = xb <= 0.0 ? 0 : int64_t(std::floor(xb)) + 1;
+    // one index of margin on each side: the precision form's sample times may
+    // sit an ulp from the division's
+    klo = xa <= 1.0 ? 0 : int64_t(std::floor(xa)) - 1;
+    khi = xb <= 0.0 ? 1 : int64_t(std::floor(xb)) + 2;
     return true;
 }
 
@@ -623,7 +625,11 @@ void validate_job(const lpk_ctx *ctx, const lpk_band_chunk &B, uint32_t q) {
     for (uint32_t p = 0; p < B.n_post; ++p) {
         const lpk_post_op &P = B.post[p];
         if (P.kind < LPK_POST_GAIN || P.kind > LPK_POST_ADC) throw bad(LPK_E_INVALID, "unknown post op");
-        if (P.kind == LPK_POST_ADC && (P.bits < 0 || P.bits > 16)) throw bad(LPK_E_INVALID, "ADC bits must be 0..16");
+        // src/rx/mod.rs:93-97 takes any u32 bits with levels = (1u64 << (bits.max(2) - 1)) - 1, which
+        // stays defined up to 64 bits; int16 codes need at most 16
+        if (P.kind == LPK_POST_ADC && (P.bits < 0 || P.bits > 64)) throw bad(LPK_E_INVALID, "ADC bits must be 0..64");
+        if (P.kind == LPK_POST_ADC && B.adc_codes_host && P.bits > 16)
+            throw bad(LPK_E_INVALID, "ADC codes are int16: at most 16 bits when adc_codes_host is set");
         if (!is_fin(P.value)) throw bad(LPK_E_NONFINITE, "post op parameter");
     }
     if (B.delta_t)
@@ -641,6 +647,14 @@ void check_form_links(int form, const lpk_band_chunk &B, uint32_t q) {
         throw Fail{LPK_E_INVALID, "job " + std::to_string(q) + ": the precision form takes at most " +
                                       std::to_string(kPrecisionMaxLinks) + " transmitters per band-chunk (" +
                                       std::to_string(B.n_active) + " given); use the plain form"};
+}
+
+// The window-relative clock lookups of a job whose window starts at
+// J.eps_k0 and reaches master index k_hi: used when every index fits 31 bits.
+void set_walk_rel(JobDev &J, const double *base, int64_t k_hi) {
+    J.eps_k0d = double(J.eps_k0);
+    J.walk_base_w = base + (J.eps_k0 >> kWalkBlockShift);
+    J.walk_rel = (k_hi - J.eps_k0 < (int64_t(1) << 31) - 2) ? 1 : 0;
 }
 
 // The job-table record of samples [i0, i0 + clen) of a band and its links
@@ -679,6 +693,7 @@ JobDev fill_job(const lpk_ctx *ctx, const lpk_band_chunk &B, uint64_t i0, uint64
         J.ant_cutoff = B.antenna_cutoff_deg;
     }
     J.fs = B.fs_hz;
+    J.inv_fs = 1.0 / B.fs_hz;
     J.lo = B.lo_hz;
     J.epoch = B.epoch_unix;
     J.i0 = i0;
@@ -846,6 +861,7 @@ void render(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form,
             J.walk_toff = static_cast<const double *>(R.toff.p);
             J.walk_local = static_cast<const float *>(R.local.p);
             J.eps_k0 = (wlo[B.rx] >> kWalkBlockShift) << kWalkBlockShift;
+            set_walk_rel(J, R.base, whi[B.rx]);
         }
         if (B.out_dev) {
             J.out = static_cast<float *>(B.out_dev);
@@ -947,7 +963,7 @@ void plan_fill(lpk_plan *P, uint64_t k, char *dst) {
     CarrySegDev *ch = reinterpret_cast<CarrySegDev *>(dst + P->off_carry);
     double *bs = static_cast<double *>(P->blocksum.p);
     // clock walks
-    std::vector<int64_t> k0(P->walk_rx.size(), 0);
+    std::vector<int64_t> k0(P->walk_rx.size(), 0), k1(P->walk_rx.size(), 0);
     uint64_t blocks = 0;
     for (size_t w = 0; w < P->walk_rx.size(); ++w) {
         RxState &R = ctx->rx[P->walk_rx[w]];
@@ -971,6 +987,7 @@ void plan_fill(lpk_plan *P, uint64_t k, char *dst) {
             S.win_block = b_lo;
             S.n_blocks = uint32_t(b_hi + 1 - first);
             k0[w] = b_lo << kWalkBlockShift;
+            k1[w] = hi;
             if (R.nb_known <= b_hi) {
                 C.first = R.nb_known;
                 C.n = b_hi + 1 - R.nb_known;
@@ -1004,6 +1021,7 @@ void plan_fill(lpk_plan *P, uint64_t k, char *dst) {
             J.walk_toff = static_cast<const double *>(P->toff[w].p);
             J.walk_local = static_cast<const float *>(P->local[w].p);
             J.eps_k0 = k0[w];
+            set_walk_rel(J, ctx->rx[B.rx].base, k1[w]);
         }
         J.out = B.out_dev ? static_cast<float *>(B.out_dev) + 2 * i0 : static_cast<float *>(P->outbuf[b].p);
         *jh++ = J;
@@ -1388,9 +1406,31 @@ int lpk_autotune(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int 
                  double *best_ms) {
     if (!ctx || !best || !best_ms) return LPK_E_INVALID;
     return guarded(ctx, [&] {
-        for (uint32_t q = 0; q < n_jobs; ++q)
-            if (jobs && (jobs[q].out_host || jobs[q].adc_codes_host || jobs[q].unoccluded))
+        if (n_jobs && !jobs) throw Fail{LPK_E_INVALID, "jobs is NULL"};
+        if (!ctx->have_scene) throw Fail{LPK_E_STATE, "no scene uploaded"};
+        if (form < LPK_FORM_PLAIN || form > LPK_FORM_LEARNED) throw Fail{LPK_E_INVALID, "unknown form"};
+        // the job set is checked once, before any candidate runs
+        uint64_t samples = 0;
+        for (uint32_t q = 0; q < n_jobs; ++q) {
+            if (jobs[q].out_host || jobs[q].adc_codes_host || jobs[q].unoccluded)
                 throw Fail{LPK_E_INVALID, "autotune jobs keep their output in device memory"};
+            validate_job(ctx, jobs[q], q);
+            check_form_links(form, jobs[q], q);
+            samples += jobs[q].clen;
+        }
+        if (!samples) throw Fail{LPK_E_INVALID, "autotune needs at least one job with samples"};
+        // a failure part-way leaves the context's tuning as it was
+        struct Restore {
+            lpk_ctx *c;
+            int ts, xcd;
+            bool keep = false;
+            ~Restore() {
+                if (!keep) {
+                    c->tile_shift = ts;
+                    c->xcd_map = xcd;
+                }
+            }
+        } restore{ctx, ctx->tile_shift, ctx->xcd_map};
         const int shifts[3] = {7, 8, 9};
         double best_t = 1e300;
         lpk_tuning pick{ctx->tile_shift, ctx->xcd_map};
@@ -1410,6 +1450,7 @@ int lpk_autotune(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int 
             }
         ctx->tile_shift = pick.tile_shift;
         ctx->xcd_map = pick.xcd_map;
+        restore.keep = true;
         *best = pick;
         *best_ms = best_t;
     });

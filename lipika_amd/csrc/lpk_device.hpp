@@ -18,6 +18,7 @@ constexpr double kTau = 6.283185307179586;              // std::f64::consts::TAU
 constexpr double kCaChipRate = 1023000.0;               // src/signal/ca_code.rs:23
 constexpr int kCaCodeLength = 1023;                     // src/signal/ca_code.rs:24
 constexpr double kNavBitRate = 50.0;                    // src/signal/ca_code.rs:27
+constexpr double kChipsPerNavBit = kCaChipRate / kNavBitRate;   // 20 460, exact
 constexpr int64_t kNavSubframeBits = 300;               // src/signal/ca_code.rs:29
 constexpr int kNewtonIterations = 2;                    // src/pipeline.rs:56
 
@@ -210,6 +211,15 @@ struct JobDev {
     // (the last clamped to the job's end)
     uint64_t pred0;
     uint32_t pred_qs, pred_nb;
+    // 1 / fs: the precision form forms its sample times by a reciprocal and
+    // one correction step instead of a correctly rounded division
+    double inv_fs;
+    // walk_rel = 1: the clock window spans fewer than 2^31 master indices, so
+    // its lookups take 32-bit indices relative to eps_k0 (as a double, exact),
+    // with walk_base_w = walk_base + eps_k0 / 65 536 (the window's first block)
+    const double *walk_base_w;
+    double eps_k0d;
+    int32_t walk_rel, _wpad;
 };
 
 // High 64 bits of a 64 x 64-bit product, and count of leading zeros, on the
@@ -281,65 +291,56 @@ __host__ __device__ __forceinline__ void philox4x64(uint64_t c[4], uint64_t k0, 
     }
 }
 
-// Box-Muller pair from one 64-bit Philox word: u1 = (m1 + 1/2) 2^-24 from bits
-// 63..40 and the angle u2 = m2 2^-24 turns from bits 39..16. The pair is
-// evaluated in f64 with correctly rounded operations only (+ - * / sqrt, no
-// FMA, no libm), so every IEEE-754 machine yields the same bits; each value is
-// then rounded once to f32.
-//   ln u1: u1 = n 2^-25 with n = 2 m1 + 1 odd; n = f 2^e with f in
-//   [sqrt(1/2), sqrt(2)] split exactly; ln f = 2 atanh(s), s = (f-1)/(f+1),
-//   |s| <= 0.1716, by its series through s^13 (truncation < 2e-13 relative).
+// Box-Muller pair from one 64-bit Philox word: u1 = n 2^-24 with n = 2 m1 + 1
+// odd from bits 63..41 (24 bits, exact in f32), and the angle u2 = m2 2^-24
+// turns from bits 39..16. The pair is evaluated in f32 with correctly rounded
+// operations only (+ - * / sqrt and explicit fmaf, no contraction, no libm
+// transcendental), so every IEEE-754 machine yields the same bits.
+//   ln u1: n = f 2^e with f in [sqrt(1/2), sqrt(2)] split exactly (u1 near 1
+//   keeps its relative accuracy); ln f = 2 atanh(s), s = (f-1)/(f+1),
+//   |s| <= 0.1716, by its series through s^9 (truncation < 1e-9 relative).
 //   sin/cos of 2 pi u2: the quadrant (top 2 bits) and the reflection about
 //   pi/4 are exact integer steps; sin and cos of the remaining angle
-//   (<= pi/4) come from their Taylor series through the 13th and 14th powers.
+//   (<= pi/4) come from their Taylor series through the 9th and 10th powers.
 __host__ __device__ __forceinline__ void box_muller(uint64_t w, float &zc, float &zs) {
-    const uint32_t n = (static_cast<uint32_t>(w >> 40) << 1) | 1u;
+    const uint32_t n = (static_cast<uint32_t>(w >> 41) << 1) | 1u;
     int e = 31 - clz32(n);
-    double f = ldexp(static_cast<double>(n), -e);
-    if (f > 1.4142135623730951) {
-        f = f * 0.5;
+    float f = ldexpf(static_cast<float>(n), -e);
+    if (f > 1.41421356f) {
+        f = f * 0.5f;
         e += 1;
     }
-    const double s = (f - 1.0) / (f + 1.0);
-    const double z = s * s;
-    double p = 1.0 / 13.0;
-    p = p * z + 1.0 / 11.0;
-    p = p * z + 1.0 / 9.0;
-    p = p * z + 1.0 / 7.0;
-    p = p * z + 1.0 / 5.0;
-    p = p * z + 1.0 / 3.0;
-    p = p * z + 1.0;
-    const double ln_u1 = static_cast<double>(e - 25) * 0.6931471805599453 + (2.0 * s) * p;
-    const double r = sqrt(-2.0 * ln_u1);
+    const float s = (f - 1.0f) / (f + 1.0f);
+    const float z = s * s;
+    float p = fmaf(z, 1.0f / 9.0f, 1.0f / 7.0f);
+    p = fmaf(p, z, 1.0f / 5.0f);
+    p = fmaf(p, z, 1.0f / 3.0f);
+    p = fmaf(p, z, 1.0f);
+    const float ln_u1 = fmaf(static_cast<float>(e - 24), 0.693147182f, (2.0f * s) * p);
+    const float r = sqrtf(-2.0f * ln_u1);
 
     const uint32_t m2 = static_cast<uint32_t>(w >> 16) & 0xFFFFFFu;
     const uint32_t quad = m2 >> 22, v = m2 & 0x3FFFFFu;
     const bool flip = v > 0x200000u;
-    const double th = static_cast<double>(flip ? 0x400000u - v : v) * (1.5707963267948966 * 0x1.0p-22);
-    const double t2 = th * th;
-    double ps = 1.0 / 6227020800.0;
-    ps = ps * t2 - 1.0 / 39916800.0;
-    ps = ps * t2 + 1.0 / 362880.0;
-    ps = ps * t2 - 1.0 / 5040.0;
-    ps = ps * t2 + 1.0 / 120.0;
-    ps = ps * t2 - 1.0 / 6.0;
-    double pc = -1.0 / 87178291200.0;
-    pc = pc * t2 + 1.0 / 479001600.0;
-    pc = pc * t2 - 1.0 / 3628800.0;
-    pc = pc * t2 + 1.0 / 40320.0;
-    pc = pc * t2 - 1.0 / 720.0;
-    pc = pc * t2 + 1.0 / 24.0;
-    pc = pc * t2 - 0.5;
-    const double sn = th + (th * t2) * ps;
-    const double cs = 1.0 + t2 * pc;
-    const double a = flip ? cs : sn, b = flip ? sn : cs;  // sin, cos within the quadrant
-    double sv, cv;
+    const float th = static_cast<float>(flip ? 0x400000u - v : v) * (1.57079632679489662f * 0x1.0p-22f);
+    const float t2 = th * th;
+    float ps = fmaf(t2, 1.0f / 362880.0f, -1.0f / 5040.0f);
+    ps = fmaf(ps, t2, 1.0f / 120.0f);
+    ps = fmaf(ps, t2, -1.0f / 6.0f);
+    float pc = fmaf(t2, -1.0f / 3628800.0f, 1.0f / 40320.0f);
+    pc = fmaf(pc, t2, -1.0f / 720.0f);
+    pc = fmaf(pc, t2, 1.0f / 24.0f);
+    pc = fmaf(pc, t2, -0.5f);
+    const float sn = fmaf(th * t2, ps, th);
+    const float cs = fmaf(t2, pc, 1.0f);
+    const float a = flip ? cs : sn, b = flip ? sn : cs;  // sin, cos within the quadrant
+    float sv, cv;
     if (quad == 0) { sv = a; cv = b; }
     else if (quad == 1) { sv = b; cv = -a; }
     else if (quad == 2) { sv = -a; cv = -b; }
     else { sv = -b; cv = a; }
-    zc = static_cast<float>(r * cv);
-    zs = static_cast<float>(r * sv);
+    zc = r * cv;
+    zs = r * sv;
 }
 
 }  // namespace lpk

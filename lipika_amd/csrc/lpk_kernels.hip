@@ -373,6 +373,14 @@ __device__ __forceinline__ double walk_eps(const JobDev &J, int64_t k) {
            static_cast<double>(J.walk_local[rel]);
 }
 
+// epsilon(eps_k0 + k) of the materialised walk window by its window-relative
+// index k: the same three terms summed in the same order as walk_eps, with
+// 32-bit index arithmetic.
+__device__ __forceinline__ double walk_eps_rel(const JobDev &J, int32_t k) {
+    return J.walk_base_w[k >> kWalkBlockShift] + J.walk_toff[k >> kWalkSubShift] +
+           static_cast<double>(J.walk_local[k]);
+}
+
 // ClockStream::delta_t_at (src/clock/mod.rs:329-346); clock start epoch 0.0.
 __device__ __forceinline__ double clock_delta(const JobDev &J, double t) {
     const double elapsed = t - 0.0;
@@ -390,18 +398,38 @@ __device__ __forceinline__ double clock_delta(const JobDev &J, double t) {
         const double e0 = J.walk ? walk_eps(J, 0) : 0.0;
         return (e0 + flick) + (J.c0 + J.y0 * 0.0);
     }
-    const int64_t k0 = static_cast<int64_t>(floor(x));
-    const double frac = x - static_cast<double>(k0);
-    double elo = 0.0, ehi = 0.0;
-    if (J.walk) {
-        elo = walk_eps(J, k0);
-        ehi = walk_eps(J, k0 + 1);
+    double frac, elo = 0.0, ehi = 0.0;
+    if (J.walk_rel) {
+        // x - eps_k0 is exact (eps_k0 is an integer no greater than x), so the
+        // floor and the fraction are those of x itself
+        const double rx = x - J.eps_k0d;
+        const double fl = floor(rx);
+        frac = rx - fl;
+        const int32_t k = static_cast<int32_t>(fl);
+        elo = walk_eps_rel(J, k);
+        ehi = walk_eps_rel(J, k + 1);
+    } else {
+        const double fl = floor(x);
+        frac = x - fl;
+        if (J.walk) {
+            const int64_t k0 = static_cast<int64_t>(fl);
+            elo = walk_eps(J, k0);
+            ehi = walk_eps(J, k0 + 1);
+        }
     }
     const double eps = elo * (1.0 - frac) + ehi * frac;
     return (eps + flick) + (J.c0 + J.y0 * elapsed);
 }
 
 // ---------------------------------------------------------------- render
+
+// Sample time i / fs of the precision form: the product with the rounded
+// reciprocal and one correction step, which rounds as the division does except
+// in rare cases one ulp away (3.6e-15 s at 30 s).
+__device__ __forceinline__ double sample_time(double i, double fs, double inv_fs) {
+    const double q = i * inv_fs;
+    return fma(fma(-q, fs, i), inv_fs, q);
+}
 
 struct TapOut {
     double *t_emit, *range, *phase, *delta_t;
@@ -429,7 +457,8 @@ struct alignas(16) Anchor {
             double ph0;            // carrier phase at the middle sample, turns in [-1/2, 1/2]
             double ph_r, ph_d, ph_t;  // its derivatives in turns per metre of range, second of delta_t and of t_local
             double epoch_rel, rate;   // signal epoch; symbol rate (framed) or omega (CW)
-            double hu, hu1;        // symbol (framed) or NAV bit (C/A) index near the tile's first sample, and the next
+            double hu, hu1;        // framed: the symbol index the tile starts in, and the next; C/A: the chip
+                                   // offset from cbase where the tile's second NAV bit starts, and the first bit
             double cbase;          // C/A: a chip count just before the tile's first sample
             const float *chip0;    // C/A: the doubled f32 chip table from residue cmod on
         };
@@ -499,7 +528,7 @@ __device__ __forceinline__ void solve_smooth(const TxDev &T, const double *obs, 
             sy = T.pos[1];
             sz = T.pos[2];
         } else {
-            const double f = (eoff + t) / T.grid_dt;
+            const double f = (eoff + t) * T.grid_inv_dt;
             const double flo = fmin(fmax(floor(f), 0.0), static_cast<double>(T.grid_n - 2));
             const int64_t lo = static_cast<int64_t>(flo);
             double frac = f - flo;
@@ -511,7 +540,7 @@ __device__ __forceinline__ void solve_smooth(const TxDev &T, const double *obs, 
             sx = fma(bx - ax, frac, ax);
             sy = fma(by - ay, frac, ay);
             sz = fma(bz - az, frac, az);
-            const double idt = 1.0 / T.grid_dt;
+            const double idt = T.grid_inv_dt;
             vx = (bx - ax) * idt;
             vy = (by - ay) * idt;
             vz = (bz - az) * idt;
@@ -522,7 +551,7 @@ __device__ __forceinline__ void solve_smooth(const TxDev &T, const double *obs, 
         const double dy = fma(-w, sx, sy) - obs[1];
         const double dz = sz - obs[2];
         const double r = sqrt(fma(dx, dx, fma(dy, dy, dz * dz)));
-        const double ir = 1.0 / r;
+        const double ir = __builtin_amdgcn_rcp(r);   // for the range rates only
         // d/d(query time) of the rotated position: V + w J V; of the Sagnac
         // angle: -omega J S, with J (x, y, z) = (y, -x, 0)
         const double pq = (dx * fma(w, vy, vx) + dy * fma(-w, vx, vy) + dz * vz) * ir;
@@ -599,19 +628,24 @@ __device__ __forceinline__ void precision_link(const Anchor &A, bool smooth, con
         hinted[q] = true;
         if (G == kGroupCa) {
             const double dt = t_emit[q] - A.epoch_rel;
-            sig_b[q] = floor(dt * kNavBitRate);
-            const double sig_k = floor(dt * kCaChipRate);
-            // the chip at residue cmod + d of the doubled table when the count is
-            // within one code period of the tile's base (NaN fails the test), else
-            // at the count's own residue
-            const double d = sig_k - A.cbase;
-            int32_t off = static_cast<int32_t>(d);
-            if (!(d >= 0.0 && d < static_cast<double>(kCaCodeLength)))
-                off = static_cast<int32_t>(rem_euclid(floor_to_i64(sig_k), kCaCodeLength)) -
+            // chips past the tile's base: fl(dt R) - cbase, exact while cbase is at
+            // least half the count (Sterbenz), so its floor is floor(dt R) - cbase
+            const double d = dt * kCaChipRate - A.cbase;
+            // within one code period of the base (NaN fails the test): residue
+            // cmod + d of the doubled table, and the NAV bit from the chip count
+            // (bit b spans chips [20 460 b, 20 460 (b + 1)); hu holds the offset
+            // from cbase where the tile's second bit starts); else the count's
+            // own residue and the bit index floor(dt 50)
+            int32_t off;
+            if (d >= 0.0 && d < static_cast<double>(kCaCodeLength))
+                off = static_cast<int32_t>(d);
+            else
+                off = static_cast<int32_t>(rem_euclid(floor_to_i64(floor(dt * kCaChipRate)), kCaCodeLength)) -
                       static_cast<int32_t>(A.cmod);
             chip[q] = ((gchip *)A.chip0)[off];
-            bre[q] = sig_b[q] == A.hu ? A.h0re : A.h1re;
-            hinted[q] = sig_b[q] == A.hu || sig_b[q] == A.hu1;
+            hinted[q] = d >= 0.0 && d < A.hu + kChipsPerNavBit;
+            if (!hinted[q]) sig_b[q] = floor(dt * kNavBitRate);
+            bre[q] = d >= A.hu ? A.h1re : A.h0re;
         } else if (G == kGroupFramed) {
             sig_b[q] = floor((t_emit[q] - A.epoch_rel) * A.rate);
             const bool h0 = sig_b[q] == A.hu;
@@ -619,7 +653,8 @@ __device__ __forceinline__ void precision_link(const Anchor &A, bool smooth, con
             bim[q] = h0 ? A.h0im : A.h1im;
             hinted[q] = h0 || sig_b[q] == A.hu1;
         } else if (G == kGroupCw) {
-            ph_cw[q] = static_cast<double>(static_cast<float>(A.rate * t_emit[q]));  // rate holds omega
+            if (A.rate != 0.0)   // rate holds omega (uniform): a tone at the carrier has no phase of its own
+                ph_cw[q] = static_cast<double>(static_cast<float>(A.rate * t_emit[q]));
             bre[q] = 1.0f;
         } else {
             const TxDev &T = txs[A.tx];
@@ -639,7 +674,7 @@ __device__ __forceinline__ void precision_link(const Anchor &A, bool smooth, con
         // and the hardware sine and cosine of that fraction
         const double dr = r[q] - A.r1_mid;
         double turns = fma(A.ph_r, dr, fma(A.ph_d, ddl[q], fma(A.ph_t, dtl[q], A.ph0)));
-        if (G == kGroupCw || G == kGroupGeneric) turns = fma(ph_cw[q], kInvTau, turns);
+        if (G == kGroupGeneric || (G == kGroupCw && A.rate != 0.0)) turns = fma(ph_cw[q], kInvTau, turns);
         const float fr = static_cast<float>(__builtin_amdgcn_fract(turns));
         const float spf = __builtin_amdgcn_sinf(fr), cpf = __builtin_amdgcn_cosf(fr);
         // amp_factor / r to first order about the middle sample's range
@@ -770,17 +805,21 @@ __device__ __forceinline__ bool anchor_fit(Anchor &A, const LinkDev &L, const Tx
         framed_value(T, A.hu, A.h0re, A.h0im);
         framed_value(T, A.hu1, A.h1re, A.h1im);
     } else if (T.signal_kind == 4) {
-        A.hu = floor((te0 - T.epoch_rel) * kNavBitRate);
-        A.hu1 = A.hu + 1.0;
-        A.h0re = static_cast<float>(nav_pm(T, A.hu));
-        A.h1re = static_cast<float>(nav_pm(T, A.hu1));
+        // chip counts and NAV bits (src/signal/ca_code.rs:167-190): 1.023e6 / 50
+        // = 20 460 chips per bit, so bit b spans chip counts [20 460 b, 20 460 (b + 1))
         const double cb = floor((te0 - T.epoch_rel) * kCaChipRate) - 1.0;
-        if (fabs(cb) < 4503599627370496.0) {   // 2^52: d = k - cbase stays exact
+        if (fabs(cb) < 4503599627370496.0) {   // 2^52: counts stay exact
             A.cbase = cb;
             A.cmod = static_cast<uint32_t>(rem_euclid(floor_to_i64(cb), kCaCodeLength));
             A.chip0 = T.ca_chips2 + A.cmod;
+            const double b_lo = floor(cb / kChipsPerNavBit);
+            A.hu = fma(b_lo + 1.0, kChipsPerNavBit, -cb);   // in [1, 20 460]: exact
+            A.hu1 = b_lo;
+            A.h0re = static_cast<float>(nav_pm(T, b_lo));
+            A.h1re = static_cast<float>(nav_pm(T, b_lo + 1.0));
         } else {
             A.cbase = __builtin_nan("");      // every lane takes the general index
+            A.hu = 0.0;
         }
     }
     return smooth;
@@ -883,7 +922,7 @@ __device__ __forceinline__ void precision_span(const JobDev &J, uint32_t span, c
                 const uint64_t first = J.i0 > t0 ? J.i0 : t0, end = jend < t0 + kTile ? jend : t0 + kTile;
                 if (first < end) {
                     const uint64_t ia = a == 0 ? first : (a == 1 ? (first + end - 1) >> 1 : end - 1);
-                    const double tl = static_cast<double>(ia) / J.fs;
+                    const double tl = sample_time(static_cast<double>(ia), J.fs, J.inv_fs);
                     const double dl = J.delta_t ? J.delta_t[ia - J.i0] : clock_delta(J, tl);
                     s_at[kk][a][0] = tl - dl;
                     s_at[kk][a][1] = tl;
@@ -938,7 +977,7 @@ __device__ __forceinline__ void precision_span(const JobDev &J, uint32_t span, c
                     i[q] = t0 + kSpanSamples * threadIdx.x + q;
                     valid[q] = i[q] >= J.i0 && i[q] < jend;
                     const uint64_t ic = valid[q] ? i[q] : first;
-                    t_local[q] = static_cast<double>(ic) / J.fs;
+                    t_local[q] = sample_time(static_cast<double>(ic), J.fs, J.inv_fs);
                     dlt[q] = J.delta_t ? J.delta_t[ic - J.i0] : clock_delta(J, t_local[q]);
                     tu[q] = t_local[q] - dlt[q];
                     x[q] = tu[q] - s_at[kk][1][0];
@@ -1201,7 +1240,13 @@ __global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM == 1 ? 4 : 6))
             // one LDS add per wave and counter, one global add per workgroup
             if (threadIdx.x < 4) s_stats[threadIdx.x] = 0;
             __syncthreads();
-            const uint64_t contrib = static_cast<uint64_t>(__popcll(__ballot(valid))) * J.n_active;
+            // links the horizon gate skipped for this tile do no solve: count only the solved ones
+            uint32_t n_solved = J.n_active;
+            if (ANT && gating) {
+                const uint32_t ng = J.n_active < kGateMaxLinks ? J.n_active : kGateMaxLinks;
+                for (uint32_t w = 0; w < (ng + 31) / 32; ++w) n_solved -= __popc(s_gated[w]);
+            }
+            const uint64_t contrib = static_cast<uint64_t>(__popcll(__ballot(valid))) * n_solved;
             const uint64_t evals = (prow ? 2u : 3u) * contrib + static_cast<uint64_t>(w_extra);
             if ((threadIdx.x & 63) == 0) {
                 if (evals) atomicAdd(&s_stats[0], static_cast<unsigned long long>(evals));

@@ -557,55 +557,47 @@ static const int FLICKER_CALIB_M = 256;
  * libm), so any IEEE-754 machine reproduces its bits; each value is rounded
  * once to f32. */
 void orc_box_muller(uint64_t w, float *zc, float *zs) {
-    /* ln of u1 = n 2^-25, n = 2 m1 + 1: exact split n = f 2^e with f in
-     * [sqrt(1/2), sqrt(2)], then ln f = 2 atanh((f-1)/(f+1)) by its series. */
-    uint32_t n = ((uint32_t)(w >> 40) << 1) | 1u;
+    /* ln of u1 = n 2^-24, n = 2 m1 + 1 from bits 63..41 (exact in f32): exact
+     * split n = f 2^e with f in [sqrt(1/2), sqrt(2)], then ln f = 2
+     * atanh((f-1)/(f+1)) by its series; every step in f32 with explicit fmaf. */
+    uint32_t n = ((uint32_t)(w >> 41) << 1) | 1u;
     int e = 31 - __builtin_clz(n);
-    double f = ldexp((double)n, -e);
-    if (f > 1.4142135623730951) {
-        f = f * 0.5;
+    float f = ldexpf((float)n, -e);
+    if (f > 1.41421356f) {
+        f = f * 0.5f;
         e += 1;
     }
-    double s = (f - 1.0) / (f + 1.0), z = s * s;
-    double p = 1.0 / 13.0;
-    p = p * z + 1.0 / 11.0;
-    p = p * z + 1.0 / 9.0;
-    p = p * z + 1.0 / 7.0;
-    p = p * z + 1.0 / 5.0;
-    p = p * z + 1.0 / 3.0;
-    p = p * z + 1.0;
-    double ln_u1 = (double)(e - 25) * 0.6931471805599453 + (2.0 * s) * p;
-    double r = sqrt(-2.0 * ln_u1);
+    float s = (f - 1.0f) / (f + 1.0f), z = s * s;
+    float p = fmaf(z, 1.0f / 9.0f, 1.0f / 7.0f);
+    p = fmaf(p, z, 1.0f / 5.0f);
+    p = fmaf(p, z, 1.0f / 3.0f);
+    p = fmaf(p, z, 1.0f);
+    float ln_u1 = fmaf((float)(e - 24), 0.693147182f, (2.0f * s) * p);
+    float r = sqrtf(-2.0f * ln_u1);
     /* angle 2 pi m2 2^-24: quadrant and reflection about pi/4 exactly in
      * integers, then the Taylor series of sin and cos on [0, pi/4]. */
     uint32_t m2 = (uint32_t)(w >> 16) & 0xFFFFFFu;
     uint32_t quad = m2 >> 22, v = m2 & 0x3FFFFFu;
     int flip = v > 0x200000u;
-    double th = (double)(flip ? 0x400000u - v : v) * (1.5707963267948966 * 0x1.0p-22);
-    double t2 = th * th;
-    double ps = 1.0 / 6227020800.0;
-    ps = ps * t2 - 1.0 / 39916800.0;
-    ps = ps * t2 + 1.0 / 362880.0;
-    ps = ps * t2 - 1.0 / 5040.0;
-    ps = ps * t2 + 1.0 / 120.0;
-    ps = ps * t2 - 1.0 / 6.0;
-    double pc = -1.0 / 87178291200.0;
-    pc = pc * t2 + 1.0 / 479001600.0;
-    pc = pc * t2 - 1.0 / 3628800.0;
-    pc = pc * t2 + 1.0 / 40320.0;
-    pc = pc * t2 - 1.0 / 720.0;
-    pc = pc * t2 + 1.0 / 24.0;
-    pc = pc * t2 - 0.5;
-    double sn = th + (th * t2) * ps, cs = 1.0 + t2 * pc;
-    double a = flip ? cs : sn, b = flip ? sn : cs, sv, cv;
+    float th = (float)(flip ? 0x400000u - v : v) * (1.57079632679489662f * 0x1.0p-22f);
+    float t2 = th * th;
+    float ps = fmaf(t2, 1.0f / 362880.0f, -1.0f / 5040.0f);
+    ps = fmaf(ps, t2, 1.0f / 120.0f);
+    ps = fmaf(ps, t2, -1.0f / 6.0f);
+    float pc = fmaf(t2, -1.0f / 3628800.0f, 1.0f / 40320.0f);
+    pc = fmaf(pc, t2, -1.0f / 720.0f);
+    pc = fmaf(pc, t2, 1.0f / 24.0f);
+    pc = fmaf(pc, t2, -0.5f);
+    float sn = fmaf(th * t2, ps, th), cs = fmaf(t2, pc, 1.0f);
+    float a = flip ? cs : sn, b = flip ? sn : cs, sv, cv;
     switch (quad) {
     case 0: sv = a; cv = b; break;
     case 1: sv = b; cv = -a; break;
     case 2: sv = -a; cv = -b; break;
     default: sv = -b; cv = a; break;
     }
-    *zc = (float)(r * cv);
-    *zs = (float)(r * sv);
+    *zc = r * cv;
+    *zs = r * sv;
 }
 
 /* Standard normal for master-grid index k of a receiver's clock walk: word

@@ -170,24 +170,62 @@ def test_awgn_matches_oracle_draws(backend):
 
 
 def test_front_end_ops_and_adc_codes(backend):
+    """src/rx/mod.rs:83-101 after the sum: gain, IP3 and an 8-bit ADC. With the
+    oracle's clock offsets every code and IQ value is bit-exact; with the
+    device clock a code may differ only where the oracle's value before the
+    ADC lies at a rounding boundary (its distance to the nearest half-LSB is
+    below the clock walk's effect on the sample, 1e-4 LSB)."""
     seed = 3
     fe = FrontendConfig(gain_db=110.0, nonlinearity=NonlinearityConfig(30.0), adc=AdcConfig(8, 1.0))
     scene = util.all_kinds_scene(duration_s=0.004, noise_power_w=None, frontend=fe)
     sc = prepare(scene)
     backend.upload(sc, seed)
     orc = OracleScene(sc, seed)
-    g, gc = gpu_band(backend, sc, 0, 0, 0, seed, codes=True)
-    o, oc, _ = orc.render_band(0, codes=True)
+    o, oc, taps = orc.render_band(0, codes=True, tap_slot=0)
     lsb = 1.0 / 127.0
-    mism = np.nonzero(gc != oc)[0]
-    _say(f"ADC: {len(mism)} of {len(oc)} codes differ; codes span [{oc.min()}, {oc.max()}]")
     assert oc.min() < -20 and oc.max() > 20          # the ADC is exercised, not idle
-    assert len(mism) <= 1e-3 * len(oc)
-    for k in mism:  # only at rounding boundaries: the codes differ by one level
-        assert abs(int(gc[k]) - int(oc[k])) == 1
+    g, gc = gpu_band(backend, sc, 0, 0, 0, seed, codes=True, delta_t=taps[3])
+    assert np.array_equal(gc, oc), "ADC codes with the oracle's clock offsets"
+    assert g.tobytes() == o.tobytes()
+    g, gc = gpu_band(backend, sc, 0, 0, 0, seed, codes=True)
+    mism = np.nonzero(gc != oc)[0]
+    _say(f"ADC: {len(mism)} of {len(oc)} codes differ on the device clock; codes span [{oc.min()}, {oc.max()}]")
+    if len(mism):
+        # the oracle's value before the ADC: the same scene with gain and IP3 only
+        pre_sc = prepare(util.all_kinds_scene(duration_s=0.004, noise_power_w=None,
+                                              frontend=FrontendConfig(gain_db=110.0,
+                                                                      nonlinearity=NonlinearityConfig(30.0))))
+        pre, _, _ = OracleScene(pre_sc, seed).render_band(0)
+        x = np.clip(pre.view(np.float32)[mism].astype(np.float64), -1.0, 1.0) / lsb
+        dist = np.abs(np.abs(x - np.floor(x)) - 0.5)
+        _say(f"ADC mismatches: distance of the pre-ADC value to the rounding boundary <= {dist.max():.2e} LSB")
+        assert np.all(dist < 1e-4)
+        assert np.all(np.abs(gc[mism].astype(int) - oc[mism].astype(int)) == 1)
     flat_o = o.view(np.float32)
     assert np.all(np.abs(flat_o) <= 1.0 + 1e-6)
     assert np.all(np.abs(np.round(flat_o / lsb) * lsb - flat_o) < 1e-6)
+
+
+def test_adc_wider_than_int16(backend):
+    """A 24-bit ADC (src/rx/mod.rs:93-97 takes any width): the quantised IQ is
+    bit-identical to the oracle's; int16 codes are refused for it, and the
+    refusal leaves the context usable."""
+    seed = 8
+    fe = FrontendConfig(gain_db=110.0, adc=AdcConfig(24, 1.0))
+    sc = prepare(util.all_kinds_scene(duration_s=0.002, noise_power_w=None, frontend=fe))
+    backend.upload(sc, seed)
+    orc = OracleScene(sc, seed)
+    o, _, taps = orc.render_band(0, tap_slot=0)
+    g, _ = gpu_band(backend, sc, 0, 0, 0, seed, delta_t=taps[3])
+    assert g.tobytes() == o.tobytes()
+    lsb = 1.0 / float((1 << 23) - 1)
+    flat = o.view(np.float32).astype(np.float64)
+    assert np.all(np.abs(np.round(flat / lsb) * lsb - flat) <= 1e-7 * np.maximum(np.abs(flat), lsb))
+    with pytest.raises(_lib.LpkError) as e:
+        gpu_band(backend, sc, 0, 0, 0, seed, codes=True, n=1000)
+    assert _lib.STATUS[e.value.code] == "LPK_E_INVALID" and "16 bits" in str(e.value)
+    g2, _ = gpu_band(backend, sc, 0, 0, 0, seed, delta_t=taps[3])
+    assert g2.tobytes() == o.tobytes()
 
 
 # ---------------------------------------------------------------- precision form
@@ -496,8 +534,23 @@ def test_tuning_and_autotune(backend):
             _say(f"autotune: tile 2^{best[0]}, xcd map {best[1]}, {ms:.3f} ms")
             assert best[0] in (7, 8, 9) and best[1] in (0, 1) and ms > 0.0
             assert backend.tuning() == best
+            # job sets the render refuses, or that hold no sample, are refused
+            # before any candidate runs, and the tuning stays as it was
+            backend.set_tuning(7, 1)
+            empty, keep_e = backend.job(0, 0, 0, seed, 0, 0, out_dev=dev.value)
+            with pytest.raises(_lib.LpkError) as e:
+                backend.autotune([empty], _lib.FORM_PRECISION)
+            assert _lib.STATUS[e.value.code] == "LPK_E_INVALID"
+            assert backend.tuning() == (7, 1)
         finally:
             backend.check(lib.lpk_device_free(backend.h, dev))
+        many = _many_links_scene()
+        backend.upload(many, 17)
+        j, keep = backend.job(0, 0, 0, 17, 0, 1000, out_dev=None)
+        with pytest.raises(_lib.LpkError) as e:
+            backend.autotune([j], _lib.FORM_PRECISION)
+        assert _lib.STATUS[e.value.code] == "LPK_E_INVALID" and "plain form" in str(e.value)
+        assert backend.tuning() == (7, 1)
     finally:
         backend.set_tuning(8, 1)
 
