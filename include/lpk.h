@@ -227,10 +227,14 @@ int lpk_autotune(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int 
                  lpk_tuning *best, double *best_ms);
 
 /* ---- streaming plans: generate_band's chunk loop (src/pipeline.rs:905-990)
- *      for many bands at once. Step k renders chunk k (samples
- *      [k*chunk_samples, (k+1)*chunk_samples)) of every band: the job, clock
- *      walk and carry tables of the step go to HBM and the walk, carry and
- *      render kernels run, all as one captured HIP graph replayed per step.
+ *      for many bands at once. Step k is a time slice: chunk_samples
+ *      samples of the band with the highest sample rate and, for every other
+ *      band, its samples in the same interval (lpk_plan_chunk_range gives
+ *      each band's [i0, i0 + clen) in step k; outputs do not depend on the
+ *      chunking, src/pipeline.rs:905-990 and tests/streaming_chunk_invariance.rs).
+ *      The job, clock walk and carry tables of the step go to HBM and the
+ *      walk, carry and render kernels run, all as one captured HIP graph
+ *      replayed per step.
  *      Each band descriptor covers the whole band (i0 = 0, clen = its
  *      sample count) with the device clock (delta_t NULL) and no host
  *      outputs or flags; out_dev, if set, holds the whole band and step k
@@ -243,6 +247,7 @@ int lpk_plan_create(lpk_ctx *ctx, const lpk_band_chunk *bands, uint32_t n_bands,
                     int form, lpk_plan **out);
 int lpk_plan_step(lpk_plan *plan, uint64_t chunk);     /* asynchronous, like lpk_render_chunk */
 int lpk_plan_info(const lpk_plan *plan, uint64_t *n_chunks, uint32_t *max_tiles);
+int lpk_plan_chunk_range(const lpk_plan *plan, uint32_t band, uint64_t chunk, uint64_t *i0, uint64_t *clen);
 int lpk_plan_output(const lpk_plan *plan, uint32_t band, void **chunk_dev);
 void lpk_plan_destroy(lpk_plan *plan);
 

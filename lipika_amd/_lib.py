@@ -160,6 +160,7 @@ def load() -> C.CDLL:
         "lpk_plan_create": (_i32, [C.c_void_p, P(BandChunk), _u32, _u64, C.c_int, P(C.c_void_p)]),
         "lpk_plan_step": (_i32, [C.c_void_p, _u64]),
         "lpk_plan_info": (_i32, [C.c_void_p, P(_u64), P(_u32)]),
+        "lpk_plan_chunk_range": (_i32, [C.c_void_p, _u32, _u64, P(_u64), P(_u64)]),
         "lpk_plan_output": (_i32, [C.c_void_p, _u32, P(C.c_void_p)]),
         "lpk_plan_destroy": (None, [C.c_void_p]),
         "lpk_timing_reset": (_i32, [C.c_void_p]),
@@ -232,7 +233,7 @@ EXPORTED = (
     "lpk_payload_qam16", "lpk_elapsed_periods", "lpk_frame_anchor", "lpk_cw_anchor_phase",
     "lpk_bands_overlap", "lpk_thermal_noise_power_w",
     "lpk_set_tuning", "lpk_get_tuning", "lpk_autotune",
-    "lpk_plan_create", "lpk_plan_step", "lpk_plan_info", "lpk_plan_output", "lpk_plan_destroy",
+    "lpk_plan_create", "lpk_plan_step", "lpk_plan_info", "lpk_plan_chunk_range", "lpk_plan_output", "lpk_plan_destroy",
     "lpk_lws_create", "lpk_lws_destroy", "lpk_lws_get_params", "lpk_lws_set_params", "lpk_lws_generate",
     "lpk_lws_infer", "lpk_lws_gradient", "lpk_lws_train", "lpk_set_warm_start", "lpk_solve_stats",
     "lpk_solve_stats_reset", "lpk_set_horizon_gate", "lpk_horizon_gate_stats",

@@ -156,6 +156,8 @@ struct lpk_plan {
     bool antenna = false;
     uint64_t chunk = 0, n_chunks = 0, next = 0;
     std::vector<lpk_band_chunk> bands;
+    std::vector<double> rate_ratio;       // per band: fs / the highest fs of the plan
+    std::vector<uint64_t> chunk_cap;      // per band: its largest chunk
     std::vector<std::vector<uint32_t>> active;
     std::vector<uint32_t> order;          // bands in job-table order (heaviest first)
     std::vector<uint64_t> link0;          // per band: its first link
@@ -924,11 +926,22 @@ void render(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form,
 
 // --------------------------------------------------------------- plans
 
-// Samples of band b in chunk k.
-void plan_chunk(const lpk_plan *P, uint32_t b, uint64_t k, uint64_t &i0, uint64_t &clen) {
+// First sample of band b in step k: steps are time slices, chunk_samples of
+// the band with the highest sample rate and, for every other band, its
+// samples in the same interval (sample j of band b sits at j / fs_b, so the
+// slice [k T, (k + 1) T) with T = chunk / fs_max starts at the first j with
+// j >= k chunk fs_b / fs_max). Each receiver's clock window per step is then
+// that interval, so the walk is generated about once over a pass.
+uint64_t plan_start(const lpk_plan *P, uint32_t b, uint64_t k) {
     const uint64_t n = P->bands[b].clen;
-    i0 = k * P->chunk;
-    clen = i0 < n ? std::min<uint64_t>(P->chunk, n - i0) : 0;
+    const double s = std::ceil(double(k) * double(P->chunk) * P->rate_ratio[b]);
+    return s >= double(n) ? n : uint64_t(s);
+}
+
+// Samples [i0, i0 + clen) of band b in step k.
+void plan_chunk(const lpk_plan *P, uint32_t b, uint64_t k, uint64_t &i0, uint64_t &clen) {
+    i0 = plan_start(P, b, k);
+    clen = plan_start(P, b, k + 1) - i0;
 }
 
 // Union of the master-index windows of a receiver's bands in chunk k.
@@ -1073,8 +1086,19 @@ lpk_plan *plan_create(lpk_ctx *ctx, const lpk_band_chunk *bands, uint32_t n_band
         P->active[b].assign(B.active, B.active + B.n_active);
         B.active = P->active[b].data();
         P->antenna = P->antenna || B.has_antenna;
-        P->n_chunks = std::max(P->n_chunks, (B.clen + chunk - 1) / chunk);
     }
+    double fs_max = 0.0;
+    for (const lpk_band_chunk &B : P->bands) fs_max = std::max(fs_max, B.fs_hz);
+    P->rate_ratio.resize(n_bands);
+    for (uint32_t b = 0; b < n_bands; ++b) P->rate_ratio[b] = P->bands[b].fs_hz / fs_max;   // 1 for the fastest
+    // the steps until every band's last sample is rendered
+    for (uint32_t b = 0; b < n_bands; ++b) {
+        uint64_t k = uint64_t(std::ceil(double(P->bands[b].clen) / (double(chunk) * P->rate_ratio[b])));
+        while (k > 0 && plan_start(P.get(), b, k - 1) >= P->bands[b].clen) --k;
+        while (plan_start(P.get(), b, k) < P->bands[b].clen) ++k;
+        P->n_chunks = std::max(P->n_chunks, k);
+    }
+    P->chunk_cap.assign(n_bands, 0);
     P->order.resize(n_bands);
     for (uint32_t b = 0; b < n_bands; ++b) P->order[b] = b;
     std::stable_sort(P->order.begin(), P->order.end(),
@@ -1112,6 +1136,7 @@ lpk_plan *plan_create(lpk_ctx *ctx, const lpk_band_chunk *bands, uint32_t n_band
             uint64_t i0, clen;
             plan_chunk(P.get(), b, k, i0, clen);
             tiles += tiles_of(i0, clen, span_shift(P->form, P->antenna, P->tile_shift));
+            P->chunk_cap[b] = std::max(P->chunk_cap[b], clen);
         }
         if (blocks > 0xFFFFFFull || tiles > 0xFFFFFFFFull) throw Fail{LPK_E_INVALID, "plan step too large"};
         P->max_blocks = std::max<uint32_t>(P->max_blocks, uint32_t(blocks));
@@ -1130,7 +1155,7 @@ lpk_plan *plan_create(lpk_ctx *ctx, const lpk_band_chunk *bands, uint32_t n_band
     P->blocksum.ensure(size_t(std::max<uint32_t>(P->max_blocks, 1)) * sizeof(double), ctx->stream);
     P->outbuf.resize(n_bands);
     for (uint32_t b = 0; b < n_bands; ++b)
-        if (!P->bands[b].out_dev) P->outbuf[b].ensure(size_t(chunk) * 8, ctx->stream);
+        if (!P->bands[b].out_dev) P->outbuf[b].ensure(size_t(std::max<uint64_t>(P->chunk_cap[b], 1)) * 8, ctx->stream);
     auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
     P->off_jobs = 256;
     P->off_links = align(P->off_jobs + n_bands * sizeof(JobDev));
@@ -1472,6 +1497,17 @@ int lpk_plan_info(const lpk_plan *plan, uint64_t *n_chunks, uint32_t *max_tiles)
     if (!plan) return LPK_E_INVALID;
     if (n_chunks) *n_chunks = plan->n_chunks;
     if (max_tiles) *max_tiles = plan->max_tiles;
+    return LPK_OK;
+}
+
+int lpk_plan_chunk_range(const lpk_plan *plan, uint32_t band, uint64_t chunk, uint64_t *i0, uint64_t *clen) {
+    if (!plan || !i0 || !clen || band >= plan->bands.size()) return LPK_E_INVALID;
+    if (chunk >= plan->n_chunks) {
+        *i0 = plan->bands[band].clen;
+        *clen = 0;
+        return LPK_OK;
+    }
+    plan_chunk(plan, band, chunk, *i0, *clen);
     return LPK_OK;
 }
 

@@ -681,6 +681,12 @@ class Plan:
     def step(self, k: int) -> None:
         self.be.check(self.be.lib.lpk_plan_step(self.h, k))
 
+    def chunk_range(self, band: int, k: int) -> tuple:
+        """(i0, clen) of band `band` in step k: the steps are time slices."""
+        i0, n = C.c_uint64(), C.c_uint64()
+        self.be.check(self.be.lib.lpk_plan_chunk_range(self.h, band, k, C.byref(i0), C.byref(n)))
+        return i0.value, n.value
+
     def output(self, band: int) -> int:
         """Device address of band `band`'s output for the last step."""
         p = C.c_void_p()
