@@ -264,11 +264,15 @@ def learned_leg(steps, warmup, chunks=2):
     first `chunks` chunks of every band are rendered in the plain form (the
     reference's two fixed iterations), the converged form (iterated to a
     1e-13 s update from the reference's start) and the learned form (the
-    same, from the network's prediction). The whole 60 s scene is 252 GB of
-    output, more than one GPU's HBM beside its clock walks; per-contribution
-    work does not depend on the time within the scene (MEO geometry, no
-    horizon gate), so the full scene's time is the step's time scaled by
-    `full_scene_factor`."""
+    same, from the network's light times at 2 ms query points, converged
+    there and interpolated). `straight_line_start` is the learned form with
+    an all-zero network (the queries start from the straight-line light
+    time): the network's own gain is the query evaluations it saves against
+    that. Evaluation counts include the queries'. The whole 60 s scene is
+    252 GB of output, more than one GPU's HBM beside its clock walks;
+    per-contribution work does not depend on the time within the scene (MEO
+    geometry, no horizon gate), so the full scene's time is the step's time
+    scaled by `full_scene_factor`."""
     from lipika_amd.pipeline import WarmStartNet, form_code
     S = Scene(5, max_chunks=chunks)
     be, lib = S.be, S.lib
@@ -279,21 +283,25 @@ def learned_leg(steps, warmup, chunks=2):
     be.warm_start(net)
     full = sum(pb.n * len(pb.active) for (_, _, _, pb, _) in S.units)
     forms = {}
-    for name in ("plain", "converged", "learned"):
+    trained_params = net.params
+    for name in ("plain", "converged", "learned", "straight_line_start"):
+        if name == "straight_line_start":
+            net.params = 0.0 * trained_params
+            be.warm_start(net)
         be.solve_stats_reset()
-        entry = time_form(S, name, steps, warmup)
+        entry = time_form(S, "learned" if name == "straight_line_start" else name, steps, warmup)
         if name == "plain":
             entry["evaluations_per_contribution"] = 2.0    # the reference's fixed two iterations
         else:
             st = be.solve_stats()
-            entry["evaluations_per_contribution"] = round(st["evaluations"] / max(st["contributions"], 1), 4)
+            entry["evaluations_per_contribution"] = round(st["evaluations"] / max(st["contributions"], 1), 6)
             entry["fallbacks"] = st["fallbacks"]
             entry["unconverged"] = st["unconverged"]
         forms[name] = entry
     be.warm_start(None)
     net.close()
     secs = chunks * (1 << 20) / 2.048e6
-    lf, cf, pf = forms["learned"], forms["converged"], forms["plain"]
+    lf, cf, pf, sf = forms["learned"], forms["converged"], forms["plain"], forms["straight_line_start"]
     factor = full / S.contributions
     out = {"workload": "config 5: " + WORKLOADS[5].format(chunks=chunks, secs=secs),
            "contributions_per_step": S.contributions, "full_scene_contributions": full,
@@ -304,6 +312,8 @@ def learned_leg(steps, warmup, chunks=2):
                                                        - lf["evaluations_per_contribution"], 4),
            "learned_speedup_vs_converged": round(cf["ms_per_step"] / lf["ms_per_step"], 3),
            "learned_speedup_vs_plain": round(pf["ms_per_step"] / lf["ms_per_step"], 3),
+           "network_evaluations_saved_per_contribution": round(sf["evaluations_per_contribution"]
+                                                               - lf["evaluations_per_contribution"], 6),
            "training": {"pairs": 10_000_000, "epochs": 16, "batch": 16384, "steps": train["steps"],
                         "train_ms": round(train["train_ms"], 1), "pair_generation_ms": round(train["gen_ms"], 2),
                         "validation_rms_s": train["val_rms_s"], "validation_max_s": train["val_max_s"]}}

@@ -88,7 +88,7 @@ typedef struct {
     double pos_ecef[3];          /* STATIONARY: ECEF metres */
     double grid_t0_unix;         /* GRID: absolute time of node 0 (scene epoch - 0.5 s) */
     double grid_dt_s;            /* GRID: node spacing */
-    uint64_t grid_n;             /* GRID: node count, >= 2 */
+    uint64_t grid_n;             /* GRID: node count, 2 .. 2^31 - 1 */
     const double *grid_xyz;      /* GRID: (3, n) row-major, as ndarray Array2 (3, N) */
     double carrier_hz;           /* SignalModel::carrier_hz */
     double tx_power_w;           /* 10^(dBW/10), src/pipeline.rs:338 */

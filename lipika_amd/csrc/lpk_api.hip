@@ -25,7 +25,7 @@ hipError_t launch_render(int form, bool antenna, int tile_shift, bool xcd_map, c
                          unsigned long long *gate_stats, hipStream_t stream);
 hipError_t launch_lws_predict(const JobDev *jobs, int n_jobs, const LinkDev *links, const TxDev *txs,
                               const LwsNetDev &net, int tile_shift, uint64_t n_queries, double *pred,
-                              hipStream_t stream);
+                              unsigned long long *stats, hipStream_t stream);
 hipError_t launch_lws_gen(uint64_t n, uint64_t seed, float *X, float *Y, double *G, hipStream_t s);
 hipError_t launch_lws_infer(const LwsNetDev &net, const float *X, uint64_t n, float *Y, hipStream_t s);
 hipError_t launch_lws_train(const LwsNetDev &net, const float *X, const float *Y, uint64_t base, uint64_t end,
@@ -340,12 +340,14 @@ TxDev make_tx(lpk_ctx *ctx, const lpk_tx_desc &d, uint32_t idx) {
         for (int k = 0; k < 3; ++k) T.pos[k] = d.pos_ecef[k];
     } else if (d.traj_kind == LPK_TRAJ_GRID) {
         if (d.grid_n < 2 || !d.grid_xyz) throw bad("grid needs at least 2 nodes and a position table");
+        if (d.grid_n > uint64_t(INT32_MAX)) throw bad("grid has more than 2^31 - 1 nodes");
         if (!(d.grid_dt_s > 0.0) || !is_fin(d.grid_dt_s) || !is_fin(d.grid_t0_unix))
             throw Fail{LPK_E_NONFINITE, "transmitter " + std::to_string(idx) + ": grid start/step"};
         T.grid_t0 = d.grid_t0_unix;
         T.grid_dt = d.grid_dt_s;
         T.grid_inv_dt = 1.0 / d.grid_dt_s;
         T.grid_n = int64_t(d.grid_n);
+        T.grid_hi = static_cast<double>(T.grid_n - 2);
         T.grid = static_cast<const double *>(upload(ctx, d.grid_xyz, 3 * d.grid_n * sizeof(double)));
     } else {
         throw bad("unknown trajectory kind");
@@ -891,7 +893,7 @@ void render(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form,
         timing_begin(ctx);
         if (pred)
             hip_check(launch_lws_predict(jd, int(n_jobs), ld, ctx->tx_dev, lws_net(ctx->lws), ts, n_pred, pred,
-                                         ctx->stream),
+                                         ctx->stats, ctx->stream),
                       "warm-start prediction launch");
         hip_check(launch_render(converged ? 2 : form, any_antenna, ts, ctx->xcd_map != 0, jd, int(n_jobs), ld,
                                 ctx->tx_dev, uint32_t(tiles), nullptr, tap ? tap->t_emit : nullptr,

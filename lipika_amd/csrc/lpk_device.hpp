@@ -97,6 +97,7 @@ struct TxDev {
     double grid_t0, grid_dt;
     double grid_inv_dt;      // 1 / grid_dt (the converged form's grid coordinate)
     int64_t grid_n;
+    double grid_hi;          // grid_n - 2: the last cell's first node (grid_n < 2^31)
     const double *grid;      // (3, n)
     double carrier;
     double neg_tau_src;      // -TAU * f_src            (src/channel/mod.rs:239)
@@ -141,16 +142,22 @@ struct TxDev {
 constexpr double kGateMarginDeg = 1.0e-3;     // src/channel/horizon.rs:64
 constexpr double kGateMaxLightTime = 100.0;   // src/channel/horizon.rs:68
 constexpr int kGateMaxLinks = 1024;           // links per tile the gate tests (more go per sample)
+constexpr int kPredStageLinks = 256;          // converged forms: links per block of starts staged in LDS
 
 // Converged emission-time solve (the converged and learned forms): the
 // reference's iteration t <- tu - |R(w (tu - t)) P(t) - O| / c on the
 // unrounded time axis, stopped at the first evaluation whose update is below
-// kSolveTol seconds. A learned start gets kLearnedEvals evaluations before it
-// falls back to the reference's start tu - 0.07 s with kColdEvals.
-// The learned start is predicted every kLwsSpacing seconds of reception time
-// (whole tiles, at least one) and interpolated linearly in between: the
-// light time's curvature (|d2tau/dt2| < 4e-7 s/s^2 even for a LEO overhead
-// pass) keeps that interpolation within 2e-13 s over 2 ms.
+// kSolveTol seconds, from the reference's start tu - 0.07 s or from a learned
+// start, for at most kColdEvals evaluations (the iteration contracts by about
+// |dr/dt| / c per evaluation, so a poor start only costs evaluations). The
+// learned start is the network's light time every kLwsSpacing seconds of
+// reception time (whole tiles, at least one), converged there by the same
+// iteration, and interpolated linearly in between. The light time's
+// curvature bounds that interpolation: about 4e-16 s over 2 ms for MEO (one
+// evaluation then passes the tolerance), 2e-13 s for a LEO overhead pass
+// (|d2tau/dt2| < 4e-7 s/s^2; two there). A query the network's start needs
+// more than kLearnedEvals evaluations to converge is counted as a fallback,
+// and so is a sample's solve that needs more than that.
 constexpr double kSolveTol = 1e-13;
 constexpr double kLwsSpacing = 2e-3;
 constexpr int kLearnedEvals = 3;

@@ -25,9 +25,9 @@
 // FORM 2 (converged) is the plain form with the emission-time solve iterated
 // to a residual tolerance (kSolveTol) on the unrounded time axis instead of
 // the reference's two fixed iterations; it starts either from the
-// reference's t - 0.07 s or, in the learned form, from light times that
-// lws_predict_kernel's network predicted at the tile's boundaries, falling
-// back to the reference's start when those do not converge quickly.
+// reference's t - 0.07 s or, in the learned form, from the light times of
+// lws_predict_kernel at the tile's query boundaries (the network's
+// predictions, converged there by the same iteration), interpolated.
 //
 // walk_gen_kernel: the receiver clock's white-FM walk (src/clock/mod.rs:
 // 299-307) on the master grid, as fixed 65 536-index blocks; values depend
@@ -85,8 +85,8 @@ __device__ __forceinline__ void tx_position(const TxDev &T, double t_abs, double
         return;
     }
     const double f = (t_abs - T.grid_t0) / T.grid_dt;
-    const double flo = fmin(fmax(floor(f), 0.0), static_cast<double>(T.grid_n - 2));
-    const int64_t lo = static_cast<int64_t>(flo);
+    const double flo = fmin(fmax(floor(f), 0.0), T.grid_hi);
+    const int32_t lo = static_cast<int32_t>(flo);
     double frac = f - flo;
     frac = frac < 0.0 ? 0.0 : (frac > 1.0 ? 1.0 : frac);
     const double omf = 1.0 - frac;
@@ -137,8 +137,8 @@ __device__ __forceinline__ void tx_position_rel(const TxDev &T, double trel, dou
         return;
     }
     const double f = trel * T.grid_inv_dt;
-    const double flo = fmin(fmax(floor(f), 0.0), static_cast<double>(T.grid_n - 2));
-    const int64_t lo = static_cast<int64_t>(flo);
+    const double flo = fmin(fmax(floor(f), 0.0), T.grid_hi);
+    const int32_t lo = static_cast<int32_t>(flo);
     double frac = f - flo;
     frac = frac < 0.0 ? 0.0 : (frac > 1.0 ? 1.0 : frac);
     const double omf = 1.0 - frac;
@@ -166,31 +166,17 @@ __device__ __forceinline__ double emission_step(const TxDev &T, const double *ob
     return tu - r * kInvC;
 }
 
-// The converged solve from emission time t: iterate until an update
-// |t_next - t| < kSolveTol. Returns the number of evaluations, or -1 when
-// max_evals did not converge; t_emit and r are the last iterate and the range
-// it came from (r / c = tu - t_emit). The first K evaluations run back to back
-// and their updates are tested afterwards, in order, so the result is that of
-// the plain loop (which stops at the first small update) while the common
-// case is straight-line code.
-template <int K>
-__device__ __forceinline__ int solve_converged(const TxDev &T, const double *obs, double tu, double eoff, double t,
-                                               int max_evals, double &t_emit, double &r) {
-    double tk[K + 1], rk[K];
-    tk[0] = t;
-#pragma unroll
-    for (int k = 0; k < K; ++k) tk[k + 1] = emission_step(T, obs, tu, eoff, tk[k], rk[k]);
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        if (fabs(tk[k + 1] - tk[k]) < kSolveTol) {
-            t_emit = tk[k + 1];
-            r = rk[k];
-            return k + 1;
-        }
-    }
-    t = tk[K];
-    r = rk[K - 1];
-    for (int e = K + 1; e <= max_evals; ++e) {
+// The converged solve: the iteration from emission time t0 until the first
+// update |t_next - t| < kSolveTol. Callers run its first evaluation
+// (t_emit = t1 from t0, with range r) and stop there when its update is
+// small, the usual case from a learned start; this goes on from t1 for every
+// other lane. Returns the number of evaluations, or -1 when max_evals did
+// not converge; t_emit and r are then the last iterate and the range it came
+// from (r / c = tu - t_emit).
+__device__ __forceinline__ int solve_converged_tail(const TxDev &T, const double *obs, double tu, double eoff,
+                                                    int max_evals, double &t_emit, double &r) {
+    double t = t_emit;
+    for (int e = 2; e <= max_evals; ++e) {
         const double tn = emission_step(T, obs, tu, eoff, t, r);
         const double step = tn - t;
         t = tn;
@@ -529,8 +515,8 @@ __device__ __forceinline__ void solve_smooth(const TxDev &T, const double *obs, 
             sz = T.pos[2];
         } else {
             const double f = (eoff + t) * T.grid_inv_dt;
-            const double flo = fmin(fmax(floor(f), 0.0), static_cast<double>(T.grid_n - 2));
-            const int64_t lo = static_cast<int64_t>(flo);
+            const double flo = fmin(fmax(floor(f), 0.0), T.grid_hi);
+            const int32_t lo = static_cast<int32_t>(flo);
             double frac = f - flo;
             frac = frac < 0.0 ? 0.0 : (frac > 1.0 ? 1.0 : frac);
             const double *g = T.grid;
@@ -1058,7 +1044,6 @@ __global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM == 1 ? 4 : 6))
     __shared__ uint64_t s_noise[kSpan ? 1 : kTile];    // one Philox word per sample
     __shared__ double s_anc_t[3][3];                   // tu, t_local, delta_t at the anchors
     __shared__ Anchor s_anc[FORM == 1 && ANT ? kAnchorGroup : 1];
-    __shared__ unsigned long long s_stats[4];
 
     const uint32_t tile = lx.tile_base + (lx.xcd_map ? xcd_tile(blockIdx.x, lx.n_tiles) : blockIdx.x);
     if (lx.n_live && tile >= *lx.n_live) return;
@@ -1138,10 +1123,13 @@ __global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM == 1 ? 4 : 6))
         // places them)
         const double *prow = nullptr;
         double frac = 0.0;
-        // solve counters of this wave, wave-uniform (scalar registers):
-        // evaluations beyond `nominal` per contribution, fallbacks, unconverged
-        int64_t w_extra = 0;
-        uint32_t w_fb = 0, w_unc = 0;
+        // solve counters (converged forms). Every solve costs two evaluations
+        // unless it ends elsewhere, so the wave counts the links it solved
+        // (uniform) and each lane only the evaluations beyond two; fallbacks
+        // (more than kLearnedEvals from a learned start) and unconverged solves
+        // are counted per wave from ballots over the valid lanes
+        uint32_t n_solved = 0, n_fb = 0, n_unc = 0;
+        int32_t n_extra = 0;
         // the links the gate skips for this tile, as a bitmap
         __shared__ uint32_t s_gated[ANT ? kGateMaxLinks / 32 : 1];
         if (ANT && gating) {
@@ -1160,6 +1148,7 @@ __global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM == 1 ? 4 : 6))
                 atomicAdd(&lx.gate_stats[1], static_cast<unsigned long long>(ng));
             }
         }
+        __shared__ double2 s_pred[FORM == 2 ? kPredStageLinks : 1];
         if (FORM == 2 && lx.pred) {
             const uint32_t bq = (tile - J.tile0) / J.pred_qs;   // the query boundary at or before this tile
             const uint64_t last = J.i0 + J.clen - 1;
@@ -1171,91 +1160,108 @@ __global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM == 1 ? 4 : 6))
             frac = ib1 > ib0 ? static_cast<double>(ic - ib0) / static_cast<double>(ib1 - ib0) : 0.0;
             prow = lx.pred + J.pred0 + bq;
         }
-        for (uint32_t s = 0; s < J.n_active; ++s) {
-            if (ANT && gating && s < kGateMaxLinks && ((s_gated[s >> 5] >> (s & 31)) & 1u)) continue;
-            const LinkDev L = links[J.link0 + s];
-            const TxDev &T = txs[L.tx];
-            double t_emit, r;
-            if (FORM == 0) {
-                int64_t c0, c1;
-                solve_emission(T, J.obs, tu, J.epoch, t_emit, r, c0, c1);
-            } else {
-                const double eoff = T.traj_kind ? J.epoch - T.grid_t0 : 0.0;
-                int e = -1;
-                uint32_t used = 0;   // evaluations before the reference's start
-                if (prow) {
-                    const double *pp = prow + static_cast<uint64_t>(s) * J.pred_nb;
-                    const double ta = pp[0], tb = pp[1];
-                    e = solve_converged<2>(T, J.obs, tu, eoff, tu - (ta + (tb - ta) * frac), kLearnedEvals, t_emit, r);
-                    if (e < 0) used = kLearnedEvals;
+        // the converged forms take the links in blocks of kPredStageLinks and
+        // stage each block's start as (lag at the tile's first query, its
+        // change to the next), so that no solve waits on a global load: the
+        // learned form's two predictions, or (0.07 s, 0) for the reference's
+        // start; the plain form takes every link in one pass
+        for (uint32_t s0 = 0, s1; s0 < J.n_active; s0 = s1) {
+            const uint32_t blk = static_cast<uint32_t>(kPredStageLinks);
+            s1 = FORM != 2 || J.n_active - s0 <= blk ? J.n_active : s0 + blk;
+            if (FORM == 2) {
+                if (s0) __syncthreads();   // every wave is done with the previous block
+                for (uint32_t s = s0 + threadIdx.x; s < s1; s += kTile) {
+                    double ta = kEmissionInitialLag, dab = 0.0;
+                    if (prow) {
+                        const double *pp = prow + static_cast<uint64_t>(s) * J.pred_nb;
+                        ta = pp[0];
+                        dab = pp[1] - ta;
+                    }
+                    s_pred[s - s0] = make_double2(ta, dab);
                 }
-                // the reference's start: three evaluations unrolled (the usual count
-                // from it), one at a time on a learned start's rare fallback
-                if (e < 0)
-                    e = prow ? solve_converged<1>(T, J.obs, tu, eoff, tu - kEmissionInitialLag, kColdEvals, t_emit, r)
-                             : solve_converged<3>(T, J.obs, tu, eoff, tu - kEmissionInitialLag, kColdEvals, t_emit, r);
-                if (lx.stats) {
-                    // lanes whose count differs from the nominal one, counted by value
-                    const uint32_t nominal = prow ? 2u : 3u;
-                    const uint32_t ev = used + (e < 0 ? static_cast<uint32_t>(kColdEvals) : static_cast<uint32_t>(e));
-                    const bool dev = valid && ev != nominal;
-                    uint64_t m = __ballot(dev);
-                    if (m) {
-                        w_fb += __popcll(__ballot(valid && used != 0));
-                        w_unc += __popcll(__ballot(valid && e < 0));
-                        while (m) {
-                            const uint32_t v = __builtin_amdgcn_readlane(ev, __ffsll(static_cast<long long>(m)) - 1);
-                            const uint64_t mm = __ballot(dev && ev == v);
-                            w_extra += (static_cast<int64_t>(v) - static_cast<int64_t>(nominal)) * __popcll(mm);
-                            m &= ~mm;
+                __syncthreads();
+            }
+            for (uint32_t s = s0; s < s1; ++s) {
+                if (ANT && gating && s < kGateMaxLinks && ((s_gated[s >> 5] >> (s & 31)) & 1u)) continue;
+                const LinkDev L = links[J.link0 + s];
+                const TxDev &T = txs[L.tx];
+                double t_emit, r;
+                if (FORM == 0) {
+                    int64_t c0, c1;
+                    solve_emission(T, J.obs, tu, J.epoch, t_emit, r, c0, c1);
+                } else {
+                    const double eoff = T.traj_kind ? J.epoch - T.grid_t0 : 0.0;
+                    // the learned start (the converged light times of the tile's
+                    // two queries, interpolated) or the reference's, t - 0.07 s;
+                    // then the iteration to the tolerance from there: one
+                    // evaluation (enough from a learned start), the rest only
+                    // in waves where some lane needs more
+                    const double2 p = s_pred[s - s0];
+                    const double t0 = tu - (p.x + p.y * frac);
+                    t_emit = emission_step(T, J.obs, tu, eoff, t0, r);
+                    const bool at_one = fabs(t_emit - t0) < kSolveTol;
+                    if (__any(!at_one)) {
+                        int e = 1;
+                        if (!at_one) e = solve_converged_tail(T, J.obs, tu, eoff, kColdEvals, t_emit, r);
+                        if (lx.stats) {
+                            const int ev = e < 0 ? kColdEvals : e;
+                            n_extra += valid ? ev - 1 : 0;
+                            n_fb += prow ? __popcll(__ballot(valid && ev > kLearnedEvals)) : 0u;
+                            n_unc += __popcll(__ballot(valid && e < 0));
                         }
                     }
+                    ++n_solved;
                 }
+                // src/channel/mod.rs:255-267
+                const double delay = r * kInvC;
+                const double phase = T.neg_tau_src * (delay + dlt) + L.tau_if * t_local;
+                if (tap.on && s == tap.slot && valid) {
+                    tap.t_emit[j] = t_emit;
+                    tap.range[j] = r;
+                    tap.phase[j] = phase;
+                }
+                // antenna gain; an occluded sample adds nothing (src/channel/mod.rs:253-255)
+                float g = 1.0f;
+                if (ANT && J.ant_on) {
+                    g = antenna_gain(J, T, t_emit);
+                    if (g != 0.0f && J.unocc) J.unocc[J.link0 + s] = 1u;
+                    if (g == 0.0f) continue;
+                }
+                float bre, bim;
+                baseband(T, t_emit, bre, bim);
+                double sp, cp;
+                sincos(phase, &sp, &cp);
+                const float cpf = static_cast<float>(cp), spf = static_cast<float>(sp);
+                const float amplitude = static_cast<float>(T.amp_factor / r) * g;
+                acc_re = acc_re + amplitude * (bre * cpf - bim * spf);
+                acc_im = acc_im + amplitude * (bre * spf + bim * cpf);
             }
-            // src/channel/mod.rs:255-267
-            const double delay = r * kInvC;
-            const double phase = T.neg_tau_src * (delay + dlt) + L.tau_if * t_local;
-            if (tap.on && s == tap.slot && valid) {
-                tap.t_emit[j] = t_emit;
-                tap.range[j] = r;
-                tap.phase[j] = phase;
-            }
-            // antenna gain; an occluded sample adds nothing (src/channel/mod.rs:253-255)
-            float g = 1.0f;
-            if (ANT && J.ant_on) {
-                g = antenna_gain(J, T, t_emit);
-                if (g != 0.0f && J.unocc) J.unocc[J.link0 + s] = 1u;
-                if (g == 0.0f) continue;
-            }
-            float bre, bim;
-            baseband(T, t_emit, bre, bim);
-            double sp, cp;
-            sincos(phase, &sp, &cp);
-            const float cpf = static_cast<float>(cp), spf = static_cast<float>(sp);
-            const float amplitude = static_cast<float>(T.amp_factor / r) * g;
-            acc_re = acc_re + amplitude * (bre * cpf - bim * spf);
-            acc_im = acc_im + amplitude * (bre * spf + bim * cpf);
         }
         if (FORM == 2 && lx.stats) {
-            // one LDS add per wave and counter, one global add per workgroup
-            if (threadIdx.x < 4) s_stats[threadIdx.x] = 0;
-            __syncthreads();
-            // links the horizon gate skipped for this tile do no solve: count only the solved ones
-            uint32_t n_solved = J.n_active;
-            if (ANT && gating) {
-                const uint32_t ng = J.n_active < kGateMaxLinks ? J.n_active : kGateMaxLinks;
-                for (uint32_t w = 0; w < (ng + 31) / 32; ++w) n_solved -= __popc(s_gated[w]);
+            // every wave solved the same links: thread 0 counts the tile's
+            // contributions and one evaluation for each; waves add what
+            // differs from that, when anything does (evaluations beyond one,
+            // signed, as a wrapping add), and their fallbacks and unconverged
+            // solves
+            if (threadIdx.x == 0) {
+                const uint64_t lo = tile_start > J.i0 ? tile_start : J.i0;
+                const uint64_t hi = tile_start + kTile < J.i0 + J.clen ? tile_start + kTile : J.i0 + J.clen;
+                const unsigned long long ct = static_cast<unsigned long long>(hi > lo ? hi - lo : 0) * n_solved;
+                if (ct) {
+                    atomicAdd(&lx.stats[0], ct);
+                    atomicAdd(&lx.stats[1], ct);
+                }
             }
-            const uint64_t contrib = static_cast<uint64_t>(__popcll(__ballot(valid))) * n_solved;
-            const uint64_t evals = (prow ? 2u : 3u) * contrib + static_cast<uint64_t>(w_extra);
+            if (__any(n_extra != 0)) {
+                int32_t extra = n_extra;
+                for (int o = 32; o >= 1; o >>= 1) extra += __shfl_xor(extra, o);
+                if ((threadIdx.x & 63) == 0 && extra != 0)
+                    atomicAdd(&lx.stats[0], static_cast<unsigned long long>(static_cast<int64_t>(extra)));
+            }
             if ((threadIdx.x & 63) == 0) {
-                if (evals) atomicAdd(&s_stats[0], static_cast<unsigned long long>(evals));
-                if (contrib) atomicAdd(&s_stats[1], static_cast<unsigned long long>(contrib));
-                if (w_fb) atomicAdd(&s_stats[2], static_cast<unsigned long long>(w_fb));
-                if (w_unc) atomicAdd(&s_stats[3], static_cast<unsigned long long>(w_unc));
+                if (n_fb) atomicAdd(&lx.stats[2], static_cast<unsigned long long>(n_fb));
+                if (n_unc) atomicAdd(&lx.stats[3], static_cast<unsigned long long>(n_unc));
             }
-            __syncthreads();
-            if (threadIdx.x < 4 && s_stats[threadIdx.x]) atomicAdd(&lx.stats[threadIdx.x], s_stats[threadIdx.x]);
         }
     } else {
         const int a_first = static_cast<int>(first - tile_start);
@@ -1344,17 +1350,23 @@ __global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM == 1 ? 4 : 6))
 
 // ---------------------------------------------------------------- learned start
 
-// Predicted light time at every pred_qs-th tile boundary of every link of a
-// render call (query q = pred0 + link * pred_nb + j of its job); query j sits
-// at sample ((i0 >> TS) + j * pred_qs) << TS clamped into the job. The prediction is
-// the straight-line light time at that sample's reception time plus the
-// network's correction (none for a trajectory without orbital elements).
-// 128 queries per workgroup, 32 per wave on the matrix cores.
+// Light time at every pred_qs-th tile boundary of every link of a render
+// call (query q = pred0 + link * pred_nb + j of its job); query j sits at
+// sample ((i0 >> TS) + j * pred_qs) << TS clamped into the job. The network
+// predicts it as the straight-line light time at that sample's reception
+// time plus its correction (none for a trajectory without orbital elements),
+// and the converged iteration refines the prediction there: the query keeps
+// the converged light time, or the prediction when twelve evaluations do not
+// converge. The refinement's evaluations are added to stats[0], and queries
+// that needed more than kLearnedEvals to stats[2] (fallbacks). 128 queries
+// per workgroup, 32 per wave on the matrix cores (lanes l and l + 32 of a
+// wave share query l).
 __global__ void __launch_bounds__(256) lws_predict_kernel(const JobDev *__restrict__ jobs, int n_jobs,
                                                           const LinkDev *__restrict__ links,
                                                           const TxDev *__restrict__ txs, LwsNetDev net,
                                                           int tile_shift, uint64_t n_queries,
-                                                          double *__restrict__ pred) {
+                                                          double *__restrict__ pred,
+                                                          unsigned long long *__restrict__ stats) {
     const int lane = static_cast<int>(threadIdx.x & 63);
     const uint64_t q = (uint64_t(blockIdx.x) * 4 + (threadIdx.x >> 6)) * 32 + (lane & 31);
     const bool valid = q < n_queries;
@@ -1389,7 +1401,30 @@ __global__ void __launch_bounds__(256) lws_predict_kernel(const JobDev *__restri
                      T.kep_nu0 + T.kep_n * dt, x);
     }
     const float y = lws_forward(net, x, nullptr, 0);
-    if (valid && lane < 32) pred[q] = T.kep_on ? tau_sl + static_cast<double>(y) * kLwsTargetScale : tau_sl;
+    double tau = T.kep_on ? tau_sl + static_cast<double>(y) * kLwsTargetScale : tau_sl;
+    double t = tu - tau, r;
+    int e = -1;
+    for (int k = 1; k <= kColdEvals; ++k) {
+        const double tn = emission_step(T, J.obs, tu, eoff, t, r);
+        const double step = tn - t;
+        t = tn;
+        if (fabs(step) < kSolveTol) {
+            e = k;
+            break;
+        }
+    }
+    if (e > 0) tau = tu - t;
+    const bool mine = valid && lane < 32;
+    if (mine) pred[q] = tau;
+    if (stats) {
+        uint32_t ev = mine ? static_cast<uint32_t>(e < 0 ? kColdEvals : e) : 0u;
+        for (int o = 16; o >= 1; o >>= 1) ev += __shfl_xor(ev, o);
+        const uint32_t fb = __popcll(__ballot(mine && (e < 0 || e > kLearnedEvals)));
+        if (lane == 0) {
+            if (ev) atomicAdd(&stats[0], static_cast<unsigned long long>(ev));
+            if (fb) atomicAdd(&stats[2], static_cast<unsigned long long>(fb));
+        }
+    }
 }
 
 // ---------------------------------------------------------------- truth export
@@ -1581,12 +1616,12 @@ hipError_t launch_render(int form, bool antenna, int tile_shift, bool xcd_map, c
 
 hipError_t launch_lws_predict(const JobDev *jobs, int n_jobs, const LinkDev *links, const TxDev *txs,
                               const LwsNetDev &net, int tile_shift, uint64_t n_queries, double *pred,
-                              hipStream_t stream) {
+                              unsigned long long *stats, hipStream_t stream) {
     if (!n_queries || n_jobs <= 0) return hipSuccess;
     const uint64_t blocks = (n_queries + 127) / 128;
     if (blocks > 0x7FFFFFFFull) return hipErrorInvalidValue;
     hipLaunchKernelGGL(lws_predict_kernel, dim3(uint32_t(blocks)), dim3(256), 0, stream, jobs, n_jobs, links, txs,
-                       net, tile_shift, n_queries, pred);
+                       net, tile_shift, n_queries, pred, stats);
     return hipGetLastError();
 }
 

@@ -7,9 +7,10 @@ C ABI, against the CPU oracles:
   - training on the device lowers the validation error of the predicted
     light time below the straight-line start by orders of magnitude;
   - the converged form is bit-identical to the oracle's converged solve (same
-    clock offsets), and the learned form gives the same IQ with fewer
-    evaluations of the emission equation; a useless network only costs the
-    fallback and changes no output bit.
+    clock offsets), and the learned form gives the same IQ (to the converged
+    solve's tolerance) with one evaluation of the emission equation per
+    contribution; a useless network only costs evaluations at the query
+    points.
 """
 import math
 
@@ -169,7 +170,11 @@ def test_converged_form_matches_oracle(backend):
 
 def test_learned_form_saves_evaluations(backend, trained):
     """Same IQ as the converged form from the reference's start (within the
-    f32 ulp bound), with fewer evaluations of the emission equation."""
+    f32 ulp bound), with one evaluation of the emission equation per
+    contribution: the converged light times at the queries, interpolated,
+    start every MEO sample within about 1e-15 s of its root, so the first
+    update is already below the tolerance. The count includes the queries'
+    own evaluations."""
     net, _ = trained
     seed = 31
     sc = prepare(_meo_scene())
@@ -187,7 +192,7 @@ def test_learned_form_saves_evaluations(backend, trained):
     assert sc_["unconverged"] == 0 and sl["unconverged"] == 0
     cold, learned = sc_["evaluations"] / sc_["contributions"], sl["evaluations"] / sl["contributions"]
     assert cold >= 2.9
-    assert learned <= 2.05
+    assert learned <= 1.05
     assert sl["fallbacks"] <= 1e-4 * sl["contributions"]
     for w in range(len(sc.receivers)):
         compare_iq(ol[w], oc[w], f"learned vs converged, receiver {w}", strict=False)
@@ -198,9 +203,15 @@ def test_learned_form_saves_evaluations(backend, trained):
 
 def test_useless_network_falls_back(backend):
     """A network that predicts light times 100 s off cannot converge within
-    its three evaluations (the first lands on the wrong part of the orbit, the
-    update after it is still ~1e-9 s): the solves fall back to the reference's
-    start and the output matches the converged form's."""
+    three evaluations (the first lands on the wrong part of the orbit, the
+    update after it is still ~1e-9 s): every query's refinement is counted as
+    a fallback and keeps iterating from where it landed (the iteration
+    contracts by about |dr/dt| / c per evaluation). The converged light times
+    at the queries then start every sample as well as a good network would:
+    one evaluation each, fewer in all than the reference's start, and the
+    output matches the converged form's to its tolerance. Both solves stop on
+    an update below 1e-13 s, so they agree to about 1e-13 s * |dr/dt| / c
+    (1e-18 s, an ulp or two of the emission time): close, not bit-identical."""
     net = WarmStartNet(backend, init_seed=6)
     th = np.zeros(_lib.LWS_PARAMS, dtype=np.float32)
     th[L.OFF_B4] = 1e8
@@ -220,13 +231,13 @@ def test_useless_network_falls_back(backend):
         net.close()
     _say(f"useless network: {st}; converged form: {sc_}")
     assert st["contributions"] == sc_["contributions"] > 0
-    assert st["fallbacks"] >= 0.99 * st["contributions"]
     assert st["unconverged"] == 0
-    # each fallback costs its three learned evaluations on top of the cold
-    # solve (a cold solve takes at most 12)
-    rest = st["contributions"] - st["fallbacks"]
-    assert st["evaluations"] >= 3 * st["fallbacks"] + sc_["evaluations"] - 12 * rest
-    compare_iq(gl, gc, "fallback vs converged")
+    # every query falls back: at least one per link and 2 ms of the band
+    n_links = len(sc.receivers[0].bands[0].active)
+    assert st["fallbacks"] >= n_links * math.floor(0.01 / 2e-3)
+    assert st["fallbacks"] < 1e-2 * st["contributions"]
+    assert st["evaluations"] < 1.05 * st["contributions"] < sc_["evaluations"]
+    compare_iq(gl, gc, "fallback vs converged", strict=False)
 
 
 def test_learned_form_without_elements_uses_straight_line(backend, trained):
