@@ -403,6 +403,14 @@ int lpk_lws_generate(lpk_ctx *ctx, uint64_t n, uint64_t seed, float *x_dev, floa
 int lpk_lws_infer(lpk_lws *net, const float *x_dev, uint64_t n, float *y_dev);   /* asynchronous */
 /* Gradient of the mean squared error over the n pairs (synchronous; loss may be NULL). */
 int lpk_lws_gradient(lpk_lws *net, const float *x_dev, const float *y_dev, uint64_t n, float *grad, double *loss);
+/* lpk_lws_gradient without any device allocation: no counterpart in the
+ * reference, for callers that cannot allocate (HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc). Same kernels and results; the
+ * workspace is a static device array of 4.7 MB per device, so n is at most
+ * 8192 pairs (LPK_E_INVALID beyond). Calls that overlap on one device race
+ * on that workspace. Synchronous; loss may be NULL. */
+int lpk_lws_gradient_noalloc(lpk_lws *net, const float *x_dev, const float *y_dev, uint64_t n, float *grad,
+                             double *loss);
 /* Generates the pairs and trains with Adam (beta 0.9 / 0.999, eps 1e-8); synchronous. */
 int lpk_lws_train(lpk_lws *net, const lpk_lws_train_cfg *cfg, lpk_lws_train_report *rep);
 /* The network LPK_FORM_LEARNED predicts with (NULL: none). */

@@ -189,6 +189,7 @@ def load() -> C.CDLL:
         "lpk_lws_generate": (_i32, [C.c_void_p, _u64, _u64, C.c_void_p, C.c_void_p, C.c_void_p]),
         "lpk_lws_infer": (_i32, [C.c_void_p, C.c_void_p, _u64, C.c_void_p]),
         "lpk_lws_gradient": (_i32, [C.c_void_p, C.c_void_p, C.c_void_p, _u64, P(C.c_float), P(_d)]),
+        "lpk_lws_gradient_noalloc": (_i32, [C.c_void_p, C.c_void_p, C.c_void_p, _u64, P(C.c_float), P(_d)]),
         "lpk_lws_train": (_i32, [C.c_void_p, P(LwsTrainCfg), P(LwsTrainReport)]),
         "lpk_set_warm_start": (_i32, [C.c_void_p, C.c_void_p]),
         "lpk_solve_stats": (_i32, [C.c_void_p, P(_u64)]),
@@ -235,7 +236,7 @@ EXPORTED = (
     "lpk_set_tuning", "lpk_get_tuning", "lpk_autotune",
     "lpk_plan_create", "lpk_plan_step", "lpk_plan_info", "lpk_plan_chunk_range", "lpk_plan_output", "lpk_plan_destroy",
     "lpk_lws_create", "lpk_lws_destroy", "lpk_lws_get_params", "lpk_lws_set_params", "lpk_lws_generate",
-    "lpk_lws_infer", "lpk_lws_gradient", "lpk_lws_train", "lpk_set_warm_start", "lpk_solve_stats",
+    "lpk_lws_infer", "lpk_lws_gradient", "lpk_lws_gradient_noalloc", "lpk_lws_train", "lpk_set_warm_start", "lpk_solve_stats",
     "lpk_solve_stats_reset", "lpk_set_horizon_gate", "lpk_horizon_gate_stats",
     "lpk_stream_to_files", "lpk_stream_to_sink", "lpk_sink_fnv1a", "lpk_fnv1a64", "lpk_host_alloc", "lpk_host_free",
     "lpk_truth",

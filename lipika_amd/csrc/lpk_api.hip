@@ -9,6 +9,7 @@
 #include <cstring>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -42,6 +43,44 @@ hipError_t launch_walk_carry(const CarrySegDev *segs, int n_segs, hipStream_t st
 }  // namespace lpk
 
 using namespace lpk;
+
+// Workspace of lpk_lws_gradient_noalloc: kNoAllocRows rows of per-slice
+// partial gradients (kTrainQ pairs each, so at most kNoAllocRows * kTrainQ
+// pairs per call), then the reduced gradient and loss from kNoAllocGradOff
+// (a 256-byte boundary). 4.7 MB, held on every device the library runs on
+// until the process ends. Host code never passes `scratch` itself (a host
+// stand-in): it uses the device address hipGetSymbolAddress gives for the
+// current device.
+constexpr int kNoAllocRows = 128;
+constexpr size_t kNoAllocGradOff = size_t(kNoAllocRows) * kLwsRow;
+constexpr size_t kNoAllocFloats = kNoAllocGradOff + size_t((kLwsRow + 63) / 64) * 64;
+static_assert(kNoAllocGradOff % 64 == 0, "gradient block on a 256-byte boundary");
+static __device__ __align__(256) float scratch[kNoAllocFloats];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+bool g_scratch_done[kMaxDevices] = {};
+hipError_t g_scratch_rc[kMaxDevices] = {};
+float *g_scratch[kMaxDevices] = {};
+
+// Looks up `scratch`'s address on `device` (which must be current) once per
+// device; later calls return what the first found.
+hipError_t scratch_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_done[device]) {
+        void *p = nullptr;
+        const hipError_t rc = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        g_scratch_rc[device] = rc;
+        g_scratch[device] = rc == hipSuccess ? static_cast<float *>(p) : nullptr;
+        g_scratch_done[device] = true;
+    }
+    return g_scratch_rc[device];
+}
+
+}  // namespace
 
 namespace {
 
@@ -1246,6 +1285,9 @@ int lpk_ctx_create(int hip_device, lpk_ctx **out) {
         delete ctx;
         return LPK_E_HIP;
     }
+    // the allocation-free gradient's workspace on this device; a failure is
+    // reported by that entry point, not here
+    (void)scratch_setup(hip_device);
     *out = ctx;
     return LPK_OK;
 }
@@ -1644,6 +1686,34 @@ int lpk_lws_gradient(lpk_lws *net, const float *x_dev, const float *y_dev, uint6
         std::memcpy(grad, h.data(), kLwsParams * sizeof(float));
         if (loss) *loss = double(h[kLwsParams]) / double(n);
     });
+}
+
+int lpk_lws_gradient_noalloc(lpk_lws *net, const float *x_dev, const float *y_dev, uint64_t n, float *grad,
+                             double *loss) {
+    if (!net || !x_dev || !y_dev || !grad || n == 0) return LPK_E_INVALID;
+    lpk_ctx *ctx = net->ctx;
+    const uint64_t rows = (n + kTrainQ - 1) / kTrainQ;
+    if (rows > uint64_t(kNoAllocRows))
+        return fail(ctx, Fail{LPK_E_INVALID, "batch larger than the static gradient workspace (at most " +
+                                                 std::to_string(kNoAllocRows * kTrainQ) + " pairs)"});
+    if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, Fail{LPK_E_HIP, "hipSetDevice failed"});
+    const hipError_t rc = scratch_setup(ctx->device);
+    if (rc != hipSuccess)
+        return fail(ctx, Fail{LPK_E_HIP, std::string("gradient workspace lookup: ") + hipGetErrorString(rc)});
+    float *part = g_scratch[ctx->device];
+    float *g = part + kNoAllocGradOff;
+    float last = 0.0f;
+    hipError_t e = launch_lws_train(lws_net(net), x_dev, y_dev, 0, n, uint32_t(rows), float(2.0 / double(n)), part,
+                                    ctx->stream);
+    if (e == hipSuccess)
+        e = launch_lws_update(part, int(rows), net->theta, net->m1, net->m2, net->packed, 0.f, 0.f, 0.f, 0.f, 0.f,
+                              0.f, g, g + kLwsParams, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(grad, g, kLwsParams * sizeof(float), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&last, g + kLwsParams, sizeof(float), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return fail(ctx, Fail{LPK_E_HIP, std::string("gradient: ") + hipGetErrorString(e)});
+    if (loss) *loss = double(last) / double(n);
+    return LPK_OK;
 }
 
 int lpk_lws_train(lpk_lws *net, const lpk_lws_train_cfg *cfg, lpk_lws_train_report *rep) {
