@@ -176,18 +176,21 @@ struct PostDev {
     float a, b;              // GAIN: g; NONLINEARITY: k; ADC: full scale, lsb
 };
 
-// Transmitter groups of the precision form: a job's links are ordered C/A,
-// framed (BPSK, OOK, QAM16), CW, then everything else (gated transmitters,
-// anchored CW), and each group has a sample loop of its own. The precision
-// form sums exactly, so the order does not change its result; the plain form
-// keeps the reference's order.
-constexpr int kGroupCa = 0, kGroupFramed = 1, kGroupCw = 2, kGroupGeneric = 3;
+// Transmitter groups of the precision form: a job's links are ordered framed
+// (BPSK, OOK, QAM16), CW, everything else (gated transmitters, anchored CW),
+// then C/A, and each group has a sample loop of its own. C/A comes last
+// because its chip reads are the only vector-memory loads of the loops' fast
+// paths: loads complete in order, so a chip read waits for the clock-walk
+// stage of the next tile issued before the loops, which the other groups'
+// work covers. The precision form sums exactly, so the order does not change
+// its result; the plain form keeps the reference's order.
+constexpr int kGroupFramed = 0, kGroupCw = 1, kGroupGeneric = 2, kGroupCa = 3;
 
 struct JobDev {
     uint32_t tile0;          // first tile of this job in the launch
     uint32_t n_tiles;
     uint32_t link0, n_active;
-    uint32_t group_end[3];   // precision form: end of the C/A, framed and CW groups
+    uint32_t group_end[3];   // precision form: end of the framed, CW and other groups
     uint32_t _gpad;
     double fs, lo, epoch;
     uint64_t i0, clen;

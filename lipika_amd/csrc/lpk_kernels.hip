@@ -37,6 +37,7 @@
 // reference's rounding; the precision form asks for its FMAs explicitly.
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <cstdint>
 
 #include "lpk_device.hpp"
@@ -417,6 +418,28 @@ __device__ __forceinline__ double sample_time(double i, double fs, double inv_fs
     return fma(fma(-q, fs, i), inv_fs, q);
 }
 
+// ClockStream::delta_t_at (src/clock/mod.rs:329-346) for the precision form
+// of a band sampled at its receiver's master rate (fs == master): the master
+// index x = (i / fs) * master lies within one ulp of the sample index i, so
+// the interpolation weights are 1 and 0 (or within 1e-16 of them) and the
+// walk value is epsilon(i) itself, read at one index instead of two. Where x
+// rounds to just below i the reference weighs epsilon(i - 1) by about 1e-16;
+// leaving that out moves delta_t by under 1e-27 s. The terms are summed in
+// the reference's order.
+__device__ __forceinline__ double clock_delta_at_master(const JobDev &J, double t, uint64_t i) {
+    const double elapsed = t - 0.0;
+    double flick = 0.0;
+    if (J.flicker) {
+        const double xf = fmax(elapsed, 0.0) * kFlickerRate;
+        const double mf = floor(xf);
+        const int64_t m0 = static_cast<int64_t>(mf);
+        const double fr = xf - mf;
+        flick = J.flicker[m0] * (1.0 - fr) + J.flicker[m0 + 1] * fr;
+    }
+    const double eps = walk_eps_rel(J, static_cast<int32_t>(static_cast<int64_t>(i) - J.eps_k0));
+    return (eps + flick) + (J.c0 + J.y0 * elapsed);
+}
+
 struct TapOut {
     double *t_emit, *range, *phase, *delta_t;
     uint32_t slot, on;
@@ -441,7 +464,8 @@ struct alignas(16) Anchor {
     union {
         struct {                   // written by the fit, read by the sample loop
             double ph0;            // carrier phase at the middle sample, turns in [-1/2, 1/2]
-            double ph_r, ph_d, ph_t;  // its derivatives in turns per metre of range, second of delta_t and of t_local
+            double ph_r, ph_x;     // its derivatives in turns per metre of range and per second of tu
+            double ph_spare;
             double epoch_rel, rate;   // signal epoch; symbol rate (framed) or omega (CW)
             double hu, hu1;        // framed: the symbol index the tile starts in, and the next; C/A: the chip
                                    // offset from cbase where the tile's second NAV bit starts, and the first bit
@@ -453,15 +477,16 @@ struct alignas(16) Anchor {
             int64_t cell[2][2];    // grid nodes of both iterations at the first and last anchors
         };
     };
+    float h0re, h0im;              // value of index hu: symbol (framed) or NAV sign (C/A, h0re)
+    float h1re, h1im;              // value of index hu1 (the four are read as one 16-byte vector)
     float amp_m;                   // (float)(amp_factor / r1_mid)
     float amp_k;                   // amp_m / r1_mid: amplitude slope per metre of range
-    float h0re, h0im;              // value of index hu: symbol (framed) or NAV sign (C/A, h0re)
-    float h1re, h1im;              // value of index hu1
     uint32_t cmod;                 // C/A: cbase mod 1023
     uint32_t tx;                   // transmitter (slow paths read its table)
     double neg_tau_src, tau_if;    // -TAU f_src, TAU (f_src - f_lo), for taps
 };
 static_assert(sizeof(Anchor) == 208, "Anchor layout");
+static_assert(offsetof(Anchor, h0re) % 16 == 0, "hint values are one aligned 16-byte read");
 constexpr int kAnchorGroup = 64;  // transmitters per anchor pass
 constexpr int kSpanRecords = 128; // (tile, transmitter) records of one anchor batch of a span
 constexpr int kSpanSamples = 2;    // adjacent samples per lane of the span kernels
@@ -563,14 +588,13 @@ __device__ __forceinline__ void solve_smooth(const TxDev &T, const double *obs, 
 // carrier phase in turns from the middle sample's, the hardware sine and
 // cosine, the f32 mix, and the exact f64 sum. Slow paths (a tile crossing a
 // grid node, a symbol, NAV bit or chip outside the tile's hints, gates) read
-// the transmitter's table. ddl and dtl are delta_t and t_local less their
-// values at the middle sample. The N samples are independent chains that
-// share the record's reads.
+// the transmitter's table. x is tu less its value at the middle sample, as
+// (t_local - t_local_mid) - (delta_t - delta_t_mid), and gq = -f_lo (t_local -
+// t_local_mid), the band's part of the phase. The N samples are independent chains that share the record's reads.
 template <int G, bool ANT, int N>
 __device__ __forceinline__ void precision_link(const Anchor &A, bool smooth, const TxDev *__restrict__ txs,
                                                const JobDev &J, uint32_t link, const double *tu, const double *x,
-                                               const double *del0, const double *ddl, const double *dtl,
-                                               double *sum_re, double *sum_im) {
+                                               const double *del0, const double *gq, double *sum_re, double *sum_im) {
     static_assert(!ANT || N == 1, "antenna kernels take one sample per lane");
     double r[N];
     if (smooth) {
@@ -607,6 +631,9 @@ __device__ __forceinline__ void precision_link(const Anchor &A, bool smooth, con
     float bre[N], bim[N], chip[N];
     double ph_cw[N], sig_b[N];
     bool hinted[N];
+    // the hint values as one vector read, selected per lane
+    float4 hv = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (G == kGroupCa || G == kGroupFramed) hv = *reinterpret_cast<const float4 *>(&A.h0re);
 #pragma unroll
     for (int q = 0; q < N; ++q) {
         bre[q] = bim[q] = chip[q] = 0.0f;
@@ -631,12 +658,12 @@ __device__ __forceinline__ void precision_link(const Anchor &A, bool smooth, con
             chip[q] = ((gchip *)A.chip0)[off];
             hinted[q] = d >= 0.0 && d < A.hu + kChipsPerNavBit;
             if (!hinted[q]) sig_b[q] = floor(dt * kNavBitRate);
-            bre[q] = d >= A.hu ? A.h1re : A.h0re;
+            bre[q] = d >= A.hu ? hv.z : hv.x;
         } else if (G == kGroupFramed) {
             sig_b[q] = floor((t_emit[q] - A.epoch_rel) * A.rate);
             const bool h0 = sig_b[q] == A.hu;
-            bre[q] = h0 ? A.h0re : A.h1re;
-            bim[q] = h0 ? A.h0im : A.h1im;
+            bre[q] = h0 ? hv.x : hv.z;
+            bim[q] = h0 ? hv.y : hv.w;
             hinted[q] = h0 || sig_b[q] == A.hu1;
         } else if (G == kGroupCw) {
             if (A.rate != 0.0)   // rate holds omega (uniform): a tone at the carrier has no phase of its own
@@ -656,10 +683,13 @@ __device__ __forceinline__ void precision_link(const Anchor &A, bool smooth, con
     }
 #pragma unroll
     for (int q = 0; q < N; ++q) {
-        // the carrier phase in turns (linear in its three inputs), its fraction,
-        // and the hardware sine and cosine of that fraction
+        // the carrier phase in turns, its fraction, and the hardware sine and
+        // cosine of that fraction. The phase -f (r / c + delta_t) + (f - f_lo)
+        // t_local is linear in range, delta_t and t_local; with the offsets
+        // dtl of t_local and ddl of delta_t from the middle sample and
+        // x = dtl - ddl, its change is ph_r dr + f x - f_lo dtl (gq = -f_lo dtl)
         const double dr = r[q] - A.r1_mid;
-        double turns = fma(A.ph_r, dr, fma(A.ph_d, ddl[q], fma(A.ph_t, dtl[q], A.ph0)));
+        double turns = fma(A.ph_r, dr, fma(A.ph_x, x[q], A.ph0 + gq[q]));
         if (G == kGroupGeneric || (G == kGroupCw && A.rate != 0.0)) turns = fma(ph_cw[q], kInvTau, turns);
         const float fr = static_cast<float>(__builtin_amdgcn_fract(turns));
         const float spf = __builtin_amdgcn_sinf(fr), cpf = __builtin_amdgcn_cosf(fr);
@@ -769,8 +799,8 @@ __device__ __forceinline__ bool anchor_fit(Anchor &A, const LinkDev &L, const Tx
     A.tau_if = L.tau_if;
     A.ph0 = carrier_turns(T.neg_tau_src, L.tau_if, r1m, at[1][2], at[1][1]);
     A.ph_r = T.neg_tau_src * kInvTau * kInvC;
-    A.ph_d = T.neg_tau_src * kInvTau;
-    A.ph_t = L.tau_if * kInvTau;
+    A.ph_x = -T.neg_tau_src * kInvTau;
+    A.ph_spare = 0.0;
     A.epoch_rel = T.epoch_rel;
     A.rate = T.signal_kind == 0 ? T.omega : T.rate;
     A.chip0 = T.ca_chips2;
@@ -819,28 +849,28 @@ template <bool ANT, int N>
 __device__ __forceinline__ void precision_links(const Anchor *R, uint64_t smask, uint64_t gmask,
                                                 const TxDev *__restrict__ txs, const JobDev &J, uint32_t g0,
                                                 uint32_t ng, const double *tu, const double *x, const double *del0,
-                                                const double *ddl, const double *dtl, double *sum_re,
+                                                const double *gq, double *sum_re,
                                                 double *sum_im) {
-    const uint32_t e_ca = J.group_end[kGroupCa] > g0 ? min(J.group_end[kGroupCa] - g0, ng) : 0u;
     const uint32_t e_fr = J.group_end[kGroupFramed] > g0 ? min(J.group_end[kGroupFramed] - g0, ng) : 0u;
     const uint32_t e_cw = J.group_end[kGroupCw] > g0 ? min(J.group_end[kGroupCw] - g0, ng) : 0u;
+    const uint32_t e_gn = J.group_end[kGroupGeneric] > g0 ? min(J.group_end[kGroupGeneric] - g0, ng) : 0u;
     uint32_t s = 0;
-    for (; s < e_ca; ++s)
-        if (!ANT || !((gmask >> s) & 1u))
-            precision_link<kGroupCa, ANT, N>(R[s], (smask >> s) & 1u, txs, J, g0 + s, tu, x, del0, ddl, dtl, sum_re,
-                                             sum_im);
     for (; s < e_fr; ++s)
         if (!ANT || !((gmask >> s) & 1u))
-            precision_link<kGroupFramed, ANT, N>(R[s], (smask >> s) & 1u, txs, J, g0 + s, tu, x, del0, ddl, dtl,
+            precision_link<kGroupFramed, ANT, N>(R[s], (smask >> s) & 1u, txs, J, g0 + s, tu, x, del0, gq,
                                                  sum_re, sum_im);
     for (; s < e_cw; ++s)
         if (!ANT || !((gmask >> s) & 1u))
-            precision_link<kGroupCw, ANT, N>(R[s], (smask >> s) & 1u, txs, J, g0 + s, tu, x, del0, ddl, dtl, sum_re,
+            precision_link<kGroupCw, ANT, N>(R[s], (smask >> s) & 1u, txs, J, g0 + s, tu, x, del0, gq, sum_re,
                                              sum_im);
+    for (; s < e_gn; ++s)
+        if (!ANT || !((gmask >> s) & 1u))
+            precision_link<kGroupGeneric, ANT, N>(R[s], (smask >> s) & 1u, txs, J, g0 + s, tu, x, del0, gq,
+                                                  sum_re, sum_im);
     for (; s < ng; ++s)
         if (!ANT || !((gmask >> s) & 1u))
-            precision_link<kGroupGeneric, ANT, N>(R[s], (smask >> s) & 1u, txs, J, g0 + s, tu, x, del0, ddl, dtl,
-                                                  sum_re, sum_im);
+            precision_link<kGroupCa, ANT, N>(R[s], (smask >> s) & 1u, txs, J, g0 + s, tu, x, del0, gq, sum_re,
+                                             sum_im);
 }
 
 // RxChain::apply_post_sum (src/rx/mod.rs:142-146 and :83-101) and the
@@ -895,6 +925,8 @@ __device__ __forceinline__ void precision_span(const JobDev &J, uint32_t span, c
     const uint32_t na = J.n_active;
     const int kb_max = na == 0 ? kSub : (na >= uint32_t(kAnchorGroup) ? 1 : min(kSub, kSpanRecords / int(na)));
     const bool noisy = J.sigma > 0.0f;
+    // a band sampled at its receiver's master rate reads one walk value per sample
+    const bool at_master = J.walk && J.walk_rel && !J.delta_t && J.fs == J.master;
     for (int k0 = 0; k0 < kSub; k0 += kb_max) {
         const int kb = min(kb_max, kSub - k0);
         double sum_re[kSpanSamples] = {}, sum_im[kSpanSamples] = {};
@@ -957,38 +989,49 @@ __device__ __forceinline__ void precision_span(const JobDev &J, uint32_t span, c
                 uint64_t i[kSpanSamples];
                 bool valid[kSpanSamples];
                 double t_local[kSpanSamples], dlt[kSpanSamples], tu[kSpanSamples], x[kSpanSamples];
-                double del0[kSpanSamples], ddl[kSpanSamples], dtl[kSpanSamples];
+                double del0[kSpanSamples], gq[kSpanSamples];
 #pragma unroll
                 for (int q = 0; q < kSpanSamples; ++q) {
                     i[q] = t0 + kSpanSamples * threadIdx.x + q;
                     valid[q] = i[q] >= J.i0 && i[q] < jend;
                     const uint64_t ic = valid[q] ? i[q] : first;
                     t_local[q] = sample_time(static_cast<double>(ic), J.fs, J.inv_fs);
-                    dlt[q] = J.delta_t ? J.delta_t[ic - J.i0] : clock_delta(J, t_local[q]);
+                    dlt[q] = J.delta_t ? J.delta_t[ic - J.i0]
+                                       : (at_master ? clock_delta_at_master(J, t_local[q], ic)
+                                                    : clock_delta(J, t_local[q]));
                     tu[q] = t_local[q] - dlt[q];
-                    x[q] = tu[q] - s_at[kk][1][0];
+                    // tu less its value at the middle sample, formed as the difference of
+                    // the exact offsets of t_local and delta_t (the phase must not carry the
+                    // rounding of tu itself; the fitted range does not notice it)
+                    x[q] = (t_local[q] - s_at[kk][1][1]) - (dlt[q] - s_at[kk][1][2]);
                     // rounding of the iteration-1 query time, exact (Sterbenz) per sample
                     const double t0q = tu[q] - kEmissionInitialLag;
                     del0[q] = ((J.epoch + t0q) - J.epoch) - t0q;
-                    // the phase's per-sample inputs less the middle sample's
-                    ddl[q] = dlt[q] - s_at[kk][1][2];
-                    dtl[q] = t_local[q] - s_at[kk][1][1];
+                    // the band's part of the carrier phase from the middle sample's
+                    gq[q] = -J.lo * (t_local[q] - s_at[kk][1][1]);
                 }
                 const uint64_t smask = uniform_u64(s_mask[kk][0], s_mask[kk][1]);
                 const Anchor *R = s_rec + kk * ng;
-                precision_links<false, kSpanSamples>(R, smask, 0ull, txs, J, g0, ng, tu, x, del0, ddl, dtl, sum_re,
+                precision_links<false, kSpanSamples>(R, smask, 0ull, txs, J, g0, ng, tu, x, del0, gq, sum_re,
                                                      sum_im);
-                if (tap.on && tap.slot >= g0 && tap.slot < g0 + ng) {
-                    const uint32_t ts = tap.slot - g0;
+                if (tap.on) {
+                    // the sample times and clock offsets again (as above), for the taps
 #pragma unroll
                     for (int q = 0; q < kSpanSamples; ++q) {
                         if (!valid[q]) continue;
-                        double te, r, ph;
-                        precision_tap(R[ts], (smask >> ts) & 1u, txs, J, tu[q], t_local[q], dlt[q], x[q], del0[q],
-                                      te, r, ph);
-                        tap.t_emit[i[q] - J.i0] = te;
-                        tap.range[i[q] - J.i0] = r;
-                        tap.phase[i[q] - J.i0] = ph;
+                        const double tl = sample_time(static_cast<double>(i[q]), J.fs, J.inv_fs);
+                        const double dl = J.delta_t ? J.delta_t[i[q] - J.i0]
+                                                    : (at_master ? clock_delta_at_master(J, tl, i[q])
+                                                                 : clock_delta(J, tl));
+                        if (last_group) tap.delta_t[i[q] - J.i0] = dl;
+                        if (tap.slot >= g0 && tap.slot < g0 + ng) {
+                            const uint32_t ts = tap.slot - g0;
+                            double te, r, ph;
+                            precision_tap(R[ts], (smask >> ts) & 1u, txs, J, tu[q], tl, dl, x[q], del0[q], te, r, ph);
+                            tap.t_emit[i[q] - J.i0] = te;
+                            tap.range[i[q] - J.i0] = r;
+                            tap.phase[i[q] - J.i0] = ph;
+                        }
                     }
                 }
                 if (last_group) {
@@ -998,7 +1041,6 @@ __device__ __forceinline__ void precision_span(const JobDev &J, uint32_t span, c
                     if (noisy) philox_words((t0 >> 1) + threadIdx.x, J.noise_seed, kAwgnTag, w);
 #pragma unroll
                     for (int q = 0; q < kSpanSamples; ++q) {
-                        if (tap.on && valid[q]) tap.delta_t[i[q] - J.i0] = dlt[q];
                         float acc_re = static_cast<float>(sum_re[q]), acc_im = static_cast<float>(sum_im[q]);
                         if (noisy) {
                             float zr, zi;
@@ -1277,12 +1319,14 @@ __global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM == 1 ? 4 : 6))
             }
         }
         __syncthreads();
-        const double x = tu - s_anc_t[1][0];
+        // tu less its value at the middle sample, from the exact offsets of
+        // t_local and delta_t
+        const double x = (t_local - s_anc_t[1][1]) - (dlt - s_anc_t[1][2]);
         // rounding of the iteration-1 query time, exact (Sterbenz) per sample
         const double t0q = tu - kEmissionInitialLag;
         const double del0 = ((J.epoch + t0q) - J.epoch) - t0q;
-        // the phase's per-sample inputs less the middle sample's
-        const double ddl = dlt - s_anc_t[1][2], dtl = t_local - s_anc_t[1][1];
+        // the band's part of the carrier phase from the middle sample's
+        const double gq = -J.lo * (t_local - s_anc_t[1][1]);
         const bool fit_ok = a_last - a_first >= 2;
         // the group's smooth and gated links as bits (wave 0 holds the group)
         __shared__ uint32_t s_mask[4];
@@ -1320,7 +1364,7 @@ __global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM == 1 ? 4 : 6))
             __syncthreads();
             const uint64_t smask = uniform_u64(s_mask[0], s_mask[1]);
             const uint64_t gmask = uniform_u64(s_mask[2], s_mask[3]);
-            precision_links<ANT, 1>(s_anc, smask, gmask, txs, J, g0, ng, &tu, &x, &del0, &ddl, &dtl, &sum_re, &sum_im);
+            precision_links<ANT, 1>(s_anc, smask, gmask, txs, J, g0, ng, &tu, &x, &del0, &gq, &sum_re, &sum_im);
             if (tap.on && valid && tap.slot >= g0 && tap.slot < g0 + ng) {
                 const uint32_t ts = tap.slot - g0;
                 double te, r, ph;
