@@ -125,6 +125,43 @@ __device__ __forceinline__ void solve_emission(const TxDev &T, const double *obs
     t_emit = t;
 }
 
+// solve_emission for the precision form's tiles that cross a trajectory grid
+// node: the same iteration with the grid coordinate scaled by 1 / dt instead
+// of divided by dt (one rounding more in the coordinate, under 1e-13 m of
+// position; the lerp is continuous across a node, so a coordinate that lands
+// on the other side of one moves nothing).
+__device__ __forceinline__ double solve_emission_scaled(const TxDev &T, const double *obs, double tu, double epoch) {
+    double t = tu - kEmissionInitialLag, r = 0.0;
+#pragma unroll
+    for (int it = 0; it < kNewtonIterations; ++it) {
+        double sx, sy, sz;
+        if (T.traj_kind == 0) {
+            sx = T.pos[0];
+            sy = T.pos[1];
+            sz = T.pos[2];
+        } else {
+            const double f = ((epoch + t) - T.grid_t0) * T.grid_inv_dt;
+            const double flo = fmin(fmax(floor(f), 0.0), T.grid_hi);
+            const int32_t lo = static_cast<int32_t>(flo);
+            double frac = f - flo;
+            frac = frac < 0.0 ? 0.0 : (frac > 1.0 ? 1.0 : frac);
+            const double omf = 1.0 - frac;
+            const double *g = T.grid;
+            const int64_t n = T.grid_n;
+            sx = g[lo] * omf + g[lo + 1] * frac;
+            sy = g[n + lo] * omf + g[n + lo + 1] * frac;
+            sz = g[2 * n + lo] * omf + g[2 * n + lo + 1] * frac;
+        }
+        const double wtau = kEarthRotation * (tu - t);
+        const double dx = (sx + wtau * sy) - obs[0];
+        const double dy = (sy - wtau * sx) - obs[1];
+        const double dz = sz - obs[2];
+        r = sqrt(dx * dx + dy * dy + dz * dz);
+        t = tu - r * kInvC;
+    }
+    return r;
+}
+
 // PrecomputedTrajectory::position_ecef (src/geometry/precomputed.rs:108-121)
 // at time trel after the grid's node 0, given directly instead of as an
 // absolute time, so it is not rounded at the scale of the epoch, and scaled
@@ -607,11 +644,7 @@ __device__ __forceinline__ void precision_link(const Anchor &A, bool smooth, con
         }
     } else {
 #pragma unroll
-        for (int q = 0; q < N; ++q) {
-            double te;
-            int64_t c0, c1;
-            solve_emission(txs[A.tx], J.obs, tu[q], J.epoch, te, r[q], c0, c1);
-        }
+        for (int q = 0; q < N; ++q) r[q] = solve_emission_scaled(txs[A.tx], J.obs, tu[q], J.epoch);
     }
     double t_emit[N];
 #pragma unroll
@@ -708,8 +741,9 @@ __device__ __forceinline__ void precision_link(const Anchor &A, bool smooth, con
             cre = ab * cpf;
             cim = ab * spf;
         } else {
-            cre = amplitude * (bre[q] * cpf - bim[q] * spf);
-            cim = amplitude * (bre[q] * spf + bim[q] * cpf);
+            const float ar = amplitude * bre[q], ai = amplitude * bim[q];
+            cre = fmaf(ar, cpf, -(ai * spf));
+            cim = fmaf(ar, spf, ai * cpf);
         }
         // the f32 contributions summed exactly in f64
         sum_re[q] += static_cast<double>(cre);
@@ -729,8 +763,7 @@ __device__ __forceinline__ void precision_tap(const Anchor &A, bool smooth, cons
         const double del1 = ((J.epoch + t1) - J.epoch) - t1;
         r = fma(A.rd1q, del1, fma(-A.k01, del0, fma(x, fma(A.q1c2, x, A.q1d1), A.r1m)));
     } else {
-        int64_t c0, c1;
-        solve_emission(txs[A.tx], J.obs, tu, J.epoch, te, r, c0, c1);
+        r = solve_emission_scaled(txs[A.tx], J.obs, tu, J.epoch);
     }
     te = fma(-r, kInvC, tu);
     ph = A.neg_tau_src * (r * kInvC + dlt) + A.tau_if * t_local;
@@ -786,10 +819,8 @@ __device__ __forceinline__ bool anchor_fit(Anchor &A, const LinkDev &L, const Tx
         const double d1m = ((J.epoch + t1m) - J.epoch) - t1m;
         r1m = fma(A.rd1q, d1m, fma(-A.k01, d0m, r1s[1]));
     } else {
-        // the exact solve at the middle sample (the lanes solve exactly too)
-        double te;
-        int64_t c0, c1;
-        solve_emission(T, J.obs, at[1][0], J.epoch, te, r1m, c0, c1);
+        // the exact solve at the middle sample, as the lanes solve
+        r1m = solve_emission_scaled(T, J.obs, at[1][0], J.epoch);
     }
     A.r1_mid = r1m;
     const float amp_m = static_cast<float>(T.amp_factor / r1m);
