@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define LPK_ABI_VERSION 3
+#define LPK_ABI_VERSION 4
 
 typedef enum {
     LPK_OK = 0,
@@ -124,16 +124,28 @@ typedef struct {
     double kepler_a_m, kepler_inc_deg, kepler_raan_deg, kepler_nu0_deg, kepler_epoch_unix;
 } lpk_tx_desc;
 
-/* One stationary receiver with its clock (src/pipeline.rs:344-364,
- * NodeClockSpec src/clock/mod.rs:20-46). */
+/* Receiver trajectory kinds (src/pipeline.rs:77-107 as Receiver::from_config
+ * builds them, :352-364). */
+enum lpk_rx_traj_kind {
+    LPK_RX_STATIONARY = 0,       /* fixed at pos_ecef (src/geometry/stationary.rs:21-39) */
+    LPK_RX_KEPLER = 1            /* CircularKeplerian (src/geometry/kepler.rs:19-96), evaluated on the
+                                  * device at every sample's epoch + t_utc, as position_ecef_batch is
+                                  * queried in the chunk body (src/pipeline.rs:929-930) */
+};
+
+/* One receiver with its clock (src/pipeline.rs:344-364, NodeClockSpec
+ * src/clock/mod.rs:20-46). */
 typedef struct {
-    double pos_ecef[3];
+    double pos_ecef[3];          /* LPK_RX_STATIONARY: the position */
     double allan_deviation_1s;   /* white-FM sigma_y(1 s); 0 = no walk */
     double frequency_offset_y0;
     double initial_time_offset_s;
     double master_rate_hz;       /* highest band rate of the receiver, src/pipeline.rs:864-866 */
     uint64_t clock_seed;         /* src/pipeline.rs:825-827 */
     double flicker_floor;        /* flicker-FM Allan floor (src/clock/mod.rs:26-30); 0 = none */
+    int32_t traj_kind, _pad;     /* lpk_rx_traj_kind */
+    /* LPK_RX_KEPLER: the elements, as TrajectoryConfig::CircularKeplerian holds them */
+    double kepler_a_m, kepler_inc_deg, kepler_raan_deg, kepler_nu0_deg, kepler_epoch_unix;
 } lpk_rx_desc;
 
 typedef struct {
@@ -280,6 +292,10 @@ uint64_t lpk_grid_size(double duration_s, double grid_dt_s);
 int lpk_grid_fill_kepler(double a_m, double inc_deg, double raan_deg, double nu0_deg,
                          double orbit_epoch_unix, double scene_epoch_unix, double duration_s,
                          double grid_dt_s, double *t0_out, double *xyz, uint64_t n);
+/* CircularKeplerian::position_ecef (src/geometry/kepler.rs:27-85) at t_unix:
+ * the position a Keplerian receiver has at that time. */
+void lpk_kepler_position(double a_m, double inc_deg, double raan_deg, double nu0_deg, double orbit_epoch_unix,
+                         double t_unix, double out[3]);
 /* src/signal/ca_code.rs:66-78 */
 int lpk_ca_code(int prn, int8_t out[1023]);
 /* src/signal/ca_code.rs:146-151 */
@@ -468,8 +484,9 @@ int lpk_host_free(lpk_ctx *ctx, void *host);
 /* export_truth's geometry (src/pipeline.rs:548-611) for receiver rx over the
  * epoch-relative grid t_rel[n]: range[q * n + k] of every uploaded transmitter
  * q from solve_emission_times with `iterations` iterations (the reference uses
- * 4), and src_xyz[(q * 3 + c) * n + k] the source position at epoch + t_rel[k].
- * Synchronous; host arrays. */
+ * 4) against the receiver's position at epoch + t_rel[k] (its trajectory's, as
+ * :562-563 queries it), and src_xyz[(q * 3 + c) * n + k] the source position
+ * at epoch + t_rel[k]. Synchronous; host arrays. */
 int lpk_truth(lpk_ctx *ctx, uint32_t rx, const double *t_rel, uint64_t n, double epoch_unix, int iterations,
               double *range, double *src_xyz);
 

@@ -12,7 +12,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblpk.so")
 
-ABI_VERSION = 3  # LPK_ABI_VERSION of the header these structures mirror
+ABI_VERSION = 4  # LPK_ABI_VERSION of the header these structures mirror
 LPK_OK = 0
 STATUS = {
     0: "LPK_OK", -1: "LPK_E_INVALID", -2: "LPK_E_NOMEM", -3: "LPK_E_HIP",
@@ -21,6 +21,7 @@ STATUS = {
 SIG_CW, SIG_BPSK, SIG_OOK, SIG_QAM16, SIG_CA = range(5)
 NAV_OFF, NAV_DUMMY, NAV_FULL = range(3)
 TRAJ_STATIONARY, TRAJ_GRID = range(2)
+RX_STATIONARY, RX_KEPLER = range(2)
 POST_GAIN, POST_NONLINEARITY, POST_ADC = range(3)
 FORM_PLAIN, FORM_PRECISION, FORM_CONVERGED, FORM_LEARNED = 0, 1, 2, 3
 LWS_PARAMS = 9089
@@ -80,6 +81,9 @@ class RxDesc(C.Structure):
         ("allan_deviation_1s", _d), ("frequency_offset_y0", _d),
         ("initial_time_offset_s", _d), ("master_rate_hz", _d),
         ("clock_seed", _u64), ("flicker_floor", _d),
+        ("traj_kind", _i32), ("_pad", _i32),
+        ("kepler_a_m", _d), ("kepler_inc_deg", _d), ("kepler_raan_deg", _d), ("kepler_nu0_deg", _d),
+        ("kepler_epoch_unix", _d),
     ]
 
 
@@ -173,6 +177,7 @@ def load() -> C.CDLL:
         "lpk_grid_size": (_u64, [_d, _d]),
         "lpk_grid_fill_kepler": (_i32, [_d, _d, _d, _d, _d, _d, _d, _d, P(_d), P(_d), _u64]),
         "lpk_ca_code": (_i32, [C.c_int, P(C.c_int8)]),
+        "lpk_kepler_position": (None, [_d, _d, _d, _d, _d, _d, P(_d)]),
         "lpk_nav_fold_seed": (_u64, [_u64, _u64]),
         "lpk_payload_bpsk": (_i32, [P(C.c_int8), _u64, _u64, _u64, P(C.c_int8)]),
         "lpk_payload_ook": (_i32, [P(C.c_uint8), _u64, _u64, _u64, P(C.c_int8)]),
@@ -230,7 +235,7 @@ EXPORTED = (
     "lpk_scene_upload", "lpk_render_chunk", "lpk_sync", "lpk_clock_rewind", "lpk_render_taps",
     "lpk_last_kernel_ms", "lpk_timing_reset", "lpk_timing_read", "lpk_device_alloc", "lpk_device_free", "lpk_copy_to_host",
     "lpk_device_fnv1a", "lpk_geodetic_to_ecef", "lpk_grid_size", "lpk_grid_fill_kepler",
-    "lpk_ca_code", "lpk_nav_fold_seed", "lpk_payload_bpsk", "lpk_payload_ook",
+    "lpk_ca_code", "lpk_kepler_position", "lpk_nav_fold_seed", "lpk_payload_bpsk", "lpk_payload_ook",
     "lpk_payload_qam16", "lpk_elapsed_periods", "lpk_frame_anchor", "lpk_cw_anchor_phase",
     "lpk_bands_overlap", "lpk_thermal_noise_power_w",
     "lpk_set_tuning", "lpk_get_tuning", "lpk_autotune",

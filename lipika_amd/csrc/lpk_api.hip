@@ -685,11 +685,18 @@ void validate_job(const lpk_ctx *ctx, const lpk_band_chunk &B, uint32_t q) {
 }
 
 // The precision form takes at most kPrecisionMaxLinks transmitters per job.
-void check_form_links(int form, const lpk_band_chunk &B, uint32_t q) {
+// Its fitted solve and the learned form's features assume a receiver fixed
+// on the Earth; a moving receiver renders in the plain or converged form.
+void check_form_links(const lpk_ctx *ctx, int form, const lpk_band_chunk &B, uint32_t q) {
     if (form == LPK_FORM_PRECISION && B.n_active > kPrecisionMaxLinks)
         throw Fail{LPK_E_INVALID, "job " + std::to_string(q) + ": the precision form takes at most " +
                                       std::to_string(kPrecisionMaxLinks) + " transmitters per band-chunk (" +
                                       std::to_string(B.n_active) + " given); use the plain form"};
+    if ((form == LPK_FORM_PRECISION || form == LPK_FORM_LEARNED) && B.rx < ctx->rx.size() &&
+        ctx->rx[B.rx].d.traj_kind != LPK_RX_STATIONARY)
+        throw Fail{LPK_E_INVALID, "job " + std::to_string(q) + ": the " +
+                                      std::string(form == LPK_FORM_PRECISION ? "precision" : "learned") +
+                                      " form takes stationary receivers; use the plain or converged form"};
 }
 
 // The window-relative clock lookups of a job whose window starts at
@@ -742,6 +749,20 @@ JobDev fill_job(const lpk_ctx *ctx, const lpk_band_chunk &B, uint64_t i0, uint64
     J.i0 = i0;
     J.clen = clen;
     for (int k = 0; k < 3; ++k) J.obs[k] = R.d.pos_ecef[k];
+    if (R.d.traj_kind == LPK_RX_KEPLER) {
+        // CircularKeplerian (src/geometry/kepler.rs:27-45): the per-receiver
+        // constants in the reference's operation order, with glibc's sincos
+        // for the inclination and node as its sin_cos pairs them
+        const double deg = 3.141592653589793 / 180.0;   // f64::to_radians
+        const double a = R.d.kepler_a_m;
+        J.rx_kind = 1;
+        J.rk_a = a;
+        J.rk_n = std::sqrt(kGmEarth / (a * a * a));
+        J.rk_nu0 = R.d.kepler_nu0_deg * deg;
+        sincos(R.d.kepler_inc_deg * deg, &J.rk_si, &J.rk_ci);
+        sincos(R.d.kepler_raan_deg * deg, &J.rk_so, &J.rk_co);
+        J.rk_epoch = R.d.kepler_epoch_unix;
+    }
     J.c0 = R.d.initial_time_offset_s;
     J.y0 = R.d.frequency_offset_y0;
     J.master = R.d.master_rate_hz;
@@ -809,7 +830,7 @@ void render(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form,
     for (uint32_t q = 0; q < n_jobs; ++q) {
         const lpk_band_chunk &B = jobs[q];
         validate_job(ctx, B, q);
-        check_form_links(form, B, q);
+        check_form_links(ctx, form, B, q);
         if (B.unoccluded && B.n_active) want_flags = true;
         if (B.has_antenna) any_antenna = true;
         total_links += B.n_active;
@@ -1119,7 +1140,7 @@ lpk_plan *plan_create(lpk_ctx *ctx, const lpk_band_chunk *bands, uint32_t n_band
     for (uint32_t b = 0; b < n_bands; ++b) {
         lpk_band_chunk &B = P->bands[b];
         validate_job(ctx, B, b);
-        check_form_links(form, B, b);
+        check_form_links(ctx, form, B, b);
         if (B.i0 != 0 || B.delta_t || B.out_host || B.adc_codes_host || B.unoccluded)
             throw Fail{LPK_E_INVALID, "band " + std::to_string(b) +
                                           ": a plan band starts at sample 0, uses the device clock and has "
@@ -1336,8 +1357,17 @@ int lpk_scene_upload(lpk_ctx *ctx, const lpk_tx_desc *tx, uint32_t n_tx, const l
         ctx->rx.resize(n_rx);
         for (uint32_t r = 0; r < n_rx; ++r) {
             const lpk_rx_desc &d = rx[r];
-            for (int k = 0; k < 3; ++k)
-                if (!is_fin(d.pos_ecef[k])) throw Fail{LPK_E_NONFINITE, "receiver position"};
+            if (d.traj_kind == LPK_RX_STATIONARY) {
+                for (int k = 0; k < 3; ++k)
+                    if (!is_fin(d.pos_ecef[k])) throw Fail{LPK_E_NONFINITE, "receiver position"};
+            } else if (d.traj_kind == LPK_RX_KEPLER) {
+                if (!is_fin(d.kepler_a_m) || !is_fin(d.kepler_inc_deg) || !is_fin(d.kepler_raan_deg) ||
+                    !is_fin(d.kepler_nu0_deg) || !is_fin(d.kepler_epoch_unix))
+                    throw Fail{LPK_E_NONFINITE, "receiver " + std::to_string(r) + ": Keplerian elements"};
+                if (!(d.kepler_a_m > 0.0)) throw Fail{LPK_E_INVALID, "receiver semi-major axis must be positive"};
+            } else {
+                throw Fail{LPK_E_INVALID, "receiver " + std::to_string(r) + ": unknown trajectory kind"};
+            }
             if (!(d.master_rate_hz > 0.0) || !is_fin(d.master_rate_hz))
                 throw Fail{LPK_E_INVALID, "receiver master_rate_hz must be positive"};
             if (!is_fin(d.allan_deviation_1s) || d.allan_deviation_1s < 0.0 || !is_fin(d.frequency_offset_y0) ||
@@ -1484,7 +1514,7 @@ int lpk_autotune(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int 
             if (jobs[q].out_host || jobs[q].adc_codes_host || jobs[q].unoccluded)
                 throw Fail{LPK_E_INVALID, "autotune jobs keep their output in device memory"};
             validate_job(ctx, jobs[q], q);
-            check_form_links(form, jobs[q], q);
+            check_form_links(ctx, form, jobs[q], q);
             samples += jobs[q].clen;
         }
         if (!samples) throw Fail{LPK_E_INVALID, "autotune needs at least one job with samples"};
@@ -1971,19 +2001,31 @@ int lpk_truth(lpk_ctx *ctx, uint32_t rx, const double *t_rel, uint64_t n, double
         if (rx >= ctx->rx.size()) throw Fail{LPK_E_INVALID, "receiver index out of range"};
         const uint64_t nt = ctx->n_tx;
         if (!n || !nt) return;
-        DevBuf tb, rb, sb;
+        DevBuf tb, rb, sb, ob;
         struct Rel {
-            DevBuf *b[3];
+            DevBuf *b[4];
             ~Rel() {
                 for (DevBuf *q : b) q->release();
             }
-        } rel{{&tb, &rb, &sb}};
+        } rel{{&tb, &rb, &sb, &ob}};
+        // the receiver at every grid time, as position_ecef_batch of its
+        // trajectory at epoch + t (src/pipeline.rs:562-563)
+        const lpk_rx_desc &D = ctx->rx[rx].d;
+        std::vector<double> obs(3 * n);
+        for (uint64_t k = 0; k < n; ++k) {
+            if (D.traj_kind == LPK_RX_KEPLER)
+                lpk_kepler_position(D.kepler_a_m, D.kepler_inc_deg, D.kepler_raan_deg, D.kepler_nu0_deg,
+                                    D.kepler_epoch_unix, epoch_unix + t_rel[k], &obs[3 * k]);
+            else
+                for (int a = 0; a < 3; ++a) obs[3 * k + a] = D.pos_ecef[a];
+        }
         double *td = static_cast<double *>(tb.ensure(n * 8, ctx->stream));
         double *rd = static_cast<double *>(rb.ensure(nt * n * 8, ctx->stream));
         double *sd = static_cast<double *>(sb.ensure(nt * 3 * n * 8, ctx->stream));
+        double *od = static_cast<double *>(ob.ensure(3 * n * 8, ctx->stream));
         hip_check(hipMemcpyAsync(td, t_rel, n * 8, hipMemcpyHostToDevice, ctx->stream), "truth grid upload");
-        hip_check(launch_truth(ctx->tx_dev, uint32_t(nt), ctx->rx[rx].d.pos_ecef, epoch_unix, td, n, iterations, rd, sd,
-                               ctx->stream),
+        hip_check(hipMemcpyAsync(od, obs.data(), 3 * n * 8, hipMemcpyHostToDevice, ctx->stream), "truth grid upload");
+        hip_check(launch_truth(ctx->tx_dev, uint32_t(nt), od, epoch_unix, td, n, iterations, rd, sd, ctx->stream),
                   "truth launch");
         hip_check(hipMemcpyAsync(range, rd, nt * n * 8, hipMemcpyDeviceToHost, ctx->stream), "truth download");
         hip_check(hipMemcpyAsync(src_xyz, sd, nt * 3 * n * 8, hipMemcpyDeviceToHost, ctx->stream), "truth download");

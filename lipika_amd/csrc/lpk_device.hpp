@@ -194,7 +194,12 @@ struct JobDev {
     uint32_t _gpad;
     double fs, lo, epoch;
     uint64_t i0, clen;
-    double obs[3];
+    double obs[3];           // stationary receiver position
+    // moving receiver (rx_kind 1, CircularKeplerian, src/geometry/kepler.rs:27-96):
+    // radius, mean motion, true anomaly at its epoch (rad), sin and cos of the
+    // inclination and of the node, and the elements' epoch (unix s)
+    int32_t rx_kind, _rxpad;
+    double rk_a, rk_n, rk_nu0, rk_si, rk_ci, rk_so, rk_co, rk_epoch;
     // clock
     const double *delta_t;   // host-provided offsets (device copy) or null
     const double *walk_base; // per 65 536-index block, absolute block index

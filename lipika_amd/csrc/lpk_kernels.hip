@@ -99,6 +99,38 @@ __device__ __forceinline__ void tx_position(const TxDev &T, double t_abs, double
     cell = lo;
 }
 
+// The receiver's position at reception time tu (epoch-relative), queried at
+// the absolute time epoch + tu as the chunk body queries position_ecef_batch
+// (src/pipeline.rs:929-930): fixed (src/geometry/stationary.rs:21-24), or
+// CircularKeplerian::position_ecef (src/geometry/kepler.rs:27-85, eci_at and
+// eci_to_ecef in the reference's operation order; the device's f64 sincos is
+// within an ulp of glibc's).
+__device__ __forceinline__ void rx_position(const JobDev &J, double tu, double *o) {
+    if (J.rx_kind == 0) {
+        o[0] = J.obs[0];
+        o[1] = J.obs[1];
+        o[2] = J.obs[2];
+        return;
+    }
+    const double t_unix = J.epoch + tu;
+    const double dt = t_unix - J.rk_epoch;
+    const double nu = J.rk_nu0 + J.rk_n * dt;
+    double s_nu, c_nu;
+    sincos(nu, &s_nu, &c_nu);
+    const double xp = J.rk_a * c_nu;
+    const double yp = J.rk_a * s_nu;
+    const double y1 = yp * J.rk_ci;
+    const double z1 = yp * J.rk_si;
+    const double x_eci = xp * J.rk_co - y1 * J.rk_so;
+    const double y_eci = xp * J.rk_so + y1 * J.rk_co;
+    const double theta = kEarthRotation * (t_unix - J.rk_epoch);
+    double s, c;
+    sincos(theta, &s, &c);
+    o[0] = x_eci * c + y_eci * s;
+    o[1] = -x_eci * s + y_eci * c;
+    o[2] = z1;
+}
+
 // solve_emission_times_for, one sample (src/channel/mod.rs:70-99). cell0 and
 // cell1 are the grid nodes the two iterations' queries fell in.
 __device__ __forceinline__ void solve_emission(const TxDev &T, const double *obs, double tu,
@@ -230,12 +262,13 @@ __device__ __forceinline__ int solve_converged_tail(const TxDev &T, const double
 // certainly_below_cutoff (src/channel/horizon.rs:150-210) for a stationary
 // receiver (ball radius 0 around O) over reception times [t_min, t_max]
 // (epoch-relative). True only if every sample of that span would get zero gain.
-__device__ __forceinline__ bool gate_below_cutoff(const JobDev &J, const TxDev &T, double t_min, double t_max) {
+__device__ __forceinline__ bool gate_below_cutoff(const JobDev &J, const TxDev &T, double t_min, double t_max,
+                                                  const double *oc, double o_rad) {
     if (!T.gate_ok || !(t_min <= t_max)) return false;
-    const double ox = J.obs[0], oy = J.obs[1], oz = J.obs[2];
+    const double ox = oc[0], oy = oc[1], oz = oc[2];
     const double o_norm = sqrt(ox * ox + oy * oy + oz * oz);
-    if (!isfinite(o_norm)) return false;
-    const double lt = (1.01 * T.max_radius + o_norm + 0.0) * kInvC;
+    if (!isfinite(o_norm) || !isfinite(o_rad)) return false;
+    const double lt = (1.01 * T.max_radius + o_norm + o_rad) * kInvC;
     const double light_time = lt > kEmissionInitialLag ? lt : kEmissionInitialLag;
     if (!(light_time <= kGateMaxLightTime)) return false;
     const double t_mag = fabs(t_min) > fabs(t_max) ? fabs(t_min) : fabs(t_max);
@@ -251,13 +284,13 @@ __device__ __forceinline__ bool gate_below_cutoff(const JobDev &J, const TxDev &
     const double source_radius = T.max_speed * half_span * (1.0 + 1e-9) + T.gate_slack;
     const double dx = sx - ox, dy = sy - oy, dz = sz - oz;
     const double range = sqrt(dx * dx + dy * dy + dz * dz);
-    const double rho = source_radius + 0.0;
-    if (!(rho < range) || !(0.0 < o_norm)) return false;
+    const double rho = source_radius + o_rad;
+    if (!(rho < range) || !(o_rad < o_norm)) return false;
     const double sin_el = (dx * ox + dy * oy + dz * oz) / (range * o_norm);
     const double c = sin_el < -1.0 ? -1.0 : (sin_el > 1.0 ? 1.0 : sin_el);
     const double deg = 180.0 / 3.141592653589793;
     const double el_centre = asin(c) * deg;
-    const double spread = asin(rho / range) * deg + asin(0.0 / o_norm) * deg;
+    const double spread = asin(rho / range) * deg + asin(o_rad / o_norm) * deg;
     return el_centre + spread + kGateMarginDeg < J.ant_cutoff;
 }
 
@@ -266,11 +299,11 @@ __device__ __forceinline__ bool gate_below_cutoff(const JobDev &J, const TxDev &
 // position is queried at epoch + t_emit (no Sagnac rotation), sin(el) =
 // LOS . up with up = O / |O|, clamped, then asin and x * (180 / pi). Only the
 // render kernels instantiated for antenna scenes contain it.
-__device__ __forceinline__ float antenna_gain(const JobDev &J, const TxDev &T, double t_emit) {
+__device__ __forceinline__ float antenna_gain(const JobDev &J, const double *obs, const TxDev &T, double t_emit) {
     double sx, sy, sz;
     int64_t cell;
     tx_position(T, J.epoch + t_emit, sx, sy, sz, cell);
-    const double ox = J.obs[0], oy = J.obs[1], oz = J.obs[2];
+    const double ox = obs[0], oy = obs[1], oz = obs[2];
     const double on = sqrt(ox * ox + oy * oy + oz * oz);
     const double dx = sx - ox, dy = sy - oy, dz = sz - oz;
     const double dn = sqrt(dx * dx + dy * dy + dz * dz);
@@ -652,7 +685,7 @@ __device__ __forceinline__ void precision_link(const Anchor &A, bool smooth, con
     // antenna gain; an occluded sample adds nothing (src/channel/mod.rs:253-255)
     float g = 1.0f;
     if (ANT && J.ant_on) {
-        g = antenna_gain(J, txs[A.tx], t_emit[0]);
+        g = antenna_gain(J, J.obs, txs[A.tx], t_emit[0]);
         if (g != 0.0f && J.unocc) J.unocc[J.link0 + link] = 1u;
         if (g == 0.0f) return;
     }
@@ -1155,36 +1188,62 @@ __global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM == 1 ? 4 : 6))
     const double t_local = static_cast<double>(ic) / J.fs;
     const double dlt = J.delta_t ? J.delta_t[ic - J.i0] : clock_delta(J, t_local);
     const double tu = t_local - dlt;
+    // the receiver at this sample's reception time (src/pipeline.rs:929-930)
+    double obs[3];
+    rx_position(J, tu, obs);
 
-    // Horizon gate (antenna kernels): the span of the tile's reception times
-    // (src/channel/horizon.rs:97-145 for a stationary receiver); a
-    // non-finite time anywhere turns the gate off for the tile.
+    // Horizon gate (antenna kernels), src/channel/horizon.rs:97-145 over the
+    // tile: the span of its reception times and a ball holding every receiver
+    // position (centre: the first sample's; radius: the largest distance of
+    // any sample's from it; 0 for a fixed receiver); a non-finite time or
+    // position anywhere turns the gate off for the tile.
     const bool gating = ANT && lx.gate && !tap.on && J.ant_on && J.ant_cut;
-    double g_tmin = 0.0, g_tmax = 0.0;
+    double g_tmin = 0.0, g_tmax = 0.0, g_orad = 0.0;
+    double g_oc[3] = {J.obs[0], J.obs[1], J.obs[2]};
     bool g_ok = false;
     if (ANT && gating) {
-        __shared__ double s_tspan[2 * (kTile / 64)];
+        __shared__ double s_tspan[3 * (kTile / 64)];
         __shared__ uint32_t s_tbad[kTile / 64];
-        double mn = tu, mx = tu;
+        __shared__ double s_ocen[3];
+        const bool moving = J.rx_kind != 0;   // uniform
+        if (moving) {
+            if (i == first) {
+                s_ocen[0] = obs[0];
+                s_ocen[1] = obs[1];
+                s_ocen[2] = obs[2];
+            }
+            __syncthreads();
+            g_oc[0] = s_ocen[0];
+            g_oc[1] = s_ocen[1];
+            g_oc[2] = s_ocen[2];
+        }
+        const double d0 = obs[0] - g_oc[0], d1 = obs[1] - g_oc[1], d2 = obs[2] - g_oc[2];
+        const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+        double mn = tu, mx = tu, mr = r2;
         for (int o = 32; o >= 1; o >>= 1) {
             mn = fmin(mn, __shfl_xor(mn, o));
             mx = fmax(mx, __shfl_xor(mx, o));
+            mr = fmax(mr, __shfl_xor(mr, o));
         }
-        const bool bad = __ballot(!isfinite(tu)) != 0;
+        const bool bad = __ballot(!isfinite(tu) || !isfinite(r2)) != 0;
         if ((threadIdx.x & 63) == 0) {
-            s_tspan[2 * (threadIdx.x >> 6)] = mn;
-            s_tspan[2 * (threadIdx.x >> 6) + 1] = mx;
+            s_tspan[3 * (threadIdx.x >> 6)] = mn;
+            s_tspan[3 * (threadIdx.x >> 6) + 1] = mx;
+            s_tspan[3 * (threadIdx.x >> 6) + 2] = mr;
             s_tbad[threadIdx.x >> 6] = bad ? 1u : 0u;
         }
         __syncthreads();
         g_tmin = s_tspan[0];
         g_tmax = s_tspan[1];
+        double r2max = s_tspan[2];
         g_ok = s_tbad[0] == 0;
         for (int w = 1; w < kTile / 64; ++w) {
-            g_tmin = fmin(g_tmin, s_tspan[2 * w]);
-            g_tmax = fmax(g_tmax, s_tspan[2 * w + 1]);
+            g_tmin = fmin(g_tmin, s_tspan[3 * w]);
+            g_tmax = fmax(g_tmax, s_tspan[3 * w + 1]);
+            r2max = fmax(r2max, s_tspan[3 * w + 2]);
             g_ok = g_ok && s_tbad[w] == 0;
         }
+        g_orad = sqrt(r2max);
     }
 
     float acc_re = 0.0f, acc_im = 0.0f;
@@ -1211,7 +1270,7 @@ __global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM == 1 ? 4 : 6))
             const uint32_t ng = J.n_active < kGateMaxLinks ? J.n_active : kGateMaxLinks;
             if (g_ok)
                 for (uint32_t s = threadIdx.x; s < ng; s += kTile)
-                    if (gate_below_cutoff(J, txs[links[J.link0 + s].tx], g_tmin, g_tmax))
+                    if (gate_below_cutoff(J, txs[links[J.link0 + s].tx], g_tmin, g_tmax, g_oc, g_orad))
                         atomicOr(&s_gated[s >> 5], 1u << (s & 31));
             __syncthreads();
             if (threadIdx.x == 0 && lx.gate_stats) {
@@ -1261,7 +1320,7 @@ __global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM == 1 ? 4 : 6))
                 double t_emit, r;
                 if (FORM == 0) {
                     int64_t c0, c1;
-                    solve_emission(T, J.obs, tu, J.epoch, t_emit, r, c0, c1);
+                    solve_emission(T, obs, tu, J.epoch, t_emit, r, c0, c1);
                 } else {
                     const double eoff = T.traj_kind ? J.epoch - T.grid_t0 : 0.0;
                     // the learned start (the converged light times of the tile's
@@ -1271,11 +1330,11 @@ __global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM == 1 ? 4 : 6))
                     // in waves where some lane needs more
                     const double2 p = s_pred[s - s0];
                     const double t0 = tu - (p.x + p.y * frac);
-                    t_emit = emission_step(T, J.obs, tu, eoff, t0, r);
+                    t_emit = emission_step(T, obs, tu, eoff, t0, r);
                     const bool at_one = fabs(t_emit - t0) < kSolveTol;
                     if (__any(!at_one)) {
                         int e = 1;
-                        if (!at_one) e = solve_converged_tail(T, J.obs, tu, eoff, kColdEvals, t_emit, r);
+                        if (!at_one) e = solve_converged_tail(T, obs, tu, eoff, kColdEvals, t_emit, r);
                         if (lx.stats) {
                             const int ev = e < 0 ? kColdEvals : e;
                             n_extra += valid ? ev - 1 : 0;
@@ -1296,7 +1355,7 @@ __global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM == 1 ? 4 : 6))
                 // antenna gain; an occluded sample adds nothing (src/channel/mod.rs:253-255)
                 float g = 1.0f;
                 if (ANT && J.ant_on) {
-                    g = antenna_gain(J, T, t_emit);
+                    g = antenna_gain(J, obs, T, t_emit);
                     if (g != 0.0f && J.unocc) J.unocc[J.link0 + s] = 1u;
                     if (g == 0.0f) continue;
                 }
@@ -1375,7 +1434,7 @@ __global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM == 1 ? 4 : 6))
                 const LinkDev L = links[J.link0 + g0 + s];
                 const TxDev &T = txs[L.tx];
                 smooth = anchor_fit(s_anc[s], L, T, J, s_anc_t, fit_ok);
-                gated = ANT && gating && g_ok && gate_below_cutoff(J, T, g_tmin, g_tmax);
+                gated = ANT && gating && g_ok && gate_below_cutoff(J, T, g_tmin, g_tmax, J.obs, 0.0);
                 if (ANT && gating && lx.gate_stats) {
                     if (gated) atomicAdd(&lx.gate_stats[0], 1ull);
                     atomicAdd(&lx.gate_stats[1], 1ull);
@@ -1509,17 +1568,17 @@ __global__ void __launch_bounds__(256) lws_predict_kernel(const JobDev *__restri
 // solve_emission_times with `iterations` iterations (the reference uses 4 on
 // this coarse grid) and the source position at epoch + t[k]. One lane per
 // (transmitter, time).
-__global__ void __launch_bounds__(256) truth_kernel(const TxDev *__restrict__ txs, uint32_t n_tx, double ox, double oy,
-                                                    double oz, double epoch, const double *__restrict__ t, uint64_t n,
-                                                    int iterations, double *__restrict__ range,
-                                                    double *__restrict__ src) {
+__global__ void __launch_bounds__(256) truth_kernel(const TxDev *__restrict__ txs, uint32_t n_tx,
+                                                    const double *__restrict__ obs3, double epoch,
+                                                    const double *__restrict__ t, uint64_t n, int iterations,
+                                                    double *__restrict__ range, double *__restrict__ src) {
     const uint64_t g = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (g >= uint64_t(n_tx) * n) return;
     const uint32_t q = uint32_t(g / n);
     const uint64_t k = g - uint64_t(q) * n;
     const TxDev &T = txs[q];
     const double tu = t[k];
-    const double obs[3] = {ox, oy, oz};
+    const double obs[3] = {obs3[3 * k], obs3[3 * k + 1], obs3[3 * k + 2]};   // the receiver at t[k]
     double tt = tu - kEmissionInitialLag, r = 0.0;
     for (int it = 0; it < iterations; ++it) {
         double sx, sy, sz;
@@ -1706,8 +1765,8 @@ hipError_t launch_truth(const TxDev *txs, uint32_t n_tx, const double *obs, doub
     if (!total) return hipSuccess;
     const uint64_t blocks = (total + 255) / 256;
     if (blocks > 0x7FFFFFFFull) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(truth_kernel, dim3(uint32_t(blocks)), dim3(256), 0, stream, txs, n_tx, obs[0], obs[1], obs[2],
-                       epoch, t, n, iterations, range, src);
+    hipLaunchKernelGGL(truth_kernel, dim3(uint32_t(blocks)), dim3(256), 0, stream, txs, n_tx, obs, epoch, t, n,
+                       iterations, range, src);
     return hipGetLastError();
 }
 

@@ -125,6 +125,25 @@ int lpk_grid_fill_kepler(double a_m, double inc_deg, double raan_deg, double nu0
     return LPK_OK;
 }
 
+void lpk_kepler_position(double a_m, double inc_deg, double raan_deg, double nu0_deg, double orbit_epoch_unix,
+                         double t_unix, double out[3]) {
+    // CircularKeplerian::position_ecef, src/geometry/kepler.rs:27-85
+    const double mean_motion = std::sqrt(kGM / (a_m * a_m * a_m));
+    const double nu = radians(nu0_deg) + mean_motion * (t_unix - orbit_epoch_unix);
+    double snu, cnu, si, ci, so, co, s, c;
+    ::sincos(nu, &snu, &cnu);
+    ::sincos(radians(inc_deg), &si, &ci);
+    ::sincos(radians(raan_deg), &so, &co);
+    const double xp = a_m * cnu, yp = a_m * snu;
+    const double y1 = yp * ci, z1 = yp * si;
+    const double xe = xp * co - y1 * so;
+    const double ye = xp * so + y1 * co;
+    ::sincos(kOmegaE * (t_unix - orbit_epoch_unix), &s, &c);
+    out[0] = xe * c + ye * s;
+    out[1] = -xe * s + ye * c;
+    out[2] = z1;
+}
+
 int lpk_ca_code(int prn, int8_t out[1023]) {
     // IS-GPS-200 Table 3-Ia G2 tap pairs, src/signal/ca_code.rs:35-40
     static const uint8_t taps[32][2] = {

@@ -114,10 +114,13 @@ class PreparedBand:
 @dataclass
 class PreparedRx:
     name: str
-    pos: np.ndarray
+    pos: Optional[np.ndarray]     # a stationary receiver's position, else None
     clock: object
     master_rate_hz: float
     bands: List[PreparedBand]
+    # a CircularKeplerian receiver's elements (a, inclination, node, true
+    # anomaly at the epoch, epoch), evaluated per sample on the device
+    kepler: Optional[tuple] = None
 
 
 @dataclass
@@ -417,7 +420,19 @@ def prepare(scene: SceneConfig, time_base: TimeBase = TimeBase.SCENE) -> Prepare
     for r in scene.receivers:
         if not r.bands:
             raise ValueError(f"receiver '{r.name}' has no bands; at least one [[receiver.band]] required")
-        pos = station_ecef(r.trajectory)
+        kepler = None
+        if isinstance(r.trajectory, CircularKeplerian):
+            k = r.trajectory
+            kepler = tuple(float(v) for v in (k.semi_major_axis_m, k.inclination_deg, k.raan_deg,
+                                              k.true_anomaly_deg_at_epoch, k.epoch_unix))
+            if not all(math.isfinite(v) for v in kepler) or not kepler[0] > 0.0:
+                raise ValueError(f"receiver '{r.name}': Keplerian elements must be finite with a positive radius")
+            pos = None
+        elif isinstance(r.trajectory, (StationaryGeodetic, StationaryEcef)):
+            pos = station_ecef(r.trajectory)
+        else:
+            raise NotImplementedError(f"receiver '{r.name}': only stationary and circular Keplerian receivers "
+                                      f"are on the GPU path")
         master = 0.0
         for b in r.bands:
             master = max(master, float(b.sample_rate_hz))
@@ -431,7 +446,7 @@ def prepare(scene: SceneConfig, time_base: TimeBase = TimeBase.SCENE) -> Prepare
                 float(lib.lpk_thermal_noise_power_w(b.noise_temperature_k, b.sample_rate_hz))
             bands.append(PreparedBand(b, n, np.asarray(active, dtype=np.uint32), float(p), _post_ops(b),
                                       _antenna(b)))
-        rxs.append(PreparedRx(r.name, pos, r.clock, master, bands))
+        rxs.append(PreparedRx(r.name, pos, r.clock, master, bands, kepler))
     return PreparedScene(meta, txs, rxs)
 
 
@@ -514,7 +529,13 @@ class Backend:
         rx_arr = (_lib.RxDesc * max(n_rx, 1))()
         for r_idx, r in enumerate(scene.receivers):
             d = rx_arr[r_idx]
-            d.pos_ecef[:] = [float(v) for v in r.pos]
+            if r.kepler is None:
+                d.traj_kind = _lib.RX_STATIONARY
+                d.pos_ecef[:] = [float(v) for v in r.pos]
+            else:
+                d.traj_kind = _lib.RX_KEPLER
+                (d.kepler_a_m, d.kepler_inc_deg, d.kepler_raan_deg, d.kepler_nu0_deg,
+                 d.kepler_epoch_unix) = r.kepler
             d.allan_deviation_1s = r.clock.allan_deviation_1s
             d.frequency_offset_y0 = r.clock.frequency_offset_y0
             d.initial_time_offset_s = r.clock.initial_time_offset_s

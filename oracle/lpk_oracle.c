@@ -107,6 +107,18 @@ void orc_kepler_position(double a_m, double inc_deg, double raan_deg, double nu0
     out[2] = z_eci;
 }
 
+/* src/pipeline.rs:352-364, :929-930: the receiver's trajectory at t_unix */
+void orc_rx_position(const orc_rx *rx, double t_unix, double out[3]) {
+    if (rx->traj_kind == 1) {
+        orc_kepler_position(rx->kep_a_m, rx->kep_inc_deg, rx->kep_raan_deg, rx->kep_nu0_deg, rx->kep_epoch_unix,
+                            t_unix, out);
+    } else {
+        out[0] = rx->pos_ecef[0];
+        out[1] = rx->pos_ecef[1];
+        out[2] = rx->pos_ecef[2];
+    }
+}
+
 /* src/geometry/precomputed.rs:54-58 */
 uint64_t orc_grid_size(double duration_s, double grid_dt_s) {
     double total = duration_s + 2.0 * 0.5;
@@ -811,12 +823,14 @@ double orc_arrival_elevation_deg(const double obs[3], const double src[3]) {
     return asin(c) * (180.0 / PI);
 }
 
-static void tx_contribution(const orc_tx *tx, const orc_band *band, const double obs[3],
+static void tx_contribution(const orc_tx *tx, const orc_band *band, const double *obs3,
                             const double *t_local, const double *t_utc, const double *delta_t,
                             uint64_t n, double lo, double epoch, double *t_emit, double *ranges,
                             float *bb, float *out, uint8_t *unoccluded, double *tap_t_emit,
                             double *tap_range, double *tap_phase) {
+    /* obs3: the receiver position of every sample, [3 i + axis] (src/pipeline.rs:929-930) */
     for (uint64_t i = 0; i < n; ++i) {
+        const double *obs = obs3 + 3 * i;
         if (band->converged)
             orc_solve_converged(tx, obs, t_utc[i], epoch, t_utc[i] - EMISSION_INITIAL_LAG_S, 1e-13, 12, &t_emit[i],
                                 &ranges[i]);
@@ -847,7 +861,7 @@ static void tx_contribution(const orc_tx *tx, const orc_band *band, const double
         if (band->ant_on) {
             double src[3];
             tx_position(tx, epoch + t_emit[i], src);
-            g = orc_antenna_amplitude(band, orc_arrival_elevation_deg(obs, src));
+            g = orc_antenna_amplitude(band, orc_arrival_elevation_deg(obs3 + 3 * i, src));
             if (g == 0.0f) continue;
         }
         any = 1;
@@ -863,7 +877,7 @@ static void tx_contribution(const orc_tx *tx, const orc_band *band, const double
 }
 
 typedef struct {
-    double *t_local, *delta_t, *t_utc, *t_emit, *ranges;
+    double *t_local, *delta_t, *t_utc, *t_emit, *ranges, *obs;
     float *bb, *contrib, *acc;
 } chunk_scratch;
 
@@ -872,15 +886,17 @@ static int scratch_alloc(chunk_scratch *s, uint64_t clen, uint32_t n_active) {
     s->t_local = malloc(clen * sizeof(double));
     s->delta_t = malloc(clen * sizeof(double));
     s->t_utc = malloc(clen * sizeof(double));
+    s->obs = malloc(3 * clen * sizeof(double));
     s->t_emit = malloc((n_active ? n_active : 1) * clen * sizeof(double));
     s->ranges = malloc((n_active ? n_active : 1) * clen * sizeof(double));
     s->bb = malloc((n_active ? n_active : 1) * 2 * clen * sizeof(float));
     s->contrib = malloc((n_active ? n_active : 1) * 2 * clen * sizeof(float));
     s->acc = malloc(2 * clen * sizeof(float));
-    return s->t_local && s->delta_t && s->t_utc && s->t_emit && s->ranges && s->bb && s->contrib && s->acc;
+    return s->t_local && s->delta_t && s->t_utc && s->obs && s->t_emit && s->ranges && s->bb && s->contrib &&
+           s->acc;
 }
 static void scratch_free(chunk_scratch *s) {
-    free(s->t_local); free(s->delta_t); free(s->t_utc); free(s->t_emit);
+    free(s->t_local); free(s->delta_t); free(s->t_utc); free(s->obs); free(s->t_emit);
     free(s->ranges); free(s->bb); free(s->contrib); free(s->acc);
 }
 
@@ -888,7 +904,7 @@ static void scratch_free(chunk_scratch *s) {
  * transmitter fan-out runs as OpenMP tasks when `parallel_tx` is set (the
  * reference's nested rayon par_iter, :943-964); the sum is always in active
  * order (:965-976). */
-static void render_chunk(const orc_tx *txs, const orc_band *band, const double obs[3],
+static void render_chunk(const orc_tx *txs, const orc_band *band, const orc_rx *rx,
                          orc_clock *clk, float sigma, uint64_t i0, uint64_t clen,
                          chunk_scratch *s, int parallel_tx, uint32_t tap_slot,
                          double *tap_t_emit, double *tap_range, double *tap_phase,
@@ -897,6 +913,8 @@ static void render_chunk(const orc_tx *txs, const orc_band *band, const double o
     for (uint64_t j = 0; j < clen; ++j) s->t_local[j] = (double)(i0 + j) / fs;
     for (uint64_t j = 0; j < clen; ++j) s->delta_t[j] = orc_clock_delta_t_at(clk, s->t_local[j]);
     for (uint64_t j = 0; j < clen; ++j) s->t_utc[j] = s->t_local[j] - s->delta_t[j];
+    /* src/pipeline.rs:929-930: positions at the absolute UTC grid epoch + t_utc */
+    for (uint64_t j = 0; j < clen; ++j) orc_rx_position(rx, band->epoch_unix + s->t_utc[j], s->obs + 3 * j);
     if (tap_delta_t)
         for (uint64_t j = 0; j < clen; ++j) tap_delta_t[i0 + j] = s->delta_t[j];
     uint32_t na = band->n_active;
@@ -905,7 +923,7 @@ static void render_chunk(const orc_tx *txs, const orc_band *band, const double o
         for (uint32_t k = 0; k < na; ++k) {
             float *c = s->contrib + (uint64_t)k * 2 * clen;
             memset(c, 0, 2 * clen * sizeof(float));
-            tx_contribution(&txs[band->active[k]], band, obs, s->t_local, s->t_utc, s->delta_t, clen,
+            tx_contribution(&txs[band->active[k]], band, s->obs, s->t_local, s->t_utc, s->delta_t, clen,
                             band->lo_hz, band->epoch_unix, s->t_emit + (uint64_t)k * clen,
                             s->ranges + (uint64_t)k * clen, s->bb + (uint64_t)k * 2 * clen, c,
                             band->unoccluded ? band->unoccluded + k : NULL, NULL, NULL, NULL);
@@ -915,7 +933,7 @@ static void render_chunk(const orc_tx *txs, const orc_band *band, const double o
             float *c = s->contrib + (uint64_t)k * 2 * clen;
             memset(c, 0, 2 * clen * sizeof(float));
             int tap = tap_t_emit && k == tap_slot;
-            tx_contribution(&txs[band->active[k]], band, obs, s->t_local, s->t_utc, s->delta_t, clen,
+            tx_contribution(&txs[band->active[k]], band, s->obs, s->t_local, s->t_utc, s->delta_t, clen,
                             band->lo_hz, band->epoch_unix, s->t_emit + (uint64_t)k * clen,
                             s->ranges + (uint64_t)k * clen, s->bb + (uint64_t)k * 2 * clen, c,
                             band->unoccluded ? band->unoccluded + k : NULL,
@@ -957,7 +975,7 @@ int orc_render_band(const orc_tx *txs, uint32_t n_tx, const orc_rx *rx, const or
     }
     for (uint64_t i0 = 0; i0 < n; i0 += chunk_samples) {
         uint64_t clen = n - i0 < chunk_samples ? n - i0 : chunk_samples;
-        render_chunk(txs, band, rx->pos_ecef, &clk, sigma, i0, clen, &s, 0, tap_slot,
+        render_chunk(txs, band, rx, &clk, sigma, i0, clen, &s, 0, tap_slot,
                      tap_t_emit, tap_range, tap_phase, tap_delta_t);
         orc_post_ops(band->post, band->n_post, s.acc, clen, codes ? codes + 2 * i0 : NULL);
         memcpy(out_iq + 2 * i0, s.acc, 2 * clen * sizeof(float));
@@ -971,7 +989,7 @@ int orc_render_band(const orc_tx *txs, uint32_t n_tx, const orc_rx *rx, const or
  * supplied by the caller instead of walked from the band's start. Taps, when
  * given, hold every active slot: tap_*[k * n + j]. Returns 0, or -1 on bad
  * input. */
-int orc_render_window(const orc_tx *txs, uint32_t n_tx, const orc_band *band, const double obs[3],
+int orc_render_window(const orc_tx *txs, uint32_t n_tx, const orc_band *band, const orc_rx *rx,
                       uint64_t i0, uint64_t n, const double *delta_t, float *out_iq, int16_t *codes,
                       double *tap_t_emit, double *tap_range, double *tap_phase) {
     for (uint32_t k = 0; k < band->n_active; ++k)
@@ -986,11 +1004,12 @@ int orc_render_window(const orc_tx *txs, uint32_t n_tx, const orc_band *band, co
     for (uint64_t j = 0; j < n; ++j) s.t_local[j] = (double)(i0 + j) / fs;   /* src/pipeline.rs:918 */
     for (uint64_t j = 0; j < n; ++j) s.delta_t[j] = delta_t[j];
     for (uint64_t j = 0; j < n; ++j) s.t_utc[j] = s.t_local[j] - s.delta_t[j];   /* :926 */
+    for (uint64_t j = 0; j < n; ++j) orc_rx_position(rx, band->epoch_unix + s.t_utc[j], s.obs + 3 * j);   /* :929-930 */
     const uint32_t na = band->n_active;
     for (uint32_t k = 0; k < na; ++k) {
         float *c = s.contrib + (uint64_t)k * 2 * n;
         memset(c, 0, 2 * n * sizeof(float));
-        tx_contribution(&txs[band->active[k]], band, obs, s.t_local, s.t_utc, s.delta_t, n, band->lo_hz,
+        tx_contribution(&txs[band->active[k]], band, s.obs, s.t_local, s.t_utc, s.delta_t, n, band->lo_hz,
                         band->epoch_unix, s.t_emit + (uint64_t)k * n, s.ranges + (uint64_t)k * n,
                         s.bb + (uint64_t)k * 2 * n, c, band->unoccluded ? band->unoccluded + k : NULL,
                         tap_t_emit ? tap_t_emit + (uint64_t)k * n : NULL, tap_range ? tap_range + (uint64_t)k * n : NULL,
@@ -1036,7 +1055,7 @@ static void render_band_hashed(const orc_tx *txs, const orc_rx *rx, const orc_ba
     if (scratch_alloc(&s, clen_max ? clen_max : 1, band->n_active)) {
         for (uint64_t i0 = 0; i0 < n; i0 += chunk_samples) {
             uint64_t clen = n - i0 < chunk_samples ? n - i0 : chunk_samples;
-            render_chunk(txs, band, rx->pos_ecef, &clk, sigma, i0, clen, &s, 1, 0, NULL, NULL,
+            render_chunk(txs, band, rx, &clk, sigma, i0, clen, &s, 1, 0, NULL, NULL,
                          NULL, NULL);
             orc_post_ops(band->post, band->n_post, s.acc, clen, NULL);
             h = fnv1a(h, s.acc, 2 * clen);

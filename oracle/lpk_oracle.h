@@ -87,7 +87,15 @@ typedef struct {
     double master_rate_hz;
     uint64_t clock_seed;
     double flicker_floor;        /* flicker-FM Allan floor, src/clock/mod.rs:26-30 (0 = none) */
+    /* trajectory, src/pipeline.rs:352-364: 0 fixed at pos_ecef, 1 CircularKeplerian
+     * (src/geometry/kepler.rs:19-96) queried per sample at epoch + t_utc (:929-930) */
+    int32_t traj_kind, _p;
+    double kep_a_m, kep_inc_deg, kep_raan_deg, kep_nu0_deg, kep_epoch_unix;
 } orc_rx;
+
+/* Receiver position at absolute UTC time t_unix (Trajectory::position_ecef of
+ * the receiver's trajectory). */
+void orc_rx_position(const orc_rx *rx, double t_unix, double out[3]);
 
 typedef struct {
     uint32_t rx;
@@ -195,7 +203,7 @@ int orc_render_band(const orc_tx *txs, uint32_t n_tx, const orc_rx *rx, const or
 /* Samples [i0, i0 + n) of one band with caller-supplied clock offsets
  * delta_t[0..n) (a window late in a long band without walking its clock);
  * optional taps of every active slot k at tap_*[k * n + j]. */
-int orc_render_window(const orc_tx *txs, uint32_t n_tx, const orc_band *band, const double obs[3],
+int orc_render_window(const orc_tx *txs, uint32_t n_tx, const orc_band *band, const orc_rx *rx,
                       uint64_t i0, uint64_t n, const double *delta_t, float *out_iq, int16_t *codes,
                       double *tap_t_emit, double *tap_range, double *tap_phase);
 

@@ -43,7 +43,9 @@ class OrcPost(C.Structure):
 class OrcRx(C.Structure):
     _fields_ = [("pos_ecef", _d * 3), ("allan_deviation_1s", _d), ("frequency_offset_y0", _d),
                 ("initial_time_offset_s", _d), ("master_rate_hz", _d), ("clock_seed", _u64),
-                ("flicker_floor", _d)]
+                ("flicker_floor", _d), ("traj_kind", _i32), ("_p", _i32),
+                ("kep_a_m", _d), ("kep_inc_deg", _d), ("kep_raan_deg", _d), ("kep_nu0_deg", _d),
+                ("kep_epoch_unix", _d)]
 
 
 class OrcBand(C.Structure):
@@ -81,6 +83,7 @@ def load() -> C.CDLL:
     sig = {
         "orc_geodetic_to_ecef": (None, [_d, _d, _d, P(_d)]),
         "orc_kepler_position": (None, [_d, _d, _d, _d, _d, _d, P(_d)]),
+        "orc_rx_position": (None, [P(OrcRx), _d, P(_d)]),
         "orc_grid_size": (_u64, [_d, _d]),
         "orc_grid_sample_kepler": (_u64, [_d, _d, _d, _d, _d, _d, _d, _d, P(_d), P(_d)]),
         "orc_grid_position": (None, [_d, _d, _u64, P(_d), _d, P(_d)]),
@@ -117,7 +120,7 @@ def load() -> C.CDLL:
         "orc_post_ops": (None, [P(OrcPost), _u32, P(C.c_float), _u64, P(C.c_int16)]),
         "orc_render_band": (C.c_int, [P(OrcTx), _u32, P(OrcRx), P(OrcBand), _u64, _u64, P(C.c_float),
                                       P(C.c_int16), _u32, P(_d), P(_d), P(_d), P(_d)]),
-        "orc_render_window": (C.c_int, [P(OrcTx), _u32, P(OrcBand), P(_d), _u64, _u64, P(_d), P(C.c_float),
+        "orc_render_window": (C.c_int, [P(OrcTx), _u32, P(OrcBand), P(OrcRx), _u64, _u64, P(_d), P(C.c_float),
                                          P(C.c_int16), P(_d), P(_d), P(_d)]),
         "orc_render_scene_hashed": (_u64, [P(OrcTx), _u32, P(OrcRx), P(OrcBand), _u32, P(_u64), _u64,
                                            C.c_int, P(_u64)]),
@@ -176,7 +179,11 @@ class OracleScene:
         bands = []
         for r_idx, r in enumerate(rxs):
             d = self.rxs[r_idx]
-            d.pos_ecef[:] = [float(v) for v in r.pos]
+            if r.kepler is None:
+                d.pos_ecef[:] = [float(v) for v in r.pos]
+            else:
+                d.traj_kind = 1
+                (d.kep_a_m, d.kep_inc_deg, d.kep_raan_deg, d.kep_nu0_deg, d.kep_epoch_unix) = r.kepler
             d.allan_deviation_1s = r.clock.allan_deviation_1s
             d.frequency_offset_y0 = r.clock.frequency_offset_y0
             d.initial_time_offset_s = r.clock.initial_time_offset_s
@@ -242,11 +249,10 @@ class OracleScene:
         band = self.bands[work_idx]
         band.unoccluded = P(C.c_uint8)()
         band.converged = 1 if converged else 0
-        obs = np.ascontiguousarray(self.rxs[r_idx].pos_ecef[:], dtype=np.float64)
         out = np.empty(n, dtype=np.complex64)
         cd = np.empty(2 * n, dtype=np.int16) if codes else None
         tp = [np.empty((len(pb.active), n), dtype=np.float64) for _ in range(3)] if taps else [None] * 3
-        rc = lib.orc_render_window(self.txs, self.n_tx, C.byref(band), _ptr(obs, _d), i0, n, _ptr(dt, _d),
+        rc = lib.orc_render_window(self.txs, self.n_tx, C.byref(band), C.byref(self.rxs[r_idx]), i0, n, _ptr(dt, _d),
                                    out.ctypes.data_as(P(C.c_float)), _ptr(cd, C.c_int16), *[_ptr(a, _d) for a in tp])
         band.converged = 0
         if rc != 0:
