@@ -76,7 +76,7 @@ class Scene:
     def __init__(self, n, chunk=1 << 20, max_chunks=None, noise=True):
         from lipika_amd import _lib, workloads
         from lipika_amd.pipeline import Backend, prepare
-        sc = workloads.scene(n)
+        sc = workloads.micro() if n == "micro" else workloads.scene(n)
         if not noise:
             for rx in sc.receivers:
                 for b in rx.bands:
@@ -391,6 +391,32 @@ def oracle_errors(S, units, n):
             "method": "device clock offsets on both sides; noise-free; every link tapped"}
 
 
+def micro_leg(steps, warmup, cpu):
+    """The reference's criterion micro-benchmark chunk
+    (benches/iq_generation.rs:15-60): one C/A SV at GPS altitude, 5 s at
+    1.5 MHz, one transmitter_contribution_at_receiver per step. Each form is
+    timed as a whole-chunk render call; the oracle renders the same chunk on
+    one thread, as criterion runs it."""
+    S = Scene("micro")
+    out = {"workload": "the reference's criterion micro-benchmark chunk: C/A PRN 1 from a circular orbit at GPS "
+                       "altitude, seen from Glasgow, 5 s at 1.5 MHz, no noise, zero clock offset "
+                       "(benches/iq_generation.rs:15-60)",
+           "contributions_per_step": S.contributions,
+           "forms": {f: time_form(S, f, steps, warmup) for f in ("plain", "precision")}}
+    if cpu:
+        from oracle.pyoracle import OracleScene
+        orc = OracleScene(S.sc, SEED)
+        t = time.perf_counter()
+        c, _ = orc.render_scene_hashed([pb.n for _, pb in orc.units], 1 << 30, 1)
+        dt = time.perf_counter() - t
+        out["cpu_one_thread"] = {"value": round(c / dt / 1e6, 3), "unit": "Msamples/s", "seconds": round(dt, 3),
+                                 "kind": "port"}
+        out["plain_vs_cpu_one_thread"] = round(out["forms"]["plain"]["value"] / out["cpu_one_thread"]["value"], 1)
+    log(f"[bench] micro-benchmark chunk: {json.dumps(out)}")
+    S.close()
+    return out
+
+
 def roofline(contributions, kernel_ms, n):
     achieved = contributions * FLOPS_PER_CONTRIBUTION / (kernel_ms / 1e3) / 1e12
     traffic, source = TRAFFIC.get(n, (None, None))
@@ -504,6 +530,7 @@ def main():
     S2.close()
     res["config"]["config2"] = c2
     res["config"]["dropin"] = dropin_leg(2)
+    res["config"]["micro"] = micro_leg(K, W, not args.no_cpu)
     res["config"]["config5"] = learned_leg(max(1, K // 10), 1)
     print(json.dumps(res), flush=True)
 

@@ -7,6 +7,7 @@ seeded). Values the configuration text leaves open are marked ASSUMED.
     scene(3)  32 C/A MEO x 16 receivers, 2.048 MHz, 120 s
     scene(4)  40 mixed transmitters x 64 receivers with 1-3 uneven bands, 30 s
     scene(5)  32 C/A MEO x 256 receivers, 2.048 MHz, 60 s
+    micro()   the reference's criterion micro-benchmark chunk: one C/A SV, 5 s at 1.5 MHz
 """
 from __future__ import annotations
 
@@ -129,6 +130,19 @@ def scene4(duration_s: float = 30.0, n_rx: int = 64) -> SceneConfig:
                  for b in range(nb)]
         rxs.append(_random_receiver(rng, f"rx{r:03d}", bands))
     return SceneConfig(SceneMeta(EPOCH, duration_s), txs, rxs)
+
+
+def micro(duration_s: float = 5.0) -> SceneConfig:
+    """The chunk of the reference's criterion micro-benchmark
+    (benches/iq_generation.rs:15-60): C/A PRN 1 at 1575.42 MHz and 20 W from
+    a circular orbit at GPS altitude (a = 6 378 137 + 20 200 000 m, i 55 deg,
+    node 0, true anomaly 0 at the epoch), seen from Glasgow with a zero clock
+    offset; one band at the carrier, 1.5 MHz, no noise; 5 s (7.5 M samples)."""
+    tx = TransmitterConfig("sv01", L1, CaCode(1, EPOCH), _kepler(6_378_137.0 + 20_200_000.0, 55.0, 0.0, 0.0),
+                           tx_power_dbw=10.0 * math.log10(20.0))
+    rx = ReceiverConfig("glasgow", GLASGOW, NodeClockSpec(),
+                        [ReceiverBandConfig("l1", L1, 1_500_000.0, noise_power_w=0.0)])
+    return SceneConfig(SceneMeta(EPOCH, duration_s), [tx], [rx])
 
 
 def scene(n: int, duration_s: Optional[float] = None, **kw) -> SceneConfig:
