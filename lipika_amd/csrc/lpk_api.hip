@@ -19,11 +19,11 @@
 #include "lpk_lws.hpp"
 
 namespace lpk {
-hipError_t launch_render(int form, bool antenna, int tile_shift, bool xcd_map, const JobDev *jobs, int n_jobs,
-                         const LinkDev *links, const TxDev *txs, uint32_t n_tiles, const uint32_t *n_live,
-                         double *tap_t_emit, double *tap_range, double *tap_phase, double *tap_delta_t,
-                         uint32_t tap_slot, const double *pred, unsigned long long *stats, bool gate,
-                         unsigned long long *gate_stats, hipStream_t stream);
+hipError_t launch_render(int form, bool antenna, bool moving, int tile_shift, bool xcd_map, const JobDev *jobs,
+                         int n_jobs, const LinkDev *links, const TxDev *txs, uint32_t n_tiles,
+                         const uint32_t *n_live, double *tap_t_emit, double *tap_range, double *tap_phase,
+                         double *tap_delta_t, uint32_t tap_slot, const double *pred, unsigned long long *stats,
+                         bool gate, unsigned long long *gate_stats, hipStream_t stream);
 hipError_t launch_lws_predict(const JobDev *jobs, int n_jobs, const LinkDev *links, const TxDev *txs,
                               const LwsNetDev &net, int tile_shift, uint64_t n_queries, double *pred,
                               unsigned long long *stats, hipStream_t stream);
@@ -193,6 +193,7 @@ struct lpk_plan {
     lpk_ctx *ctx = nullptr;
     int form = 0, tile_shift = kDefaultTileShift, xcd_map = 1;
     bool antenna = false;
+    bool moving = false;    // some band's receiver moves
     uint64_t chunk = 0, n_chunks = 0, next = 0;
     std::vector<lpk_band_chunk> bands;
     std::vector<double> rate_ratio;       // per band: fs / the highest fs of the plan
@@ -685,18 +686,32 @@ void validate_job(const lpk_ctx *ctx, const lpk_band_chunk &B, uint32_t q) {
 }
 
 // The precision form takes at most kPrecisionMaxLinks transmitters per job.
-// Its fitted solve and the learned form's features assume a receiver fixed
-// on the Earth; a moving receiver renders in the plain or converged form.
+// The learned form's features assume a receiver fixed on the Earth; a moving
+// receiver renders in the plain, converged or precision form.
 void check_form_links(const lpk_ctx *ctx, int form, const lpk_band_chunk &B, uint32_t q) {
     if (form == LPK_FORM_PRECISION && B.n_active > kPrecisionMaxLinks)
         throw Fail{LPK_E_INVALID, "job " + std::to_string(q) + ": the precision form takes at most " +
                                       std::to_string(kPrecisionMaxLinks) + " transmitters per band-chunk (" +
                                       std::to_string(B.n_active) + " given); use the plain form"};
-    if ((form == LPK_FORM_PRECISION || form == LPK_FORM_LEARNED) && B.rx < ctx->rx.size() &&
-        ctx->rx[B.rx].d.traj_kind != LPK_RX_STATIONARY)
-        throw Fail{LPK_E_INVALID, "job " + std::to_string(q) + ": the " +
-                                      std::string(form == LPK_FORM_PRECISION ? "precision" : "learned") +
-                                      " form takes stationary receivers; use the plain or converged form"};
+    if (form == LPK_FORM_LEARNED && B.rx < ctx->rx.size() && ctx->rx[B.rx].d.traj_kind != LPK_RX_STATIONARY)
+        throw Fail{LPK_E_INVALID, "job " + std::to_string(q) +
+                                      ": the learned form takes stationary receivers; use the plain, converged "
+                                      "or precision form"};
+}
+
+// Whether the receiver of a (validated) job moves: its render takes the
+// kernels that evaluate the receiver per sample.
+bool job_moving(const lpk_ctx *ctx, const lpk_band_chunk &B) {
+    return ctx->rx[B.rx].d.traj_kind != LPK_RX_STATIONARY;
+}
+
+// The precision form takes a moving receiver in its antenna-free kernels
+// only: a render call (or plan) with an antenna band renders every job in
+// the antenna kernel, which takes fixed receivers.
+void check_form_call(int form, bool any_moving, bool any_antenna) {
+    if (form == LPK_FORM_PRECISION && any_moving && any_antenna)
+        throw Fail{LPK_E_INVALID, "the precision form takes a moving receiver only in calls without an antenna "
+                                  "band; use the plain or converged form"};
 }
 
 // The window-relative clock lookups of a job whose window starts at
@@ -762,6 +777,24 @@ JobDev fill_job(const lpk_ctx *ctx, const lpk_band_chunk &B, uint64_t i0, uint64
         sincos(R.d.kepler_inc_deg * deg, &J.rk_si, &J.rk_ci);
         sincos(R.d.kepler_raan_deg * deg, &J.rk_so, &J.rk_co);
         J.rk_epoch = R.d.kepler_epoch_unix;
+        // the precision form's smooth model: E = epoch - rk_epoch, nu0 + n E
+        // and omega E exactly, as double-doubles (error-free sums and products)
+        auto two_sum = [](double a, double b, double *o) {
+            const double s = a + b, bb = s - a;
+            o[0] = s;
+            o[1] = (a - (s - bb)) + (b - bb);
+        };
+        auto scaled = [&](double k, double add, double *o) {   // add + k E
+            const double p = k * J.rk_e[0], pe = std::fma(k, J.rk_e[0], -p);
+            double s[2];
+            two_sum(add, p, s);
+            const double lo = s[1] + (pe + k * J.rk_e[1]);
+            o[0] = s[0] + lo;
+            o[1] = lo - (o[0] - s[0]);
+        };
+        two_sum(J.epoch, -J.rk_epoch, J.rk_e);
+        scaled(J.rk_n, J.rk_nu0, J.rk_nue);
+        scaled(kEarthRotation, 0.0, J.rk_the);
     }
     J.c0 = R.d.initial_time_offset_s;
     J.y0 = R.d.frequency_offset_y0;
@@ -826,18 +859,20 @@ void render(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form,
     if (n_jobs && !jobs) throw Fail{LPK_E_INVALID, "jobs is NULL"};
     // ---- validation
     uint64_t total_links = 0, dt_total = 0, stage_total = 0, code_total = 0;
-    bool want_flags = false, any_antenna = false;
+    bool want_flags = false, any_antenna = false, any_moving = false;
     for (uint32_t q = 0; q < n_jobs; ++q) {
         const lpk_band_chunk &B = jobs[q];
         validate_job(ctx, B, q);
         check_form_links(ctx, form, B, q);
         if (B.unoccluded && B.n_active) want_flags = true;
         if (B.has_antenna) any_antenna = true;
+        if (job_moving(ctx, B)) any_moving = true;
         total_links += B.n_active;
         if (B.delta_t) dt_total += B.clen;
         if (!B.out_dev) stage_total += B.clen;
         if (B.adc_codes_host) code_total += B.clen;
     }
+    check_form_call(form, any_moving, any_antenna);
     if (tap && n_jobs != 1) throw Fail{LPK_E_INVALID, "taps need exactly one job"};
     if (tap && tap->slot >= jobs[0].n_active) throw Fail{LPK_E_INVALID, "tap slot out of range"};
     const int ts = ctx->tile_shift;
@@ -955,7 +990,7 @@ void render(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form,
             hip_check(launch_lws_predict(jd, int(n_jobs), ld, ctx->tx_dev, lws_net(ctx->lws), ts, n_pred, pred,
                                          ctx->stats, ctx->stream),
                       "warm-start prediction launch");
-        hip_check(launch_render(converged ? 2 : form, any_antenna, ts, ctx->xcd_map != 0, jd, int(n_jobs), ld,
+        hip_check(launch_render(converged ? 2 : form, any_antenna, any_moving, ts, ctx->xcd_map != 0, jd, int(n_jobs), ld,
                                 ctx->tx_dev, uint32_t(tiles), nullptr, tap ? tap->t_emit : nullptr,
                                 tap ? tap->range : nullptr, tap ? tap->phase : nullptr, tap ? tap->delta_t : nullptr,
                                 tap_slot, pred, converged ? ctx->stats : nullptr, ctx->gate != 0,
@@ -1148,7 +1183,9 @@ lpk_plan *plan_create(lpk_ctx *ctx, const lpk_band_chunk *bands, uint32_t n_band
         P->active[b].assign(B.active, B.active + B.n_active);
         B.active = P->active[b].data();
         P->antenna = P->antenna || B.has_antenna;
+        P->moving = P->moving || job_moving(ctx, B);
     }
+    check_form_call(form, P->moving, P->antenna);
     double fs_max = 0.0;
     for (const lpk_band_chunk &B : P->bands) fs_max = std::max(fs_max, B.fs_hz);
     P->rate_ratio.resize(n_bands);
@@ -1241,7 +1278,7 @@ lpk_plan *plan_create(lpk_ctx *ctx, const lpk_band_chunk *bands, uint32_t n_band
         if (e == hipSuccess && nw)
             e = launch_walk_carry(reinterpret_cast<const CarrySegDev *>(dev + P->off_carry), int(nw), ctx->stream);
         if (e == hipSuccess && P->max_tiles)
-            e = launch_render(form, P->antenna, P->tile_shift, P->xcd_map != 0,
+            e = launch_render(form, P->antenna, P->moving, P->tile_shift, P->xcd_map != 0,
                               reinterpret_cast<const JobDev *>(dev + P->off_jobs), int(n_bands),
                               reinterpret_cast<const LinkDev *>(dev + P->off_links), ctx->tx_dev, P->max_tiles, live,
                               nullptr, nullptr, nullptr, nullptr, 0, nullptr, nullptr, ctx->gate != 0, nullptr,

@@ -200,6 +200,10 @@ struct JobDev {
     // inclination and of the node, and the elements' epoch (unix s)
     int32_t rx_kind, _rxpad;
     double rk_a, rk_n, rk_nu0, rk_si, rk_ci, rk_so, rk_co, rk_epoch;
+    // the precision form's smooth receiver model at the exact time
+    // E + tu, E = epoch - rk_epoch: E, rk_nu0 + rk_n E and omega E, each as
+    // a double-double (hi, lo)
+    double rk_e[2], rk_nue[2], rk_the[2];
     // clock
     const double *delta_t;   // host-provided offsets (device copy) or null
     const double *walk_base; // per 65 536-index block, absolute block index

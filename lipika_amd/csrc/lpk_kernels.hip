@@ -131,6 +131,62 @@ __device__ __forceinline__ void rx_position(const JobDev &J, double tu, double *
     o[2] = z1;
 }
 
+// The precision form's smooth receiver: the orbit of rx_position at the exact
+// time E + tu, E = epoch - rk_epoch (true anomaly nu = nu0 + n (E + tu) and
+// Earth angle theta = omega (E + tu), each formed as a double-double from the
+// job's constants), with the position's derivatives along the two angles (pn
+// along nu, pt along theta). The angles' low parts join to first order.
+__device__ __forceinline__ void rx_smooth(const JobDev &J, double tu, double *o, double *pn, double *pt) {
+    double hi[2], lo[2];
+    const double *cst[2] = {J.rk_nue, J.rk_the};
+    const double rate[2] = {J.rk_n, kEarthRotation};
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const double p = rate[k] * tu, pe = fma(rate[k], tu, -p);
+        const double s = cst[k][0] + p, bb = s - cst[k][0];
+        hi[k] = s;
+        lo[k] = ((cst[k][0] - (s - bb)) + (p - bb)) + (pe + cst[k][1]);
+    }
+    double s_nu, c_nu, s, c;
+    sincos(hi[0], &s_nu, &c_nu);
+    sincos(hi[1], &s, &c);
+    const double xp = J.rk_a * c_nu, yp = J.rk_a * s_nu;
+    // ECI position and its derivative along nu (R_z(node) R_x(inclination))
+    const double x = xp * J.rk_co - yp * J.rk_ci * J.rk_so, y = xp * J.rk_so + yp * J.rk_ci * J.rk_co;
+    const double dx = -yp * J.rk_co - xp * J.rk_ci * J.rk_so, dy = -yp * J.rk_so + xp * J.rk_ci * J.rk_co;
+    // to ECEF by R_z(-theta); d/d theta of (X, Y, Z) is (Y, -X, 0)
+    const double ex = x * c + y * s, ey = -x * s + y * c;
+    pn[0] = dx * c + dy * s;
+    pn[1] = -dx * s + dy * c;
+    pn[2] = xp * J.rk_si;
+    pt[0] = ey;
+    pt[1] = -ex;
+    pt[2] = 0.0;
+    o[0] = fma(pn[0], lo[0], fma(ey, lo[1], ex));
+    o[1] = fma(pn[1], lo[0], fma(-ex, lo[1], ey));
+    o[2] = fma(pn[2], lo[0], yp * J.rk_si);
+}
+
+// How rx_position's query at the rounded time epoch + tu departs from the
+// smooth receiver at E + tu: dn in the true anomaly and dth in the Earth
+// angle, each exact (error-free products and sums): the query time's own
+// rounding d, times the angle's rate, less the roundings of n dt, nu0 + n dt
+// and omega dt. Zero for a fixed receiver.
+__device__ __forceinline__ void rx_rounding(const JobDev &J, double tu, double &dn, double &dth) {
+    if (J.rx_kind == 0) {
+        dn = dth = 0.0;
+        return;
+    }
+    const double dt = (J.epoch + tu) - J.rk_epoch;                     // as rx_position forms it
+    const double d = ((dt - J.rk_e[0]) - tu) - J.rk_e[1];              // dt less E + tu
+    const double p = J.rk_n * dt, e1 = fma(J.rk_n, dt, -p);            // n dt = p + e1
+    const double s = J.rk_nu0 + p, bb = s - J.rk_nu0;
+    const double e2 = (J.rk_nu0 - (s - bb)) + (p - bb);                // nu0 + p = s + e2
+    dn = fma(J.rk_n, d, -(e1 + e2));
+    const double th = kEarthRotation * dt, e3 = fma(kEarthRotation, dt, -th);
+    dth = fma(kEarthRotation, d, -e3);
+}
+
 // solve_emission_times_for, one sample (src/channel/mod.rs:70-99). cell0 and
 // cell1 are the grid nodes the two iterations' queries fell in.
 __device__ __forceinline__ void solve_emission(const TxDev &T, const double *obs, double tu,
@@ -531,6 +587,9 @@ struct alignas(16) Anchor {
     double r1m, q1d1, q1c2, rd1q;  // r1(tu) = r1m + x (q1d1 + q1c2 x) + rd1q del1 - k01 del0
     double k01;                    // (d r1 / d t1, position and Sagnac) * rd0 / c
     double r1_mid;                 // range at the middle sample (model, or exact when not smooth)
+    // moving receiver: r0 += rxn0 dn + rxt0 dth, r1 += rxn1 dn + rxt1 dth for the
+    // receiver query's departures dn, dth (rx_rounding) in its two angles
+    double rxn0, rxt0, rxn1, rxt1;
     union {
         struct {                   // written by the fit, read by the sample loop
             double ph0;            // carrier phase at the middle sample, turns in [-1/2, 1/2]
@@ -555,7 +614,7 @@ struct alignas(16) Anchor {
     uint32_t tx;                   // transmitter (slow paths read its table)
     double neg_tau_src, tau_if;    // -TAU f_src, TAU (f_src - f_lo), for taps
 };
-static_assert(sizeof(Anchor) == 208, "Anchor layout");
+static_assert(sizeof(Anchor) == 240, "Anchor layout");
 static_assert(offsetof(Anchor, h0re) % 16 == 0, "hint values are one aligned 16-byte read");
 constexpr int kAnchorGroup = 64;  // transmitters per anchor pass
 constexpr int kSpanRecords = 128; // (tile, transmitter) records of one anchor batch of a span
@@ -593,10 +652,15 @@ __device__ __forceinline__ double carrier_turns(double neg_tau_src, double tau_i
 
 // The two-iteration solve with the trajectory queried at unrounded times
 // ((epoch - t0) + t instead of epoch + t); also returns the node cells and
-// the range rates the rounding correction needs.
+// the range rates the rounding correction needs. With the receiver's
+// derivatives along its two angles (pn, pt; null for none), cr receives each
+// iteration's range derivatives along them: (nu, theta) of iteration 1, then
+// of iteration 2.
 __device__ __forceinline__ void solve_smooth(const TxDev &T, const double *obs, double tu, double eoff,
                                              double &r0, double &r1, int64_t &c0, int64_t &c1,
-                                             double &rd0, double &rd1q, double &rd1t) {
+                                             double &rd0, double &rd1q, double &rd1t,
+                                             const double *pn = nullptr, const double *pt = nullptr,
+                                             double *cr = nullptr) {
     double t = tu - kEmissionInitialLag;
     double rr[2], rq[2], rt[2];
     int64_t cc[2];
@@ -637,6 +701,11 @@ __device__ __forceinline__ void solve_smooth(const TxDev &T, const double *obs, 
         // angle: -omega J S, with J (x, y, z) = (y, -x, 0)
         const double pq = (dx * fma(w, vy, vx) + dy * fma(-w, vx, vy) + dz * vz) * ir;
         const double pw = (dx * sy - dy * sx) * ir * -kEarthRotation;
+        if (pn) {
+            // the receiver enters as -obs: d r / d angle = -(d . d obs / d angle) / r
+            cr[2 * it] = -(dx * pn[0] + dy * pn[1] + dz * pn[2]) * ir;
+            cr[2 * it + 1] = -(dx * pt[0] + dy * pt[1] + dz * pt[2]) * ir;
+        }
         rr[it] = r;
         rq[it] = pq;
         rt[it] = pq + pw;
@@ -661,23 +730,41 @@ __device__ __forceinline__ void solve_smooth(const TxDev &T, const double *obs, 
 // the transmitter's table. x is tu less its value at the middle sample, as
 // (t_local - t_local_mid) - (delta_t - delta_t_mid), and gq = -f_lo (t_local -
 // t_local_mid), the band's part of the phase. The N samples are independent chains that share the record's reads.
-template <int G, bool ANT, int N>
+// MOV: the receiver may move; dn, dth are the samples' receiver departures
+// (rx_rounding), and obs the sample's receiver position where the caller has
+// it (antenna kernels; null otherwise).
+template <int G, bool ANT, int N, bool MOV>
 __device__ __forceinline__ void precision_link(const Anchor &A, bool smooth, const TxDev *__restrict__ txs,
                                                const JobDev &J, uint32_t link, const double *tu, const double *x,
-                                               const double *del0, const double *gq, double *sum_re, double *sum_im) {
+                                               const double *del0, const double *gq, const double *dn,
+                                               const double *dth, const double *obs, double *sum_re,
+                                               double *sum_im) {
     static_assert(!ANT || N == 1, "antenna kernels take one sample per lane");
     double r[N];
     if (smooth) {
 #pragma unroll
         for (int q = 0; q < N; ++q) {
-            const double r0 = fma(A.rd0, del0[q], fma(x[q], fma(A.q0c2, x[q], A.q0d1), A.r0m));
+            double r0 = fma(A.rd0, del0[q], fma(x[q], fma(A.q0c2, x[q], A.q0d1), A.r0m));
+            double r1 = fma(x[q], fma(A.q1c2, x[q], A.q1d1), A.r1m);
+            if (MOV) {
+                r0 = fma(A.rxn0, dn[q], fma(A.rxt0, dth[q], r0));
+                r1 = fma(A.rxn1, dn[q], fma(A.rxt1, dth[q], r1));
+            }
             const double t1 = fma(-r0, kInvC, tu[q]);
             const double del1 = ((J.epoch + t1) - J.epoch) - t1;
-            r[q] = fma(A.rd1q, del1, fma(-A.k01, del0[q], fma(x[q], fma(A.q1c2, x[q], A.q1d1), A.r1m)));
+            r[q] = fma(A.rd1q, del1, fma(-A.k01, del0[q], r1));
         }
     } else {
 #pragma unroll
-        for (int q = 0; q < N; ++q) r[q] = solve_emission_scaled(txs[A.tx], J.obs, tu[q], J.epoch);
+        for (int q = 0; q < N; ++q) {
+            if (ANT || !MOV) {
+                r[q] = solve_emission_scaled(txs[A.tx], ANT ? obs : J.obs, tu[q], J.epoch);
+            } else {
+                double o[3];
+                rx_position(J, tu[q], o);
+                r[q] = solve_emission_scaled(txs[A.tx], o, tu[q], J.epoch);
+            }
+        }
     }
     double t_emit[N];
 #pragma unroll
@@ -685,7 +772,7 @@ __device__ __forceinline__ void precision_link(const Anchor &A, bool smooth, con
     // antenna gain; an occluded sample adds nothing (src/channel/mod.rs:253-255)
     float g = 1.0f;
     if (ANT && J.ant_on) {
-        g = antenna_gain(J, J.obs, txs[A.tx], t_emit[0]);
+        g = antenna_gain(J, obs, txs[A.tx], t_emit[0]);
         if (g != 0.0f && J.unocc) J.unocc[J.link0 + link] = 1u;
         if (g == 0.0f) return;
     }
@@ -787,28 +874,43 @@ __device__ __forceinline__ void precision_link(const Anchor &A, bool smooth, con
 // The tapped link's emission time, range and carrier phase in the precision
 // form: the range as the sample loop forms it, the phase in the reference's
 // f64 operation order (src/channel/mod.rs:255-258) from that range.
+template <bool MOV>
 __device__ __forceinline__ void precision_tap(const Anchor &A, bool smooth, const TxDev *__restrict__ txs,
                                               const JobDev &J, double tu, double t_local, double dlt, double x,
-                                              double del0, double &te, double &r, double &ph) {
+                                              double del0, double dn, double dth, double &te, double &r,
+                                              double &ph) {
     if (smooth) {
-        const double r0 = fma(A.rd0, del0, fma(x, fma(A.q0c2, x, A.q0d1), A.r0m));
+        double r0 = fma(A.rd0, del0, fma(x, fma(A.q0c2, x, A.q0d1), A.r0m));
+        double r1 = fma(x, fma(A.q1c2, x, A.q1d1), A.r1m);
+        if (MOV) {
+            r0 = fma(A.rxn0, dn, fma(A.rxt0, dth, r0));
+            r1 = fma(A.rxn1, dn, fma(A.rxt1, dth, r1));
+        }
         const double t1 = fma(-r0, kInvC, tu);
         const double del1 = ((J.epoch + t1) - J.epoch) - t1;
-        r = fma(A.rd1q, del1, fma(-A.k01, del0, fma(x, fma(A.q1c2, x, A.q1d1), A.r1m)));
+        r = fma(A.rd1q, del1, fma(-A.k01, del0, r1));
     } else {
-        r = solve_emission_scaled(txs[A.tx], J.obs, tu, J.epoch);
+        double o[3] = {J.obs[0], J.obs[1], J.obs[2]};
+        if (MOV) rx_position(J, tu, o);
+        r = solve_emission_scaled(txs[A.tx], o, tu, J.epoch);
     }
     te = fma(-r, kInvC, tu);
     ph = A.neg_tau_src * (r * kInvC + dlt) + A.tau_if * t_local;
 }
 
 // Anchor pass of one (tile, transmitter) record: the smooth solve at anchor a
-// (0 first, 1 middle, 2 last valid sample of the tile) of reception time tu.
-__device__ __forceinline__ void anchor_solve(Anchor &A, const TxDev &T, const JobDev &J, double tu, uint32_t a) {
+// (0 first, 1 middle, 2 last valid sample of the tile) of reception time tu,
+// the receiver at obs. A moving receiver's derivatives along its angles at
+// the middle anchor (pn, pt; null for a fixed receiver) give the record's
+// receiver coefficients, the effect on r1 including that of r0 through t1.
+__device__ __forceinline__ void anchor_solve(Anchor &A, const TxDev &T, const JobDev &J, double tu, uint32_t a,
+                                             const double *obs, const double *pn = nullptr,
+                                             const double *pt = nullptr) {
     const double eoff = T.traj_kind ? J.epoch - T.grid_t0 : 0.0;
-    double r0, r1, rd0, rd1q, rd1t;
+    double r0, r1, rd0, rd1q, rd1t, cr[4];
     int64_t c0, c1;
-    solve_smooth(T, J.obs, tu, eoff, r0, r1, c0, c1, rd0, rd1q, rd1t);
+    if (pn && a == 1) solve_smooth(T, obs, tu, eoff, r0, r1, c0, c1, rd0, rd1q, rd1t, pn, pt, cr);
+    else solve_smooth(T, obs, tu, eoff, r0, r1, c0, c1, rd0, rd1q, rd1t);
     A.r0s[a] = r0;
     A.r1s[a] = r1;
     if (a != 1) {
@@ -818,14 +920,23 @@ __device__ __forceinline__ void anchor_solve(Anchor &A, const TxDev &T, const Jo
         A.rd0 = rd0;
         A.rd1q = rd1q;
         A.k01 = rd1t * rd0 * kInvC;
+        if (pn) {
+            A.rxn0 = cr[0];
+            A.rxt0 = cr[1];
+            A.rxn1 = cr[2] - rd1t * cr[0] * kInvC;
+            A.rxt1 = cr[3] - rd1t * cr[1] * kInvC;
+        }
     }
 }
 
 // The fit of one (tile, transmitter) record from its anchor pass; at[a] holds
 // (tu, t_local, delta_t) of anchor a. Returns whether the tile is smooth for
 // the transmitter (the lanes then use the fitted range, else the exact solve).
+// MOV: the receiver may move (the record then holds its coefficients).
+template <bool MOV>
 __device__ __forceinline__ bool anchor_fit(Anchor &A, const LinkDev &L, const TxDev &T, const JobDev &J,
                                            const double (*at)[3], bool fit_ok) {
+    const bool mov = MOV && J.rx_kind != 0;
     // the anchor pass's results share storage with fields written below
     const double r0s[3] = {A.r0s[0], A.r0s[1], A.r0s[2]};
     const double r1s[3] = {A.r1s[0], A.r1s[1], A.r1s[2]};
@@ -847,13 +958,21 @@ __device__ __forceinline__ bool anchor_fit(Anchor &A, const LinkDev &L, const Tx
         // the model at the middle sample (x = 0)
         const double t0m = at[1][0] - kEmissionInitialLag;
         const double d0m = ((J.epoch + t0m) - J.epoch) - t0m;
-        const double r0m = fma(A.rd0, d0m, r0s[1]);
+        double r0m = fma(A.rd0, d0m, r0s[1]), r1c = r1s[1];
+        if (mov) {
+            double dn, dth;
+            rx_rounding(J, at[1][0], dn, dth);
+            r0m = fma(A.rxn0, dn, fma(A.rxt0, dth, r0m));
+            r1c = fma(A.rxn1, dn, fma(A.rxt1, dth, r1c));
+        }
         const double t1m = fma(-r0m, kInvC, at[1][0]);
         const double d1m = ((J.epoch + t1m) - J.epoch) - t1m;
-        r1m = fma(A.rd1q, d1m, fma(-A.k01, d0m, r1s[1]));
+        r1m = fma(A.rd1q, d1m, fma(-A.k01, d0m, r1c));
     } else {
         // the exact solve at the middle sample, as the lanes solve
-        r1m = solve_emission_scaled(T, J.obs, at[1][0], J.epoch);
+        double o[3] = {J.obs[0], J.obs[1], J.obs[2]};
+        if (mov) rx_position(J, at[1][0], o);
+        r1m = solve_emission_scaled(T, o, at[1][0], J.epoch);
     }
     A.r1_mid = r1m;
     const float amp_m = static_cast<float>(T.amp_factor / r1m);
@@ -909,31 +1028,31 @@ __device__ __forceinline__ bool anchor_fit(Anchor &A, const LinkDev &L, const Tx
 // records are R[0..ng): the transmitter groups in order, links whose bit is
 // set in gmask skipped (antenna kernels), the fitted range for those whose
 // bit is set in smask.
-template <bool ANT, int N>
+template <bool ANT, int N, bool MOV>
 __device__ __forceinline__ void precision_links(const Anchor *R, uint64_t smask, uint64_t gmask,
                                                 const TxDev *__restrict__ txs, const JobDev &J, uint32_t g0,
                                                 uint32_t ng, const double *tu, const double *x, const double *del0,
-                                                const double *gq, double *sum_re,
-                                                double *sum_im) {
+                                                const double *gq, const double *dn, const double *dth,
+                                                const double *obs, double *sum_re, double *sum_im) {
     const uint32_t e_fr = J.group_end[kGroupFramed] > g0 ? min(J.group_end[kGroupFramed] - g0, ng) : 0u;
     const uint32_t e_cw = J.group_end[kGroupCw] > g0 ? min(J.group_end[kGroupCw] - g0, ng) : 0u;
     const uint32_t e_gn = J.group_end[kGroupGeneric] > g0 ? min(J.group_end[kGroupGeneric] - g0, ng) : 0u;
     uint32_t s = 0;
     for (; s < e_fr; ++s)
         if (!ANT || !((gmask >> s) & 1u))
-            precision_link<kGroupFramed, ANT, N>(R[s], (smask >> s) & 1u, txs, J, g0 + s, tu, x, del0, gq,
-                                                 sum_re, sum_im);
+            precision_link<kGroupFramed, ANT, N, MOV>(R[s], (smask >> s) & 1u, txs, J, g0 + s, tu, x, del0, gq, dn, dth,
+                                                 obs, sum_re, sum_im);
     for (; s < e_cw; ++s)
         if (!ANT || !((gmask >> s) & 1u))
-            precision_link<kGroupCw, ANT, N>(R[s], (smask >> s) & 1u, txs, J, g0 + s, tu, x, del0, gq, sum_re,
+            precision_link<kGroupCw, ANT, N, MOV>(R[s], (smask >> s) & 1u, txs, J, g0 + s, tu, x, del0, gq, dn, dth, obs, sum_re,
                                              sum_im);
     for (; s < e_gn; ++s)
         if (!ANT || !((gmask >> s) & 1u))
-            precision_link<kGroupGeneric, ANT, N>(R[s], (smask >> s) & 1u, txs, J, g0 + s, tu, x, del0, gq,
-                                                  sum_re, sum_im);
+            precision_link<kGroupGeneric, ANT, N, MOV>(R[s], (smask >> s) & 1u, txs, J, g0 + s, tu, x, del0, gq, dn, dth,
+                                                  obs, sum_re, sum_im);
     for (; s < ng; ++s)
         if (!ANT || !((gmask >> s) & 1u))
-            precision_link<kGroupCa, ANT, N>(R[s], (smask >> s) & 1u, txs, J, g0 + s, tu, x, del0, gq, sum_re,
+            precision_link<kGroupCa, ANT, N, MOV>(R[s], (smask >> s) & 1u, txs, J, g0 + s, tu, x, del0, gq, dn, dth, obs, sum_re,
                                              sum_im);
 }
 
@@ -973,8 +1092,8 @@ __device__ __forceinline__ void finish_sample(const JobDev &J, float acc_re, flo
 // dependent global reads is paid once per batch; the tiles of the batch then
 // run their sample loops in turn, each lane's two samples as independent
 // chains that share every record read. Per tile the arithmetic is that of a
-// one-tile workgroup.
-template <int TS>
+// one-tile workgroup. MOV: the receiver may move.
+template <int TS, bool MOV>
 __device__ __forceinline__ void precision_span(const JobDev &J, uint32_t span, const LinkDev *__restrict__ links,
                                                const TxDev *__restrict__ txs, const TapOut &tap) {
     constexpr int kTile = 1 << TS, kLanes = kTile / kSpanSamples, kSub = 1 << kSuperShift;
@@ -982,6 +1101,8 @@ __device__ __forceinline__ void precision_span(const JobDev &J, uint32_t span, c
     static_assert(kWaves >= 1, "at least one wavefront per span");
     __shared__ Anchor s_rec[kSpanRecords];        // record kk * ng + s: tile kk of the batch, link g0 + s
     __shared__ double s_at[kSub][3][3];           // (tu, t_local, delta_t) at each tile's anchors
+    __shared__ double s_rx[MOV ? kSub : 1][3][3];  // moving receiver: its smooth position at the anchors
+    __shared__ double s_rxp[MOV ? kSub : 1][2][3]; // and its derivatives along nu and theta at the middle one
     __shared__ uint32_t s_mask[kSub][2];          // smooth links of each tile of the batch
     __shared__ uint8_t s_smooth[kSpanRecords];     // per record: the tile is smooth for the link
     const uint64_t span0 = ((J.i0 >> (TS + kSuperShift)) + span) << (TS + kSuperShift);
@@ -1009,6 +1130,17 @@ __device__ __forceinline__ void precision_span(const JobDev &J, uint32_t span, c
                     s_at[kk][a][0] = tl - dl;
                     s_at[kk][a][1] = tl;
                     s_at[kk][a][2] = dl;
+                    if (MOV) {
+                        double pn[3] = {0.0, 0.0, 0.0}, pt[3] = {0.0, 0.0, 0.0}, o[3] = {J.obs[0], J.obs[1], J.obs[2]};
+                        if (J.rx_kind) rx_smooth(J, tl - dl, o, pn, pt);
+                        for (int k = 0; k < 3; ++k) {
+                            s_rx[kk][a][k] = o[k];
+                            if (a == 1) {
+                                s_rxp[kk][0][k] = pn[k];
+                                s_rxp[kk][1][k] = pt[k];
+                            }
+                        }
+                    }
                 }
             }
             __syncthreads();
@@ -1018,7 +1150,8 @@ __device__ __forceinline__ void precision_span(const JobDev &J, uint32_t span, c
                 const uint64_t t0 = span0 + uint64_t(k0 + kk) * kTile;
                 if ((J.i0 > t0 ? J.i0 : t0) >= (jend < t0 + kTile ? jend : t0 + kTile)) continue;
                 const LinkDev L = links[J.link0 + g0 + (rec - kk * ng)];
-                anchor_solve(s_rec[rec], txs[L.tx], J, s_at[kk][a][0], a);
+                if (MOV) anchor_solve(s_rec[rec], txs[L.tx], J, s_at[kk][a][0], a, s_rx[kk][a], s_rxp[kk][0], s_rxp[kk][1]);
+                else anchor_solve(s_rec[rec], txs[L.tx], J, s_at[kk][a][0], a, J.obs);
             }
             __syncthreads();
             // the fits, one record a lane
@@ -1029,7 +1162,7 @@ __device__ __forceinline__ void precision_span(const JobDev &J, uint32_t span, c
                 bool smooth = false;
                 if (first < end) {
                     const LinkDev L = links[J.link0 + g0 + (rec - kk * ng)];
-                    smooth = anchor_fit(s_rec[rec], L, txs[L.tx], J, s_at[kk], end - 1 - first >= 2);
+                    smooth = anchor_fit<MOV>(s_rec[rec], L, txs[L.tx], J, s_at[kk], end - 1 - first >= 2);
                 }
                 s_smooth[rec] = smooth ? 1u : 0u;
             }
@@ -1053,7 +1186,7 @@ __device__ __forceinline__ void precision_span(const JobDev &J, uint32_t span, c
                 uint64_t i[kSpanSamples];
                 bool valid[kSpanSamples];
                 double t_local[kSpanSamples], dlt[kSpanSamples], tu[kSpanSamples], x[kSpanSamples];
-                double del0[kSpanSamples], gq[kSpanSamples];
+                double del0[kSpanSamples], gq[kSpanSamples], dn[kSpanSamples], dth[kSpanSamples];
 #pragma unroll
                 for (int q = 0; q < kSpanSamples; ++q) {
                     i[q] = t0 + kSpanSamples * threadIdx.x + q;
@@ -1073,11 +1206,13 @@ __device__ __forceinline__ void precision_span(const JobDev &J, uint32_t span, c
                     del0[q] = ((J.epoch + t0q) - J.epoch) - t0q;
                     // the band's part of the carrier phase from the middle sample's
                     gq[q] = -J.lo * (t_local[q] - s_at[kk][1][1]);
+                    dn[q] = dth[q] = 0.0;
+                    if (MOV) rx_rounding(J, tu[q], dn[q], dth[q]);
                 }
                 const uint64_t smask = uniform_u64(s_mask[kk][0], s_mask[kk][1]);
                 const Anchor *R = s_rec + kk * ng;
-                precision_links<false, kSpanSamples>(R, smask, 0ull, txs, J, g0, ng, tu, x, del0, gq, sum_re,
-                                                     sum_im);
+                precision_links<false, kSpanSamples, MOV>(R, smask, 0ull, txs, J, g0, ng, tu, x, del0, gq, dn, dth,
+                                                          nullptr, sum_re, sum_im);
                 if (tap.on) {
                     // the sample times and clock offsets again (as above), for the taps
 #pragma unroll
@@ -1091,7 +1226,8 @@ __device__ __forceinline__ void precision_span(const JobDev &J, uint32_t span, c
                         if (tap.slot >= g0 && tap.slot < g0 + ng) {
                             const uint32_t ts = tap.slot - g0;
                             double te, r, ph;
-                            precision_tap(R[ts], (smask >> ts) & 1u, txs, J, tu[q], tl, dl, x[q], del0[q], te, r, ph);
+                            precision_tap<MOV>(R[ts], (smask >> ts) & 1u, txs, J, tu[q], tl, dl, x[q], del0[q], dn[q],
+                                               dth[q], te, r, ph);
                             tap.t_emit[i[q] - J.i0] = te;
                             tap.range[i[q] - J.i0] = r;
                             tap.phase[i[q] - J.i0] = ph;
@@ -1141,12 +1277,13 @@ struct RenderLaunch {
     unsigned long long *gate_stats;  // (tile, link) pairs the gate skipped, and pairs it tested
 };
 
-template <int FORM, bool ANT, int TS>
+template <int FORM, bool ANT, int TS, bool MOV>
 __global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM == 1 ? 4 : 6))
     render_kernel(const JobDev *__restrict__ jobs, int n_jobs, const LinkDev *__restrict__ links,
                   const TxDev *__restrict__ txs, RenderLaunch lx, TapOut tap) {
     constexpr int kTile = 1 << TS;
     constexpr bool kSpan = FORM == 1 && !ANT;          // the super-tile path has LDS of its own
+    static_assert(!MOV || kSpan, "only the precision form's span kernels take a moving receiver");
     __shared__ uint64_t s_noise[kSpan ? 1 : kTile];    // one Philox word per sample
     __shared__ double s_anc_t[3][3];                   // tu, t_local, delta_t at the anchors
     __shared__ Anchor s_anc[FORM == 1 && ANT ? kAnchorGroup : 1];
@@ -1161,7 +1298,7 @@ __global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM == 1 ? 4 : 6))
     }
     const JobDev &J = jobs[lo];
     if constexpr (kSpan) {
-        precision_span<TS>(J, tile - J.tile0, links, txs, tap);
+        precision_span<TS, MOV>(J, tile - J.tile0, links, txs, tap);
         return;
     }
     const uint64_t tile_start = ((J.i0 >> TS) + (tile - J.tile0)) << TS;
@@ -1417,6 +1554,7 @@ __global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM == 1 ? 4 : 6))
         const double del0 = ((J.epoch + t0q) - J.epoch) - t0q;
         // the band's part of the carrier phase from the middle sample's
         const double gq = -J.lo * (t_local - s_anc_t[1][1]);
+        const double dn = 0.0, dth = 0.0;   // a fixed receiver (the antenna kernels take no other)
         const bool fit_ok = a_last - a_first >= 2;
         // the group's smooth and gated links as bits (wave 0 holds the group)
         __shared__ uint32_t s_mask[4];
@@ -1426,14 +1564,14 @@ __global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM == 1 ? 4 : 6))
             for (uint32_t q = threadIdx.x; q < 3 * ng; q += kTile) {
                 const uint32_t s = q / 3, a = q - 3 * s;
                 const LinkDev L = links[J.link0 + g0 + s];
-                anchor_solve(s_anc[s], txs[L.tx], J, s_anc_t[a][0], a);
+                anchor_solve(s_anc[s], txs[L.tx], J, s_anc_t[a][0], a, J.obs);
             }
             __syncthreads();
             bool smooth = false, gated = false;
             for (uint32_t s = threadIdx.x; s < ng; s += kTile) {
                 const LinkDev L = links[J.link0 + g0 + s];
                 const TxDev &T = txs[L.tx];
-                smooth = anchor_fit(s_anc[s], L, T, J, s_anc_t, fit_ok);
+                smooth = anchor_fit<false>(s_anc[s], L, T, J, s_anc_t, fit_ok);
                 gated = ANT && gating && g_ok && gate_below_cutoff(J, T, g_tmin, g_tmax, J.obs, 0.0);
                 if (ANT && gating && lx.gate_stats) {
                     if (gated) atomicAdd(&lx.gate_stats[0], 1ull);
@@ -1454,11 +1592,13 @@ __global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM == 1 ? 4 : 6))
             __syncthreads();
             const uint64_t smask = uniform_u64(s_mask[0], s_mask[1]);
             const uint64_t gmask = uniform_u64(s_mask[2], s_mask[3]);
-            precision_links<ANT, 1>(s_anc, smask, gmask, txs, J, g0, ng, &tu, &x, &del0, &gq, &sum_re, &sum_im);
+            precision_links<ANT, 1, false>(s_anc, smask, gmask, txs, J, g0, ng, &tu, &x, &del0, &gq, &dn, &dth,
+                                           J.obs, &sum_re, &sum_im);
             if (tap.on && valid && tap.slot >= g0 && tap.slot < g0 + ng) {
                 const uint32_t ts = tap.slot - g0;
                 double te, r, ph;
-                precision_tap(s_anc[ts], (smask >> ts) & 1u, txs, J, tu, t_local, dlt, x, del0, te, r, ph);
+                precision_tap<false>(s_anc[ts], (smask >> ts) & 1u, txs, J, tu, t_local, dlt, x, del0, dn, dth, te,
+                                     r, ph);
                 tap.t_emit[j] = te;
                 tap.range[j] = r;
                 tap.phase[j] = ph;
@@ -1720,22 +1860,24 @@ namespace lpk {
 template <int FORM, bool ANT>
 using RenderFn = void (*)(const JobDev *, int, const LinkDev *, const TxDev *, RenderLaunch, TapOut);
 
-template <int FORM, bool ANT>
+template <int FORM, bool ANT, bool MOV = false>
 RenderFn<FORM, ANT> render_fn(int tile_shift) {
-    return tile_shift == 7 ? render_kernel<FORM, ANT, 7>
-                           : (tile_shift == 9 ? render_kernel<FORM, ANT, 9> : render_kernel<FORM, ANT, 8>);
+    return tile_shift == 7 ? render_kernel<FORM, ANT, 7, MOV>
+                           : (tile_shift == 9 ? render_kernel<FORM, ANT, 9, MOV> : render_kernel<FORM, ANT, 8, MOV>);
 }
 
-hipError_t launch_render(int form, bool antenna, int tile_shift, bool xcd_map, const JobDev *jobs, int n_jobs,
-                         const LinkDev *links, const TxDev *txs, uint32_t n_tiles, const uint32_t *n_live,
-                         double *tap_t_emit, double *tap_range, double *tap_phase, double *tap_delta_t,
-                         uint32_t tap_slot, const double *pred, unsigned long long *stats, bool gate,
-                         unsigned long long *gate_stats, hipStream_t stream) {
+hipError_t launch_render(int form, bool antenna, bool moving, int tile_shift, bool xcd_map, const JobDev *jobs,
+                         int n_jobs, const LinkDev *links, const TxDev *txs, uint32_t n_tiles,
+                         const uint32_t *n_live, double *tap_t_emit, double *tap_range, double *tap_phase,
+                         double *tap_delta_t, uint32_t tap_slot, const double *pred, unsigned long long *stats,
+                         bool gate, unsigned long long *gate_stats, hipStream_t stream) {
     TapOut tap{tap_t_emit, tap_range, tap_phase, tap_delta_t, tap_slot, tap_t_emit ? 1u : 0u};
     if (tile_shift < 7 || tile_shift > 9 || form < 0 || form > 2) return hipErrorInvalidValue;
+    if (moving && antenna && form == 1) return hipErrorInvalidValue;   // no such kernel
     void (*kernel)(const JobDev *, int, const LinkDev *, const TxDev *, RenderLaunch, TapOut) =
         form == 0   ? (antenna ? render_fn<0, true>(tile_shift) : render_fn<0, false>(tile_shift))
-        : form == 1 ? (antenna ? render_fn<1, true>(tile_shift) : render_fn<1, false>(tile_shift))
+        : form == 1 ? (antenna ? render_fn<1, true>(tile_shift)
+                               : (moving ? render_fn<1, false, true>(tile_shift) : render_fn<1, false>(tile_shift)))
                     : (antenna ? render_fn<2, true>(tile_shift) : render_fn<2, false>(tile_shift));
     for (uint32_t base = 0; base < n_tiles; base += kMaxLaunchTiles) {
         const uint32_t n = n_tiles - base < kMaxLaunchTiles ? n_tiles - base : kMaxLaunchTiles;

@@ -104,11 +104,12 @@ def leo_sky():
     return _SKY
 
 
-def collect(backend, gate, sky, rx, b, start, duration, chunk):
+def collect(backend, gate, sky, rx, b, start, duration, chunk, form="plain"):
     backend.set_horizon_gate(gate)
     try:
         iq = []
-        s = render_window(sky, rx, b, start, duration, WindowOptions(11, chunk, 1), iq.append, backend=backend)
+        s = render_window(sky, rx, b, start, duration, WindowOptions(11, chunk, 1), iq.append, form=form,
+                          backend=backend)
     finally:
         backend.set_horizon_gate(True)
     iq = np.concatenate(iq) if iq else np.zeros(0, dtype=np.complex64)
