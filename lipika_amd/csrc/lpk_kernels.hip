@@ -587,9 +587,6 @@ struct alignas(16) Anchor {
     double r1m, q1d1, q1c2, rd1q;  // r1(tu) = r1m + x (q1d1 + q1c2 x) + rd1q del1 - k01 del0
     double k01;                    // (d r1 / d t1, position and Sagnac) * rd0 / c
     double r1_mid;                 // range at the middle sample (model, or exact when not smooth)
-    // moving receiver: r0 += rxn0 dn + rxt0 dth, r1 += rxn1 dn + rxt1 dth for the
-    // receiver query's departures dn, dth (rx_rounding) in its two angles
-    double rxn0, rxt0, rxn1, rxt1;
     union {
         struct {                   // written by the fit, read by the sample loop
             double ph0;            // carrier phase at the middle sample, turns in [-1/2, 1/2]
@@ -614,7 +611,14 @@ struct alignas(16) Anchor {
     uint32_t tx;                   // transmitter (slow paths read its table)
     double neg_tau_src, tau_if;    // -TAU f_src, TAU (f_src - f_lo), for taps
 };
-static_assert(sizeof(Anchor) == 240, "Anchor layout");
+static_assert(sizeof(Anchor) == 208, "Anchor layout");
+
+// A moving receiver's coefficients of a (tile, transmitter) record, beside
+// its Anchor: r0 += n0 dn + t0 dth and r1 += n1 dn + t1 dth for the receiver
+// query's departures dn, dth (rx_rounding) in its two angles.
+struct RxCoef {
+    double n0, t0, n1, t1;
+};
 static_assert(offsetof(Anchor, h0re) % 16 == 0, "hint values are one aligned 16-byte read");
 constexpr int kAnchorGroup = 64;  // transmitters per anchor pass
 constexpr int kSpanRecords = 128; // (tile, transmitter) records of one anchor batch of a span
@@ -734,7 +738,8 @@ __device__ __forceinline__ void solve_smooth(const TxDev &T, const double *obs, 
 // (rx_rounding), and obs the sample's receiver position where the caller has
 // it (antenna kernels; null otherwise).
 template <int G, bool ANT, int N, bool MOV>
-__device__ __forceinline__ void precision_link(const Anchor &A, bool smooth, const TxDev *__restrict__ txs,
+__device__ __forceinline__ void precision_link(const Anchor &A, const RxCoef *C, bool smooth,
+                                               const TxDev *__restrict__ txs,
                                                const JobDev &J, uint32_t link, const double *tu, const double *x,
                                                const double *del0, const double *gq, const double *dn,
                                                const double *dth, const double *obs, double *sum_re,
@@ -747,8 +752,8 @@ __device__ __forceinline__ void precision_link(const Anchor &A, bool smooth, con
             double r0 = fma(A.rd0, del0[q], fma(x[q], fma(A.q0c2, x[q], A.q0d1), A.r0m));
             double r1 = fma(x[q], fma(A.q1c2, x[q], A.q1d1), A.r1m);
             if (MOV) {
-                r0 = fma(A.rxn0, dn[q], fma(A.rxt0, dth[q], r0));
-                r1 = fma(A.rxn1, dn[q], fma(A.rxt1, dth[q], r1));
+                r0 = fma(C->n0, dn[q], fma(C->t0, dth[q], r0));
+                r1 = fma(C->n1, dn[q], fma(C->t1, dth[q], r1));
             }
             const double t1 = fma(-r0, kInvC, tu[q]);
             const double del1 = ((J.epoch + t1) - J.epoch) - t1;
@@ -875,7 +880,8 @@ __device__ __forceinline__ void precision_link(const Anchor &A, bool smooth, con
 // form: the range as the sample loop forms it, the phase in the reference's
 // f64 operation order (src/channel/mod.rs:255-258) from that range.
 template <bool MOV>
-__device__ __forceinline__ void precision_tap(const Anchor &A, bool smooth, const TxDev *__restrict__ txs,
+__device__ __forceinline__ void precision_tap(const Anchor &A, const RxCoef *C, bool smooth,
+                                              const TxDev *__restrict__ txs,
                                               const JobDev &J, double tu, double t_local, double dlt, double x,
                                               double del0, double dn, double dth, double &te, double &r,
                                               double &ph) {
@@ -883,8 +889,8 @@ __device__ __forceinline__ void precision_tap(const Anchor &A, bool smooth, cons
         double r0 = fma(A.rd0, del0, fma(x, fma(A.q0c2, x, A.q0d1), A.r0m));
         double r1 = fma(x, fma(A.q1c2, x, A.q1d1), A.r1m);
         if (MOV) {
-            r0 = fma(A.rxn0, dn, fma(A.rxt0, dth, r0));
-            r1 = fma(A.rxn1, dn, fma(A.rxt1, dth, r1));
+            r0 = fma(C->n0, dn, fma(C->t0, dth, r0));
+            r1 = fma(C->n1, dn, fma(C->t1, dth, r1));
         }
         const double t1 = fma(-r0, kInvC, tu);
         const double del1 = ((J.epoch + t1) - J.epoch) - t1;
@@ -905,7 +911,7 @@ __device__ __forceinline__ void precision_tap(const Anchor &A, bool smooth, cons
 // receiver coefficients, the effect on r1 including that of r0 through t1.
 __device__ __forceinline__ void anchor_solve(Anchor &A, const TxDev &T, const JobDev &J, double tu, uint32_t a,
                                              const double *obs, const double *pn = nullptr,
-                                             const double *pt = nullptr) {
+                                             const double *pt = nullptr, RxCoef *C = nullptr) {
     const double eoff = T.traj_kind ? J.epoch - T.grid_t0 : 0.0;
     double r0, r1, rd0, rd1q, rd1t, cr[4];
     int64_t c0, c1;
@@ -921,10 +927,10 @@ __device__ __forceinline__ void anchor_solve(Anchor &A, const TxDev &T, const Jo
         A.rd1q = rd1q;
         A.k01 = rd1t * rd0 * kInvC;
         if (pn) {
-            A.rxn0 = cr[0];
-            A.rxt0 = cr[1];
-            A.rxn1 = cr[2] - rd1t * cr[0] * kInvC;
-            A.rxt1 = cr[3] - rd1t * cr[1] * kInvC;
+            C->n0 = cr[0];
+            C->t0 = cr[1];
+            C->n1 = cr[2] - rd1t * cr[0] * kInvC;
+            C->t1 = cr[3] - rd1t * cr[1] * kInvC;
         }
     }
 }
@@ -932,10 +938,10 @@ __device__ __forceinline__ void anchor_solve(Anchor &A, const TxDev &T, const Jo
 // The fit of one (tile, transmitter) record from its anchor pass; at[a] holds
 // (tu, t_local, delta_t) of anchor a. Returns whether the tile is smooth for
 // the transmitter (the lanes then use the fitted range, else the exact solve).
-// MOV: the receiver may move (the record then holds its coefficients).
+// MOV: the receiver may move (C then holds the record's coefficients).
 template <bool MOV>
-__device__ __forceinline__ bool anchor_fit(Anchor &A, const LinkDev &L, const TxDev &T, const JobDev &J,
-                                           const double (*at)[3], bool fit_ok) {
+__device__ __forceinline__ bool anchor_fit(Anchor &A, const RxCoef *C, const LinkDev &L, const TxDev &T,
+                                           const JobDev &J, const double (*at)[3], bool fit_ok) {
     const bool mov = MOV && J.rx_kind != 0;
     // the anchor pass's results share storage with fields written below
     const double r0s[3] = {A.r0s[0], A.r0s[1], A.r0s[2]};
@@ -962,8 +968,8 @@ __device__ __forceinline__ bool anchor_fit(Anchor &A, const LinkDev &L, const Tx
         if (mov) {
             double dn, dth;
             rx_rounding(J, at[1][0], dn, dth);
-            r0m = fma(A.rxn0, dn, fma(A.rxt0, dth, r0m));
-            r1c = fma(A.rxn1, dn, fma(A.rxt1, dth, r1c));
+            r0m = fma(C->n0, dn, fma(C->t0, dth, r0m));
+            r1c = fma(C->n1, dn, fma(C->t1, dth, r1c));
         }
         const double t1m = fma(-r0m, kInvC, at[1][0]);
         const double d1m = ((J.epoch + t1m) - J.epoch) - t1m;
@@ -1029,7 +1035,7 @@ __device__ __forceinline__ bool anchor_fit(Anchor &A, const LinkDev &L, const Tx
 // set in gmask skipped (antenna kernels), the fitted range for those whose
 // bit is set in smask.
 template <bool ANT, int N, bool MOV>
-__device__ __forceinline__ void precision_links(const Anchor *R, uint64_t smask, uint64_t gmask,
+__device__ __forceinline__ void precision_links(const Anchor *R, const RxCoef *RC, uint64_t smask, uint64_t gmask,
                                                 const TxDev *__restrict__ txs, const JobDev &J, uint32_t g0,
                                                 uint32_t ng, const double *tu, const double *x, const double *del0,
                                                 const double *gq, const double *dn, const double *dth,
@@ -1040,19 +1046,19 @@ __device__ __forceinline__ void precision_links(const Anchor *R, uint64_t smask,
     uint32_t s = 0;
     for (; s < e_fr; ++s)
         if (!ANT || !((gmask >> s) & 1u))
-            precision_link<kGroupFramed, ANT, N, MOV>(R[s], (smask >> s) & 1u, txs, J, g0 + s, tu, x, del0, gq, dn, dth,
+            precision_link<kGroupFramed, ANT, N, MOV>(R[s], MOV ? RC + s : nullptr, (smask >> s) & 1u, txs, J, g0 + s, tu, x, del0, gq, dn, dth,
                                                  obs, sum_re, sum_im);
     for (; s < e_cw; ++s)
         if (!ANT || !((gmask >> s) & 1u))
-            precision_link<kGroupCw, ANT, N, MOV>(R[s], (smask >> s) & 1u, txs, J, g0 + s, tu, x, del0, gq, dn, dth, obs, sum_re,
+            precision_link<kGroupCw, ANT, N, MOV>(R[s], MOV ? RC + s : nullptr, (smask >> s) & 1u, txs, J, g0 + s, tu, x, del0, gq, dn, dth, obs, sum_re,
                                              sum_im);
     for (; s < e_gn; ++s)
         if (!ANT || !((gmask >> s) & 1u))
-            precision_link<kGroupGeneric, ANT, N, MOV>(R[s], (smask >> s) & 1u, txs, J, g0 + s, tu, x, del0, gq, dn, dth,
+            precision_link<kGroupGeneric, ANT, N, MOV>(R[s], MOV ? RC + s : nullptr, (smask >> s) & 1u, txs, J, g0 + s, tu, x, del0, gq, dn, dth,
                                                   obs, sum_re, sum_im);
     for (; s < ng; ++s)
         if (!ANT || !((gmask >> s) & 1u))
-            precision_link<kGroupCa, ANT, N, MOV>(R[s], (smask >> s) & 1u, txs, J, g0 + s, tu, x, del0, gq, dn, dth, obs, sum_re,
+            precision_link<kGroupCa, ANT, N, MOV>(R[s], MOV ? RC + s : nullptr, (smask >> s) & 1u, txs, J, g0 + s, tu, x, del0, gq, dn, dth, obs, sum_re,
                                              sum_im);
 }
 
@@ -1103,6 +1109,7 @@ __device__ __forceinline__ void precision_span(const JobDev &J, uint32_t span, c
     __shared__ double s_at[kSub][3][3];           // (tu, t_local, delta_t) at each tile's anchors
     __shared__ double s_rx[MOV ? kSub : 1][3][3];  // moving receiver: its smooth position at the anchors
     __shared__ double s_rxp[MOV ? kSub : 1][2][3]; // and its derivatives along nu and theta at the middle one
+    __shared__ RxCoef s_rxc[MOV ? kSpanRecords : 1]; // and each record's coefficients
     __shared__ uint32_t s_mask[kSub][2];          // smooth links of each tile of the batch
     __shared__ uint8_t s_smooth[kSpanRecords];     // per record: the tile is smooth for the link
     const uint64_t span0 = ((J.i0 >> (TS + kSuperShift)) + span) << (TS + kSuperShift);
@@ -1150,7 +1157,9 @@ __device__ __forceinline__ void precision_span(const JobDev &J, uint32_t span, c
                 const uint64_t t0 = span0 + uint64_t(k0 + kk) * kTile;
                 if ((J.i0 > t0 ? J.i0 : t0) >= (jend < t0 + kTile ? jend : t0 + kTile)) continue;
                 const LinkDev L = links[J.link0 + g0 + (rec - kk * ng)];
-                if (MOV) anchor_solve(s_rec[rec], txs[L.tx], J, s_at[kk][a][0], a, s_rx[kk][a], s_rxp[kk][0], s_rxp[kk][1]);
+                if (MOV)
+                    anchor_solve(s_rec[rec], txs[L.tx], J, s_at[kk][a][0], a, s_rx[kk][a], s_rxp[kk][0], s_rxp[kk][1],
+                                 &s_rxc[rec]);
                 else anchor_solve(s_rec[rec], txs[L.tx], J, s_at[kk][a][0], a, J.obs);
             }
             __syncthreads();
@@ -1162,7 +1171,8 @@ __device__ __forceinline__ void precision_span(const JobDev &J, uint32_t span, c
                 bool smooth = false;
                 if (first < end) {
                     const LinkDev L = links[J.link0 + g0 + (rec - kk * ng)];
-                    smooth = anchor_fit<MOV>(s_rec[rec], L, txs[L.tx], J, s_at[kk], end - 1 - first >= 2);
+                    smooth = anchor_fit<MOV>(s_rec[rec], MOV ? &s_rxc[rec] : nullptr, L, txs[L.tx], J, s_at[kk],
+                                             end - 1 - first >= 2);
                 }
                 s_smooth[rec] = smooth ? 1u : 0u;
             }
@@ -1211,7 +1221,8 @@ __device__ __forceinline__ void precision_span(const JobDev &J, uint32_t span, c
                 }
                 const uint64_t smask = uniform_u64(s_mask[kk][0], s_mask[kk][1]);
                 const Anchor *R = s_rec + kk * ng;
-                precision_links<false, kSpanSamples, MOV>(R, smask, 0ull, txs, J, g0, ng, tu, x, del0, gq, dn, dth,
+                const RxCoef *RC = MOV ? s_rxc + kk * ng : nullptr;
+                precision_links<false, kSpanSamples, MOV>(R, RC, smask, 0ull, txs, J, g0, ng, tu, x, del0, gq, dn, dth,
                                                           nullptr, sum_re, sum_im);
                 if (tap.on) {
                     // the sample times and clock offsets again (as above), for the taps
@@ -1226,7 +1237,7 @@ __device__ __forceinline__ void precision_span(const JobDev &J, uint32_t span, c
                         if (tap.slot >= g0 && tap.slot < g0 + ng) {
                             const uint32_t ts = tap.slot - g0;
                             double te, r, ph;
-                            precision_tap<MOV>(R[ts], (smask >> ts) & 1u, txs, J, tu[q], tl, dl, x[q], del0[q], dn[q],
+                            precision_tap<MOV>(R[ts], MOV ? RC + ts : nullptr, (smask >> ts) & 1u, txs, J, tu[q], tl, dl, x[q], del0[q], dn[q],
                                                dth[q], te, r, ph);
                             tap.t_emit[i[q] - J.i0] = te;
                             tap.range[i[q] - J.i0] = r;
@@ -1278,7 +1289,7 @@ struct RenderLaunch {
 };
 
 template <int FORM, bool ANT, int TS, bool MOV>
-__global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM == 1 ? 4 : 6))
+__global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM == 1 ? (MOV ? 3 : 4) : 6))
     render_kernel(const JobDev *__restrict__ jobs, int n_jobs, const LinkDev *__restrict__ links,
                   const TxDev *__restrict__ txs, RenderLaunch lx, TapOut tap) {
     constexpr int kTile = 1 << TS;
@@ -1571,7 +1582,7 @@ __global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM == 1 ? 4 : 6))
             for (uint32_t s = threadIdx.x; s < ng; s += kTile) {
                 const LinkDev L = links[J.link0 + g0 + s];
                 const TxDev &T = txs[L.tx];
-                smooth = anchor_fit<false>(s_anc[s], L, T, J, s_anc_t, fit_ok);
+                smooth = anchor_fit<false>(s_anc[s], nullptr, L, T, J, s_anc_t, fit_ok);
                 gated = ANT && gating && g_ok && gate_below_cutoff(J, T, g_tmin, g_tmax, J.obs, 0.0);
                 if (ANT && gating && lx.gate_stats) {
                     if (gated) atomicAdd(&lx.gate_stats[0], 1ull);
@@ -1592,12 +1603,12 @@ __global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM == 1 ? 4 : 6))
             __syncthreads();
             const uint64_t smask = uniform_u64(s_mask[0], s_mask[1]);
             const uint64_t gmask = uniform_u64(s_mask[2], s_mask[3]);
-            precision_links<ANT, 1, false>(s_anc, smask, gmask, txs, J, g0, ng, &tu, &x, &del0, &gq, &dn, &dth,
+            precision_links<ANT, 1, false>(s_anc, nullptr, smask, gmask, txs, J, g0, ng, &tu, &x, &del0, &gq, &dn, &dth,
                                            J.obs, &sum_re, &sum_im);
             if (tap.on && valid && tap.slot >= g0 && tap.slot < g0 + ng) {
                 const uint32_t ts = tap.slot - g0;
                 double te, r, ph;
-                precision_tap<false>(s_anc[ts], (smask >> ts) & 1u, txs, J, tu, t_local, dlt, x, del0, dn, dth, te,
+                precision_tap<false>(s_anc[ts], nullptr, (smask >> ts) & 1u, txs, J, tu, t_local, dlt, x, del0, dn, dth, te,
                                      r, ph);
                 tap.t_emit[j] = te;
                 tap.range[j] = r;
