@@ -76,7 +76,8 @@ class Scene:
     def __init__(self, n, chunk=1 << 20, max_chunks=None, noise=True):
         from lipika_amd import _lib, workloads
         from lipika_amd.pipeline import Backend, prepare
-        sc = workloads.micro() if n == "micro" else workloads.scene(n)
+        sc = (workloads.micro() if n == "micro" else workloads.scene3_orbiting() if n == "3 orbiting"
+              else workloads.scene(3, 10.0) if n == "3 10 s" else workloads.scene(n))
         if not noise:
             for rx in sc.receivers:
                 for b in rx.bands:
@@ -417,6 +418,23 @@ def micro_leg(steps, warmup, cpu):
     return out
 
 
+def moving_leg(steps, warmup):
+    """Moving receivers: configuration 3's sky and bands for 10 s with the 16
+    receivers fixed (as configuration 3) and with each on a LEO orbit, both
+    forms, every chunk in one render call."""
+    out = {"workload": "configuration 3's 32 C/A MEOs and L1 bands at 2.048 MHz, 10 s, 16 receivers fixed and "
+                       "on circular LEO orbits (6 900-7 200 km)"}
+    for key, n in (("fixed", "3 10 s"), ("orbiting", "3 orbiting")):
+        S = Scene(n)
+        out[key] = {"contributions_per_step": S.contributions,
+                    "forms": {f: time_form(S, f, steps, warmup) for f in ("plain", "precision")}}
+        S.close()
+    out["precision_vs_plain_orbiting"] = round(out["orbiting"]["forms"]["precision"]["value"] /
+                                               out["orbiting"]["forms"]["plain"]["value"], 3)
+    log(f"[bench] moving receivers: {json.dumps(out)}")
+    return out
+
+
 def roofline(contributions, kernel_ms, n):
     achieved = contributions * FLOPS_PER_CONTRIBUTION / (kernel_ms / 1e3) / 1e12
     traffic, source = TRAFFIC.get(n, (None, None))
@@ -531,6 +549,7 @@ def main():
     res["config"]["config2"] = c2
     res["config"]["dropin"] = dropin_leg(2)
     res["config"]["micro"] = micro_leg(K, W, not args.no_cpu)
+    res["config"]["moving"] = moving_leg(K, W)
     res["config"]["config5"] = learned_leg(max(1, K // 10), 1)
     print(json.dumps(res), flush=True)
 

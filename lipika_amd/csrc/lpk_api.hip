@@ -792,6 +792,10 @@ JobDev fill_job(const lpk_ctx *ctx, const lpk_band_chunk &B, uint64_t i0, uint64
             o[0] = s[0] + lo;
             o[1] = lo - (o[0] - s[0]);
         };
+        const double w = J.rk_n + kEarthRotation;
+        J.rx_v = a * w;
+        J.rx_a = a * w * w;
+        J.rx_j = a * w * w * w;
         two_sum(J.epoch, -J.rk_epoch, J.rk_e);
         scaled(J.rk_n, J.rk_nu0, J.rk_nue);
         scaled(kEarthRotation, 0.0, J.rk_the);

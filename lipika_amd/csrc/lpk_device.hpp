@@ -204,6 +204,9 @@ struct JobDev {
     // E + tu, E = epoch - rk_epoch: E, rk_nu0 + rk_n E and omega E, each as
     // a double-double (hi, lo)
     double rk_e[2], rk_nue[2], rk_the[2];
+    // bounds of the receiver's speed, acceleration and jerk in the Earth-fixed
+    // frame (0 for a fixed receiver): a w, a w^2, a w^3 with w = n + omega
+    double rx_v, rx_a, rx_j;
     // clock
     const double *delta_t;   // host-provided offsets (device copy) or null
     const double *walk_base; // per 65 536-index block, absolute block index

@@ -947,7 +947,20 @@ __device__ __forceinline__ bool anchor_fit(Anchor &A, const RxCoef *C, const Lin
     const double r0s[3] = {A.r0s[0], A.r0s[1], A.r0s[2]};
     const double r1s[3] = {A.r1s[0], A.r1s[1], A.r1s[2]};
     const bool same = A.cell[0][0] == A.cell[1][0] && A.cell[0][1] == A.cell[1][1];
-    const bool smooth = fit_ok && (T.traj_kind == 0 || same);
+    // The quadratic's truncation over the tile, |r'''| h^3 / (9 sqrt 3) with
+    // |r'''| <= j + 6 v a / r + 3 v^3 / r^2 for bounds v, a, j of the relative
+    // motion (the transmitter's grid is linear within a cell; the receiver's
+    // bounds come with the job), held to 1e-7 rad of carrier phase; and the
+    // amplitude's first-order model in the range, (v h / r)^2 <= 1e-7. A tile
+    // outside either bound (long tiles of low-rate bands seen from a moving
+    // receiver, close passes) takes the exact per-sample solve; NaN bounds fail.
+    const double h = 0.5 * (at[2][0] - at[0][0]);
+    const double v = T.max_speed + J.rx_v;
+    const double rmin = fmin(fmin(r0s[0], r0s[1]), fmin(r0s[2], fmin(r1s[0], fmin(r1s[1], r1s[2])))) - v * h;
+    const double r3 = J.rx_j + (6.0 * v * J.rx_a + 3.0 * v * v * v / rmin) / rmin;
+    const bool bounded = rmin > 0.0 && v * h <= 3.2e-4 * rmin &&
+                         fabs(T.neg_tau_src) * r3 * (h * h * h) * 0.06415002990995841 <= 1e-7 * kSpeedOfLight;
+    const bool smooth = fit_ok && (T.traj_kind == 0 || same) && bounded;
     double r1m;
     A.r0m = r0s[1];
     A.r1m = r1s[1];

@@ -145,6 +145,18 @@ def micro(duration_s: float = 5.0) -> SceneConfig:
     return SceneConfig(SceneMeta(EPOCH, duration_s), [tx], [rx])
 
 
+def scene3_orbiting(duration_s: float = 10.0, n_rx: int = 16) -> SceneConfig:
+    """Configuration 3's sky and bands with every receiver on a circular LEO
+    orbit (radius 6 900 to 7 200 km, inclination, node and anomaly uniform),
+    its elements' epoch the scene's: the moving-receiver workload."""
+    sc = scene3(duration_s, n_rx)
+    rng = np.random.default_rng(SEED + 33)
+    sc.receivers = [ReceiverConfig(rx.name, _kepler(float(rng.uniform(6.9e6, 7.2e6)), float(rng.uniform(0, 180)),
+                                                    float(rng.uniform(0, 360)), float(rng.uniform(0, 360))),
+                                   rx.clock, rx.bands) for rx in sc.receivers]
+    return sc
+
+
 def scene(n: int, duration_s: Optional[float] = None, **kw) -> SceneConfig:
     f = {1: scene1, 2: scene2, 3: scene3, 4: scene4, 5: scene5}[n]
     return f(**kw) if duration_s is None else f(duration_s, **kw)

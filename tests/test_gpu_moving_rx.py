@@ -124,6 +124,33 @@ def test_moving_receiver_precision_form_against_oracle(backend, which):
                      label=f"moving receiver ({which}), precision")
 
 
+@pytest.mark.parametrize("fs", [1_000.0, 48_000.0])
+def test_moving_receiver_low_rate_band_precision(backend, fs):
+    """A ground tone and a LEO BPSK seen from the orbiter in a UHF band at
+    1 kHz (tiles of half a second, over which the quadratic would miss the
+    receiver's orbit by tens of microradians of phase, so the fit's
+    truncation bound sends them to the exact solve) and at 48 kHz: IQ within
+    2e-6 of the peak, taps within 1e-6 rad and 1 ps."""
+    txs = [TransmitterConfig("tone", 437e6 + 100.0, Cw(25.0), StationaryGeodetic(40.0, 20.0, 100.0), 10.0),
+           TransmitterConfig("leo", 437e6 - 200.0, Bpsk(50.0, 480, 7, [1, -1, 1, 1]),
+                             CircularKeplerian(7.1e6, 60.0, 330.0, 45.0, util.EPOCH), 0.0)]
+    band_ = ReceiverBandConfig("uhf", 437e6, fs, noise_power_w=0.0)
+    rx = ReceiverConfig("orbiter", orbiter(util.EPOCH), NodeClockSpec(allan_deviation_1s=1e-11), [band_])
+    sc = prepare(SceneConfig(SceneMeta(util.EPOCH, 20.0 if fs < 10e3 else 1.0), txs, [rx]))
+    seed = 9
+    backend.upload(sc, seed)
+    orc = OracleScene(sc, seed)
+    o, _, taps = orc.render_band(0, tap_slot=0)
+    g, _ = gpu_band(backend, sc, 0, 0, 0, seed, delta_t=taps[3], form=_lib.FORM_PRECISION)
+    peak = float(np.max(np.abs(o)))
+    d = np.abs(g.astype(np.complex128) - o.astype(np.complex128))
+    _say(f"orbiter, {fs:.0f} Hz band, precision: n={len(o)} max|d|/peak={d.max() / peak:.3e}")
+    assert d.max() <= 2e-6 * peak
+    for slot in range(2):
+        compare_taps(backend, orc, sc, 0, 0, 0, seed, slot, sc.receivers[0].bands[0].n, form=_lib.FORM_PRECISION,
+                     label=f"orbiter, {fs:.0f} Hz band")
+
+
 def test_precision_call_with_moving_and_fixed_receivers(backend):
     """One precision render call holding the orbiter's band and a ground
     receiver's band (the moving-receiver kernels take the fixed receiver with
