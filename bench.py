@@ -8,8 +8,8 @@ C/A code, on seeded LEO and MEO circular orbits, carriers 437 MHz,
 1 Mi-sample chunks: 9 879 chunk jobs, all of them in one render call, the IQ
 kept in HBM. The headline form is the precision form (the north-star target
 form); the plain form (the reference's f64 path, operation for operation) and
-the batching form (a streaming plan whose chunks replay as HIP graphs) are
-timed beside it. Synthetic geometry and payloads; inputs are resident in HBM
+the batching form (a streaming plan: one step per chunk, alternating between
+two streams) are timed beside it. Synthetic geometry and payloads; inputs are resident in HBM
 before the timed region starts.
 
 Metric: contributions/s = sum over (receiver band, in-band transmitter) of
@@ -146,8 +146,8 @@ def time_form(S, name, steps, warmup):
 
 def plan_leg(S, form, steps, warmup, chunk):
     """The batching form: a streaming plan renders chunk k of every band per
-    step as one graph replay, with the context's (autotuned) tuning; a pass is
-    every step of the scene."""
+    step (steps alternate between two streams), with the context's
+    (autotuned) tuning; a pass is every step of the scene."""
     from lipika_amd.pipeline import form_code
     be, lib = S.be, S.lib
     tuning = be.tuning()
@@ -169,15 +169,13 @@ def plan_leg(S, form, steps, warmup, chunk):
             one_pass()
         be.sync()
         t1 = time.perf_counter()
-        kern_ms, n_launch = C.c_double(), C.c_uint64()
-        be.check(lib.lpk_timing_read(be.h, C.byref(kern_ms), C.byref(n_launch)))
         ms = (t1 - t0) * 1e3 / steps
-        log(f"[bench] config {S.n} plan ({form}, {chunk}-sample chunks): {plan.n_chunks} graph replays per pass: "
-            f"{ms:.3f} ms per pass; graphs {kern_ms.value / steps:.3f} ms per pass")
+        log(f"[bench] config {S.n} plan ({form}, {chunk}-sample chunks): {plan.n_chunks} steps per pass: "
+            f"{ms:.3f} ms per pass")
         return {"form": form, "tile_shift": tuning[0], "xcd_map": tuning[1], "chunk_samples": chunk,
-                "graph_replays_per_step": plan.n_chunks, "max_tiles_per_replay": plan.max_tiles,
+                "plan_steps_per_pass": plan.n_chunks, "max_workgroups_per_step": plan.max_tiles,
                 "value": round(S.contributions / (ms / 1e3) / 1e6, 3),
-                "ms_per_step": round(ms, 4), "graph_ms_per_step": round(kern_ms.value / steps, 4)}
+                "ms_per_step": round(ms, 4)}
 
 
 def dropin_leg(steps):

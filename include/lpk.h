@@ -245,13 +245,18 @@ int lpk_autotune(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int 
  *      each band's [i0, i0 + clen) in step k; outputs do not depend on the
  *      chunking, src/pipeline.rs:905-990 and tests/streaming_chunk_invariance.rs).
  *      The job, clock walk and carry tables of the step go to HBM and the
- *      walk, carry and render kernels run, all as one captured HIP graph
- *      replayed per step.
+ *      walk, carry and render kernels run: even steps on the context's
+ *      stream, odd steps on a second stream of the plan, so that a step's
+ *      walk and render start while the previous step's render finishes
+ *      (the walk carries run in step order across the two). Every other
+ *      entry point of the context, lpk_sync included, orders itself after
+ *      the steps already made.
  *      Each band descriptor covers the whole band (i0 = 0, clen = its
  *      sample count) with the device clock (delta_t NULL) and no host
  *      outputs or flags; out_dev, if set, holds the whole band and step k
- *      writes its chunk there, otherwise the plan keeps one chunk buffer per
- *      band (lpk_plan_output). Steps run in order from 0; step 0 restarts
+ *      writes its chunk there, otherwise the plan keeps two chunk buffers per
+ *      band, one per stream, and lpk_plan_output gives the one the last
+ *      step wrote. Steps run in order from 0; step 0 restarts
  *      the plan's receivers' clock walks. The tuning is the context's at
  *      creation. ---- */
 typedef struct lpk_plan lpk_plan;

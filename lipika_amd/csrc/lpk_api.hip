@@ -183,12 +183,21 @@ struct lpk_ctx {
     int gate = 1;   // apply the horizon gate in antenna scenes
     PinSlot pin[4];
     int pin_next = 0;
+    // live streaming plans: their odd steps run on a stream of their own,
+    // which every other entry point joins (the context's stream waits on the
+    // plan's last odd step) before it enqueues work
+    std::vector<lpk_plan *> plans;
+    bool foreign = false;   // other work was enqueued since the plans' last step
 };
 
-// A streaming plan (lpk_plan_*): chunk k of every band per step, as one
-// captured graph of [table copy, clock walk, carry, render] replayed per step.
-// Two graphs alternate so the host can fill one pinned table while the other
-// is in flight.
+// A streaming plan (lpk_plan_*): chunk k of every band per step as
+// [table copy, clock walk, carry, render], the even steps on the context's
+// stream and the odd ones on the plan's second stream, so that a step's
+// clock walk and the start of its render overlap the previous step's render
+// tail. The two slots have their own pinned and device tables, walk windows,
+// block sums and chunk buffers; the carries, which extend each receiver's
+// shared base table, run in step order across the streams (an event per
+// slot), and a restart at chunk 0 waits for both slots.
 struct lpk_plan {
     lpk_ctx *ctx = nullptr;
     int form = 0, tile_shift = kDefaultTileShift, xcd_map = 1;
@@ -204,18 +213,20 @@ struct lpk_plan {
     uint64_t total_links = 0;
     std::vector<uint32_t> walk_rx;        // receivers whose device clock the plan walks
     std::vector<int> walk_slot;           // per receiver: its index in walk_rx, or -1
-    std::vector<DevBuf> local, toff;      // per walked receiver: the step's window
+    std::vector<DevBuf> local[2], toff[2];  // per slot and walked receiver: the step's window
     std::vector<int64_t> win_cap;         // per walked receiver: window capacity in blocks
-    DevBuf blocksum;
+    DevBuf blocksum[2];
     uint32_t max_blocks = 0, max_tiles = 0;
-    std::vector<DevBuf> outbuf;           // per band: the chunk buffer when out_dev is NULL
+    std::vector<DevBuf> outbuf[2];        // per slot and band: the chunk buffer when out_dev is NULL
     size_t off_jobs = 0, off_links = 0, off_flags = 0, off_segs = 0, off_carry = 0, bytes = 0;
-    DevBuf tables;
+    DevBuf tables[2];
     void *pin[2] = {nullptr, nullptr};
     hipEvent_t done[2] = {nullptr, nullptr};
+    hipEvent_t carry_ev[2] = {nullptr, nullptr};  // each slot's last walk carry
+    hipEvent_t ctx_ev = nullptr;          // the context's stream, when other work came between steps
+    hipStream_t aux = nullptr;            // the odd steps' stream
+    bool aux_pending = false;             // odd steps not yet joined into the context's stream
     bool used[2] = {false, false};
-    hipGraph_t graph[2] = {nullptr, nullptr};
-    hipGraphExec_t exec[2] = {nullptr, nullptr};
 };
 
 // A warm-start network on its context's device: FP32 master parameters, the
@@ -239,9 +250,22 @@ int fail(lpk_ctx *ctx, const Fail &f) {
     return f.code;
 }
 
+// Orders the context's stream after every plan step enqueued on a plan's
+// second stream, and notes that other work follows the plans' steps (the
+// next odd step then waits for it).
+void join_plans(lpk_ctx *ctx) {
+    for (lpk_plan *P : ctx->plans)
+        if (P->aux_pending) {
+            hip_check(hipStreamWaitEvent(ctx->stream, P->done[1], 0), "plan join");
+            P->aux_pending = false;
+        }
+    ctx->foreign = true;
+}
+
 template <class F>
-int guarded(lpk_ctx *ctx, F &&body) {
+int guarded(lpk_ctx *ctx, F &&body, bool join = true) {
     try {
+        if (join && ctx) join_plans(ctx);
         body();
         return LPK_OK;
     } catch (const Fail &f) {
@@ -837,7 +861,7 @@ struct TapReq {
 };
 
 // Next timing event pair of the context (bench accounting, lpk_timing_*).
-void timing_begin(lpk_ctx *ctx) {
+void timing_begin(lpk_ctx *ctx, hipStream_t s = nullptr) {
     if (ctx->tused == ctx->tpool.size() && ctx->tused < 65536) {
         std::pair<hipEvent_t, hipEvent_t> pr{nullptr, nullptr};
         hip_check(hipEventCreate(&pr.first), "event create");
@@ -849,10 +873,10 @@ void timing_begin(lpk_ctx *ctx) {
         ctx->ev1 = ctx->tpool[ctx->tused].second;
         ++ctx->tused;
     }
-    hip_check(hipEventRecord(ctx->ev0, ctx->stream), "event record");
+    hip_check(hipEventRecord(ctx->ev0, s ? s : ctx->stream), "event record");
 }
-void timing_end(lpk_ctx *ctx) {
-    hip_check(hipEventRecord(ctx->ev1, ctx->stream), "event record");
+void timing_end(lpk_ctx *ctx, hipStream_t s = nullptr) {
+    hip_check(hipEventRecord(ctx->ev1, s ? s : ctx->stream), "event record");
     ctx->timed = true;
 }
 
@@ -1065,9 +1089,9 @@ bool plan_window(const lpk_plan *P, uint32_t r, uint64_t k, int64_t &lo, int64_t
 
 // Fills the pinned table of step k: header (live tiles, live walk blocks),
 // jobs, links, zeroed flags, walk segments and carries.
-void plan_fill(lpk_plan *P, uint64_t k, char *dst) {
+void plan_fill(lpk_plan *P, uint64_t k, int slot, char *dst) {
     lpk_ctx *ctx = P->ctx;
-    char *dev = static_cast<char *>(P->tables.p);
+    char *dev = static_cast<char *>(P->tables[slot].p);
     uint32_t *hdr = reinterpret_cast<uint32_t *>(dst);
     JobDev *jh = reinterpret_cast<JobDev *>(dst + P->off_jobs);
     LinkDev *lh = reinterpret_cast<LinkDev *>(dst + P->off_links);
@@ -1075,7 +1099,7 @@ void plan_fill(lpk_plan *P, uint64_t k, char *dst) {
     uint32_t *flags_dev = reinterpret_cast<uint32_t *>(dev + P->off_flags);
     WalkSegDev *sh = reinterpret_cast<WalkSegDev *>(dst + P->off_segs);
     CarrySegDev *ch = reinterpret_cast<CarrySegDev *>(dst + P->off_carry);
-    double *bs = static_cast<double *>(P->blocksum.p);
+    double *bs = static_cast<double *>(P->blocksum[slot].p);
     // clock walks
     std::vector<int64_t> k0(P->walk_rx.size(), 0), k1(P->walk_rx.size(), 0);
     uint64_t blocks = 0;
@@ -1085,8 +1109,8 @@ void plan_fill(lpk_plan *P, uint64_t k, char *dst) {
         S.seed = R.d.clock_seed;
         S.sigma = R.sigma_step;
         S.block0 = uint32_t(blocks);
-        S.local = static_cast<float *>(P->local[w].p);
-        S.toff = static_cast<double *>(P->toff[w].p);
+        S.local = static_cast<float *>(P->local[slot][w].p);
+        S.toff = static_cast<double *>(P->toff[slot][w].p);
         S.blocksum = bs + blocks;
         CarrySegDev C{};
         C.base = R.base;
@@ -1132,12 +1156,12 @@ void plan_fill(lpk_plan *P, uint64_t k, char *dst) {
         if (J.walk) {
             const int w = P->walk_slot[B.rx];
             J.walk_base = ctx->rx[B.rx].base;
-            J.walk_toff = static_cast<const double *>(P->toff[w].p);
-            J.walk_local = static_cast<const float *>(P->local[w].p);
+            J.walk_toff = static_cast<const double *>(P->toff[slot][w].p);
+            J.walk_local = static_cast<const float *>(P->local[slot][w].p);
             J.eps_k0 = k0[w];
             set_walk_rel(J, ctx->rx[B.rx].base, k1[w]);
         }
-        J.out = B.out_dev ? static_cast<float *>(B.out_dev) + 2 * i0 : static_cast<float *>(P->outbuf[b].p);
+        J.out = B.out_dev ? static_cast<float *>(B.out_dev) + 2 * i0 : static_cast<float *>(P->outbuf[slot][b].p);
         *jh++ = J;
     }
     if (tiles > P->max_tiles) throw Fail{LPK_E_STATE, "step beyond the plan's tile capacity"};
@@ -1147,18 +1171,24 @@ void plan_fill(lpk_plan *P, uint64_t k, char *dst) {
 
 void plan_free(lpk_plan *P) {
     if (!P) return;
-    if (P->ctx) (void)hipStreamSynchronize(P->ctx->stream);
-    for (int i = 0; i < 2; ++i) {
-        if (P->exec[i]) (void)hipGraphExecDestroy(P->exec[i]);
-        if (P->graph[i]) (void)hipGraphDestroy(P->graph[i]);
-        if (P->done[i]) (void)hipEventDestroy(P->done[i]);
-        if (P->pin[i]) (void)hipHostFree(P->pin[i]);
+    if (P->ctx) {
+        (void)hipStreamSynchronize(P->ctx->stream);
+        auto &v = P->ctx->plans;
+        v.erase(std::remove(v.begin(), v.end(), P), v.end());
     }
-    for (auto &d : P->local) d.release();
-    for (auto &d : P->toff) d.release();
-    for (auto &d : P->outbuf) d.release();
-    P->blocksum.release();
-    P->tables.release();
+    if (P->aux) (void)hipStreamSynchronize(P->aux);
+    for (int i = 0; i < 2; ++i) {
+        if (P->done[i]) (void)hipEventDestroy(P->done[i]);
+        if (P->carry_ev[i]) (void)hipEventDestroy(P->carry_ev[i]);
+        if (P->pin[i]) (void)hipHostFree(P->pin[i]);
+        for (auto &d : P->local[i]) d.release();
+        for (auto &d : P->toff[i]) d.release();
+        for (auto &d : P->outbuf[i]) d.release();
+        P->blocksum[i].release();
+        P->tables[i].release();
+    }
+    if (P->ctx_ev) (void)hipEventDestroy(P->ctx_ev);
+    if (P->aux) (void)hipStreamDestroy(P->aux);
     delete P;
 }
 
@@ -1245,20 +1275,22 @@ lpk_plan *plan_create(lpk_ctx *ctx, const lpk_band_chunk *bands, uint32_t n_band
         P->max_blocks = std::max<uint32_t>(P->max_blocks, uint32_t(blocks));
         P->max_tiles = std::max<uint32_t>(P->max_tiles, uint32_t(tiles));
     }
-    // device buffers
-    P->local.resize(nw);
-    P->toff.resize(nw);
-    for (size_t w = 0; w < nw; ++w) {
-        RxState &R = ctx->rx[P->walk_rx[w]];
-        ensure_base(ctx, R, max_bhi[w]);
-        const size_t cap = size_t(std::max<int64_t>(P->win_cap[w], 1));
-        P->local[w].ensure(cap * size_t(kWalkBlock) * sizeof(float), ctx->stream);
-        P->toff[w].ensure(cap * size_t(kWalkBlock >> kWalkSubShift) * sizeof(double), ctx->stream);
+    // device buffers, one set per slot
+    for (size_t w = 0; w < nw; ++w) ensure_base(ctx, ctx->rx[P->walk_rx[w]], max_bhi[w]);
+    for (int i = 0; i < 2; ++i) {
+        P->local[i].resize(nw);
+        P->toff[i].resize(nw);
+        for (size_t w = 0; w < nw; ++w) {
+            const size_t cap = size_t(std::max<int64_t>(P->win_cap[w], 1));
+            P->local[i][w].ensure(cap * size_t(kWalkBlock) * sizeof(float), ctx->stream);
+            P->toff[i][w].ensure(cap * size_t(kWalkBlock >> kWalkSubShift) * sizeof(double), ctx->stream);
+        }
+        P->blocksum[i].ensure(size_t(std::max<uint32_t>(P->max_blocks, 1)) * sizeof(double), ctx->stream);
+        P->outbuf[i].resize(n_bands);
+        for (uint32_t b = 0; b < n_bands; ++b)
+            if (!P->bands[b].out_dev)
+                P->outbuf[i][b].ensure(size_t(std::max<uint64_t>(P->chunk_cap[b], 1)) * 8, ctx->stream);
     }
-    P->blocksum.ensure(size_t(std::max<uint32_t>(P->max_blocks, 1)) * sizeof(double), ctx->stream);
-    P->outbuf.resize(n_bands);
-    for (uint32_t b = 0; b < n_bands; ++b)
-        if (!P->bands[b].out_dev) P->outbuf[b].ensure(size_t(std::max<uint64_t>(P->chunk_cap[b], 1)) * 8, ctx->stream);
     auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
     P->off_jobs = 256;
     P->off_links = align(P->off_jobs + n_bands * sizeof(JobDev));
@@ -1266,34 +1298,17 @@ lpk_plan *plan_create(lpk_ctx *ctx, const lpk_band_chunk *bands, uint32_t n_band
     P->off_segs = align(P->off_flags + std::max<uint64_t>(P->total_links, 1) * sizeof(uint32_t));
     P->off_carry = align(P->off_segs + std::max<size_t>(nw, 1) * sizeof(WalkSegDev));
     P->bytes = align(P->off_carry + std::max<size_t>(nw, 1) * sizeof(CarrySegDev));
-    char *dev = static_cast<char *>(P->tables.ensure(P->bytes, ctx->stream));
-    hip_check(hipStreamSynchronize(ctx->stream), "plan set-up");
-    const uint32_t *live = reinterpret_cast<const uint32_t *>(dev);
     for (int i = 0; i < 2; ++i) {
+        P->tables[i].ensure(P->bytes, ctx->stream);
         if (hipHostMalloc(&P->pin[i], P->bytes, 0) != hipSuccess) throw Fail{LPK_E_NOMEM, "plan staging allocation"};
         std::memset(P->pin[i], 0, P->bytes);
         hip_check(hipEventCreateWithFlags(&P->done[i], hipEventDisableTiming), "event create");
-        // the step's graph: table copy, clock walk, carry, render
-        hip_check(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "graph capture");
-        hipError_t e = hipMemcpyAsync(dev, P->pin[i], P->bytes, hipMemcpyHostToDevice, ctx->stream);
-        if (e == hipSuccess && nw)
-            e = launch_walk_gen(reinterpret_cast<const WalkSegDev *>(dev + P->off_segs), int(nw), P->max_blocks,
-                                live + 1, ctx->stream);
-        if (e == hipSuccess && nw)
-            e = launch_walk_carry(reinterpret_cast<const CarrySegDev *>(dev + P->off_carry), int(nw), ctx->stream);
-        if (e == hipSuccess && P->max_tiles)
-            e = launch_render(form, P->antenna, P->moving, P->tile_shift, P->xcd_map != 0,
-                              reinterpret_cast<const JobDev *>(dev + P->off_jobs), int(n_bands),
-                              reinterpret_cast<const LinkDev *>(dev + P->off_links), ctx->tx_dev, P->max_tiles, live,
-                              nullptr, nullptr, nullptr, nullptr, 0, nullptr, nullptr, ctx->gate != 0, nullptr,
-                              ctx->stream);
-        hipGraph_t g = nullptr;
-        const hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
-        P->graph[i] = g;
-        hip_check(e, "graph capture");
-        hip_check(e2, "graph capture");
-        hip_check(hipGraphInstantiate(&P->exec[i], g, nullptr, nullptr, 0), "graph instantiate");
+        hip_check(hipEventCreateWithFlags(&P->carry_ev[i], hipEventDisableTiming), "event create");
     }
+    hip_check(hipEventCreateWithFlags(&P->ctx_ev, hipEventDisableTiming), "event create");
+    hip_check(hipStreamCreateWithFlags(&P->aux, hipStreamNonBlocking), "plan stream");
+    hip_check(hipStreamSynchronize(ctx->stream), "plan set-up");
+    ctx->plans.push_back(P.get());
     return P.release();
 }
 
@@ -1302,17 +1317,50 @@ void plan_step(lpk_plan *P, uint64_t k) {
     if (k != 0 && k != P->next) throw Fail{LPK_E_STATE, "plan steps run in order; expected chunk " +
                                                             std::to_string(P->next) + " or 0"};
     lpk_ctx *ctx = P->ctx;
-    if (k == 0)
-        for (uint32_t r : P->walk_rx) ctx->rx[r].nb_known = 0;  // a fresh ClockStream per receiver
     const int slot = int(k & 1);
+    hipStream_t s = slot ? P->aux : ctx->stream;
+    if (k == 0) {
+        for (uint32_t r : P->walk_rx) ctx->rx[r].nb_known = 0;  // a fresh ClockStream per receiver
+        // the restart rewrites the walk tables the previous pass may still read
+        for (int i = 0; i < 2; ++i)
+            if (P->used[i]) hip_check(hipStreamWaitEvent(s, P->done[i], 0), "plan restart");
+    }
+    if (slot && ctx->foreign) {
+        // other work on the context's stream since the last step (it may have
+        // extended the walk tables): the odd step follows it
+        hip_check(hipEventRecord(P->ctx_ev, ctx->stream), "event record");
+        hip_check(hipStreamWaitEvent(s, P->ctx_ev, 0), "plan order");
+    }
+    ctx->foreign = false;
     if (P->used[slot]) hip_check(hipEventSynchronize(P->done[slot]), "plan staging reuse");
-    plan_fill(P, k, static_cast<char *>(P->pin[slot]));
-    timing_begin(ctx);
-    hip_check(hipGraphLaunch(P->exec[slot], ctx->stream), "graph launch");
-    timing_end(ctx);
-    hip_check(hipEventRecord(P->done[slot], ctx->stream), "event record");
+    plan_fill(P, k, slot, static_cast<char *>(P->pin[slot]));
+    char *dev = static_cast<char *>(P->tables[slot].p);
+    const uint32_t *live = reinterpret_cast<const uint32_t *>(dev);
+    const int nw = int(P->walk_rx.size());
+    timing_begin(ctx, s);
+    hip_check(hipMemcpyAsync(dev, P->pin[slot], P->bytes, hipMemcpyHostToDevice, s), "plan table copy");
+    if (nw) {
+        hip_check(launch_walk_gen(reinterpret_cast<const WalkSegDev *>(dev + P->off_segs), nw, P->max_blocks, live + 1,
+                                  s),
+                  "clock walk blocks");
+        // the carries extend the shared base tables in step order
+        if (k != 0 && P->used[1 - slot]) hip_check(hipStreamWaitEvent(s, P->carry_ev[1 - slot], 0), "carry order");
+        hip_check(launch_walk_carry(reinterpret_cast<const CarrySegDev *>(dev + P->off_carry), nw, s),
+                  "clock walk carry");
+        hip_check(hipEventRecord(P->carry_ev[slot], s), "event record");
+    }
+    if (P->max_tiles)
+        hip_check(launch_render(P->form, P->antenna, P->moving, P->tile_shift, P->xcd_map != 0,
+                                reinterpret_cast<const JobDev *>(dev + P->off_jobs), int(P->bands.size()),
+                                reinterpret_cast<const LinkDev *>(dev + P->off_links), ctx->tx_dev, P->max_tiles,
+                                live, nullptr, nullptr, nullptr, nullptr, 0, nullptr, nullptr, ctx->gate != 0, nullptr,
+                                s),
+                  "render launch");
+    timing_end(ctx, s);
+    hip_check(hipEventRecord(P->done[slot], s), "event record");
     P->used[slot] = true;
     P->next = k + 1;
+    if (slot) P->aux_pending = true;
 }
 
 }  // namespace
@@ -1605,7 +1653,7 @@ int lpk_plan_create(lpk_ctx *ctx, const lpk_band_chunk *bands, uint32_t n_bands,
 
 int lpk_plan_step(lpk_plan *plan, uint64_t chunk) {
     if (!plan) return LPK_E_INVALID;
-    return guarded(plan->ctx, [&] { plan_step(plan, chunk); });
+    return guarded(plan->ctx, [&] { plan_step(plan, chunk); }, false);
 }
 
 int lpk_plan_info(const lpk_plan *plan, uint64_t *n_chunks, uint32_t *max_tiles) {
@@ -1628,7 +1676,8 @@ int lpk_plan_chunk_range(const lpk_plan *plan, uint32_t band, uint64_t chunk, ui
 
 int lpk_plan_output(const lpk_plan *plan, uint32_t band, void **chunk_dev) {
     if (!plan || !chunk_dev || band >= plan->bands.size()) return LPK_E_INVALID;
-    *chunk_dev = plan->bands[band].out_dev ? plan->bands[band].out_dev : plan->outbuf[band].p;
+    // the buffer of the last step's slot
+    *chunk_dev = plan->bands[band].out_dev ? plan->bands[band].out_dev : plan->outbuf[(plan->next + 1) & 1][band].p;
     return LPK_OK;
 }
 

@@ -684,10 +684,11 @@ class Backend:
 
 
 class Plan:
-    """lpk_plan_*: chunk k of every band per step, replayed as one captured
-    HIP graph (generate_band's chunk loop, src/pipeline.rs:905-990, for all
-    bands at once). `bands` are whole-band descriptors from Backend.job(...,
-    i0=0, clen=n, out_dev=... or None)."""
+    """lpk_plan_*: chunk k of every band per step, the steps alternating
+    between two streams so that each overlaps the previous one's end
+    (generate_band's chunk loop, src/pipeline.rs:905-990, for all bands at
+    once). `bands` are whole-band descriptors from Backend.job(..., i0=0,
+    clen=n, out_dev=... or None)."""
 
     def __init__(self, backend: "Backend", bands: Sequence, chunk_samples: int, form: int):
         self.be = backend

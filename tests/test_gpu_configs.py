@@ -11,7 +11,7 @@ Bars, per window:
   - precision form: every link's solved range within 1 ps of delay and its
     f64 carrier phase (src/channel/mod.rs:256-258, formed from the solved
     range) within 1e-6 rad; IQ within 2e-6 of the band's peak;
-  - batching form (a streaming plan, chunks replayed as HIP graphs): as the
+  - batching form (a streaming plan, steps on two alternating streams): as the
     form it runs, against the oracle;
   - converged and learned forms (configuration 5): against the oracle's
     converged solve (orc_solve_converged), bit-identical and ulp-bounded.
@@ -158,8 +158,8 @@ def test_config4_windows_against_oracle(backend, cfg4):
 def test_config4_plan_against_oracle(backend, cfg4, form):
     """The batching form on configuration 4's units: a streaming plan over
     the first 16 384 samples of each band, steps of 4 096 samples of the
-    fastest band and the same time slice of the others, every step replayed
-    as a captured HIP graph, against the oracle."""
+    fastest band and the same time slice of the others, the steps
+    alternating between the plan's two streams, against the oracle."""
     sc, units = cfg4
     seed = workloads.SEED
     backend.upload(sc, seed)

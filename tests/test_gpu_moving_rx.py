@@ -182,7 +182,7 @@ def test_precision_call_with_moving_and_fixed_receivers(backend):
         d = np.abs(out[r].astype(np.complex128) - ref[r].astype(np.complex128))
         _say(f"mixed call, receiver {r}: n={n[r]} max|d|/peak={d.max() / peak:.3e}")
         assert d.max() <= 2e-6 * peak
-    # a streaming plan of both bands (the moving-receiver kernels in its step graph)
+    # a streaming plan of both bands (the moving-receiver kernels in its steps)
     lib = _lib.load()
     bands, hold = [], []
     for r in range(2):

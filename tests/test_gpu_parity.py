@@ -460,10 +460,10 @@ def _units(sc):
 
 @pytest.mark.parametrize("form", [_lib.FORM_PLAIN, _lib.FORM_PRECISION])
 def test_streaming_plan_replays_match_eager_chunks(backend, form):
-    """Every replay of a plan's captured step graph equals an eager render of
-    the same chunk of every band (clock walk and kTB noise on), with eager
-    renders and synchronisations between the replays; restarting at chunk 0
-    replays the same bytes again. Steps are time slices: the fastest band's
+    """Every step of a plan (alternating between its two streams) equals an
+    eager render of the same chunk of every band (clock walk and kTB noise
+    on), with eager renders and synchronisations between the steps;
+    restarting at chunk 0 renders the same bytes again. Steps are time slices: the fastest band's
     chunks hold chunk_samples samples, the others' the samples of the same
     interval, and together they tile every band exactly once."""
     seed = 21
