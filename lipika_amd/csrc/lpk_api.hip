@@ -187,7 +187,6 @@ struct lpk_ctx {
     // which every other entry point joins (the context's stream waits on the
     // plan's last odd step) before it enqueues work
     std::vector<lpk_plan *> plans;
-    bool foreign = false;   // other work was enqueued since the plans' last step
 };
 
 // A streaming plan (lpk_plan_*): chunk k of every band per step as
@@ -226,6 +225,7 @@ struct lpk_plan {
     hipEvent_t ctx_ev = nullptr;          // the context's stream, when other work came between steps
     hipStream_t aux = nullptr;            // the odd steps' stream
     bool aux_pending = false;             // odd steps not yet joined into the context's stream
+    bool foreign = false;                 // other work was enqueued since this plan's last step
     bool used[2] = {false, false};
 };
 
@@ -250,16 +250,18 @@ int fail(lpk_ctx *ctx, const Fail &f) {
     return f.code;
 }
 
-// Orders the context's stream after every plan step enqueued on a plan's
-// second stream, and notes that other work follows the plans' steps (the
-// next odd step then waits for it).
-void join_plans(lpk_ctx *ctx) {
-    for (lpk_plan *P : ctx->plans)
+// Orders the context's stream after every step enqueued on a plan's second
+// stream (but those of `self`), and notes in those plans that other work
+// follows their steps (their next odd step then waits for it).
+void join_plans(lpk_ctx *ctx, const lpk_plan *self = nullptr) {
+    for (lpk_plan *P : ctx->plans) {
+        if (P == self) continue;
         if (P->aux_pending) {
             hip_check(hipStreamWaitEvent(ctx->stream, P->done[1], 0), "plan join");
             P->aux_pending = false;
         }
-    ctx->foreign = true;
+        P->foreign = true;
+    }
 }
 
 template <class F>
@@ -1325,13 +1327,14 @@ void plan_step(lpk_plan *P, uint64_t k) {
         for (int i = 0; i < 2; ++i)
             if (P->used[i]) hip_check(hipStreamWaitEvent(s, P->done[i], 0), "plan restart");
     }
-    if (slot && ctx->foreign) {
+    join_plans(ctx, P);   // other plans' steps come before this one
+    if (slot && P->foreign) {
         // other work on the context's stream since the last step (it may have
         // extended the walk tables): the odd step follows it
         hip_check(hipEventRecord(P->ctx_ev, ctx->stream), "event record");
         hip_check(hipStreamWaitEvent(s, P->ctx_ev, 0), "plan order");
     }
-    ctx->foreign = false;
+    P->foreign = false;
     if (P->used[slot]) hip_check(hipEventSynchronize(P->done[slot]), "plan staging reuse");
     plan_fill(P, k, slot, static_cast<char *>(P->pin[slot]));
     char *dev = static_cast<char *>(P->tables[slot].p);

@@ -516,6 +516,56 @@ def test_streaming_plan_replays_match_eager_chunks(backend, form):
     _say(f"plan replays: {checked} band-chunks bit-identical to eager renders")
 
 
+def test_two_plans_stepped_in_turn(backend):
+    """Two live plans on one context (the plain form over one band, the
+    precision form over the other, both on the same receiver's clock walk)
+    stepped in turn without synchronising in between: each plan's chunks
+    equal eager renders of the same chunks made afterwards, so the plans'
+    second streams and the shared walk tables stay ordered."""
+    seed = 23
+    sc = prepare(workloads.scene2(duration_s=0.1))
+    backend.upload(sc, seed)
+    lib = _lib.load()
+    units = _units(sc)
+    assert len(units) == 2
+    chunk = 40_000
+    forms = (_lib.FORM_PLAIN, _lib.FORM_PRECISION)
+    plans, outs, keep = [], [], []
+    for w, (r, b) in enumerate(units):
+        n = sc.receivers[r].bands[b].n
+        buf = C.c_void_p()
+        backend.check(lib.lpk_device_alloc(backend.h, n * 8, C.byref(buf)))
+        outs.append((buf, n))
+        j, k = backend.job(r, b, w, seed, 0, n, out_dev=buf.value)
+        keep.append(k)
+        plans.append(backend.plan([j], chunk, forms[w]))
+    try:
+        steps = max(p.n_chunks for p in plans)
+        for rnd in range(2):   # the second pass restarts both plans at chunk 0
+            for kk in range(steps):
+                for p in plans:
+                    if kk < p.n_chunks:
+                        p.step(kk)
+            backend.sync()
+        for w, ((r, b), (buf, n)) in enumerate(zip(units, outs)):
+            got = np.empty(n, dtype=np.complex64)
+            backend.check(lib.lpk_copy_to_host(backend.h, got.ctypes.data, buf.value, n * 8))
+            ranges = [plans[w].chunk_range(0, kk) for kk in range(plans[w].n_chunks)]
+            want = np.empty(n, dtype=np.complex64)
+            for i0, clen in ranges:
+                if clen:
+                    j, k = backend.job(r, b, w, seed, i0, clen, out_host=want[i0:i0 + clen])
+                    backend.render([j], forms[w])
+            backend.sync()
+            assert got.tobytes() == want.tobytes(), w
+        _say(f"two plans in turn: {steps} steps each, both bit-identical to eager renders")
+    finally:
+        for p in plans:
+            p.close()
+        for buf, _ in outs:
+            backend.check(lib.lpk_device_free(backend.h, buf))
+
+
 def test_tuning_and_autotune(backend):
     """The plain form's samples do not depend on the tile size or the XCD
     mapping; the precision form's stay within its bound of the plain form's.
