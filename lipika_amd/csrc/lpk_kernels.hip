@@ -552,7 +552,7 @@ __device__ __forceinline__ double sample_time(double i, double fs, double inv_fs
 // rounds to just below i the reference weighs epsilon(i - 1) by about 1e-16;
 // leaving that out moves delta_t by under 1e-27 s. The terms are summed in
 // the reference's order.
-__device__ __forceinline__ double clock_delta_at_master(const JobDev &J, double t, uint64_t i) {
+__device__ __forceinline__ double clock_at_master_eps(const JobDev &J, double t, double eps) {
     const double elapsed = t - 0.0;
     double flick = 0.0;
     if (J.flicker) {
@@ -562,8 +562,10 @@ __device__ __forceinline__ double clock_delta_at_master(const JobDev &J, double 
         const double fr = xf - mf;
         flick = J.flicker[m0] * (1.0 - fr) + J.flicker[m0 + 1] * fr;
     }
-    const double eps = walk_eps_rel(J, static_cast<int32_t>(static_cast<int64_t>(i) - J.eps_k0));
     return (eps + flick) + (J.c0 + J.y0 * elapsed);
+}
+__device__ __forceinline__ double clock_delta_at_master(const JobDev &J, double t, uint64_t i) {
+    return clock_at_master_eps(J, t, walk_eps_rel(J, static_cast<int32_t>(static_cast<int64_t>(i) - J.eps_k0)));
 }
 
 struct TapOut {
@@ -1201,10 +1203,17 @@ __device__ __forceinline__ void precision_span(const JobDev &J, uint32_t span, c
                 }
             }
             __syncthreads();
+            // at the master rate, the walk values of the next tile's samples are read
+            // during this tile's link loop (the same three terms walk_eps_rel sums)
+            float pf_loc[kSpanSamples] = {0.0f, 0.0f};
+            double pf_toff = 0.0, pf_base = 0.0;
+            bool pf_same = false;   // the lane's two samples share a 32-index sub-block
+            int pf_kk = -1;         // the tile the values are for
             for (int kk = 0; kk < kb; ++kk) {
                 const uint64_t t0 = span0 + uint64_t(k0 + kk) * kTile;
                 const uint64_t first = J.i0 > t0 ? J.i0 : t0, end = jend < t0 + kTile ? jend : t0 + kTile;
                 if (first >= end) continue;   // a tile with no sample of its job (uniform)
+                const bool use_pf = pf_kk == kk && pf_same;
                 // the lane's two samples (src/pipeline.rs:918-926), clamped into the job
                 uint64_t i[kSpanSamples];
                 bool valid[kSpanSamples];
@@ -1216,9 +1225,10 @@ __device__ __forceinline__ void precision_span(const JobDev &J, uint32_t span, c
                     valid[q] = i[q] >= J.i0 && i[q] < jend;
                     const uint64_t ic = valid[q] ? i[q] : first;
                     t_local[q] = sample_time(static_cast<double>(ic), J.fs, J.inv_fs);
-                    dlt[q] = J.delta_t ? J.delta_t[ic - J.i0]
-                                       : (at_master ? clock_delta_at_master(J, t_local[q], ic)
-                                                    : clock_delta(J, t_local[q]));
+                    if (J.delta_t) dlt[q] = J.delta_t[ic - J.i0];
+                    else if (at_master && use_pf)
+                        dlt[q] = clock_at_master_eps(J, t_local[q], (pf_base + pf_toff) + static_cast<double>(pf_loc[q]));
+                    else dlt[q] = at_master ? clock_delta_at_master(J, t_local[q], ic) : clock_delta(J, t_local[q]);
                     tu[q] = t_local[q] - dlt[q];
                     // tu less its value at the middle sample, formed as the difference of
                     // the exact offsets of t_local and delta_t (the phase must not carry the
@@ -1231,6 +1241,26 @@ __device__ __forceinline__ void precision_span(const JobDev &J, uint32_t span, c
                     gq[q] = -J.lo * (t_local[q] - s_at[kk][1][1]);
                     dn[q] = dth[q] = 0.0;
                     if (MOV) rx_rounding(J, tu[q], dn[q], dth[q]);
+                }
+                pf_kk = -1;
+                // (not in the moving-receiver kernels, which are at their register limit)
+                if (!MOV && !J.delta_t && at_master && kk + 1 < kb) {
+                    const uint64_t t1 = t0 + kTile;
+                    const uint64_t first1 = J.i0 > t1 ? J.i0 : t1, end1 = jend < t1 + kTile ? jend : t1 + kTile;
+                    if (first1 < end1) {
+                        int32_t rr[kSpanSamples];
+#pragma unroll
+                        for (int q = 0; q < kSpanSamples; ++q) {
+                            const uint64_t in = t1 + kSpanSamples * threadIdx.x + q;
+                            const uint64_t icn = (in >= J.i0 && in < jend) ? in : first1;
+                            rr[q] = static_cast<int32_t>(static_cast<int64_t>(icn) - J.eps_k0);
+                            pf_loc[q] = J.walk_local[rr[q]];
+                        }
+                        pf_same = (rr[0] >> kWalkSubShift) == (rr[1] >> kWalkSubShift);
+                        pf_base = J.walk_base_w[rr[0] >> kWalkBlockShift];
+                        pf_toff = J.walk_toff[rr[0] >> kWalkSubShift];
+                        pf_kk = kk + 1;
+                    }
                 }
                 const uint64_t smask = uniform_u64(s_mask[kk][0], s_mask[kk][1]);
                 const Anchor *R = s_rec + kk * ng;
