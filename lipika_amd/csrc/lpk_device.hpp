@@ -39,8 +39,8 @@ constexpr int kFlickerCalibM = 256;                     // tau = 2 s (src/clock/
 // Clock walk: blocks of kWalkBlock master-grid increments; each of the
 // kWalkThreads threads of a block owns kWalkPerThread consecutive ones.
 // epsilon(k) = base[k >> 16] (f64) + toff[(k - k0) >> 5] (f64) + local[k - k0] (f32).
-constexpr int kWalkThreads = 1024;
-constexpr int kWalkPerThread = 64;
+constexpr int kWalkThreads = 256;
+constexpr int kWalkPerThread = 256;
 constexpr int kWalkSubShift = 5;
 constexpr int kWalkBlockShift = 16;
 constexpr int64_t kWalkBlock = int64_t(1) << kWalkBlockShift;

@@ -1810,14 +1810,12 @@ __global__ void __launch_bounds__(256) truth_kernel(const TxDev *__restrict__ tx
 // thread totals places each thread in the block, and the block total goes to
 // blocksum.
 constexpr int kWalkSub = 1 << kWalkSubShift;
-// One workgroup per 65 536-index block: each lane walks kWalkPerThread
-// consecutive indices (sub-blocks of 32) and stores each sub-block's
-// within-sub-block prefixes as f32 with 16-byte stores; the lanes' totals are
-// scanned in a fixed tree. Short per-lane chains keep a step's few blocks from
-// being bound by one long dependent sum.
+constexpr int kWalkStageRow = kWalkSub + 1;  // padded: conflict-free rows and columns
+
 __global__ void __launch_bounds__(kWalkThreads) walk_gen_kernel(const WalkSegDev *__restrict__ segs, int n_segs,
                                                                 const uint32_t *n_live) {
     __shared__ double lds[kWalkThreads];
+    __shared__ float stage[kWalkThreads * kWalkStageRow];
     constexpr int kSubsPerThread = kWalkPerThread / kWalkSub;
     if (n_live && blockIdx.x >= *n_live) return;  // streaming plans launch the largest grid
     int lo = 0, hi = n_segs - 1;
@@ -1832,7 +1830,7 @@ __global__ void __launch_bounds__(kWalkThreads) walk_gen_kernel(const WalkSegDev
     const int64_t b = S.first_block + rb;
     const bool win = b >= S.win_block;  // uniform over the workgroup
     const int64_t k0 = (b << kWalkBlockShift) + int64_t(t) * kWalkPerThread;
-    float *out = win ? S.local + ((b - S.win_block) << kWalkBlockShift) + int64_t(t) * kWalkPerThread : nullptr;
+    float *out = win ? S.local + ((b - S.win_block) << kWalkBlockShift) : nullptr;
     const uint64_t seed = S.seed;
     const double sigma = S.sigma;
     double s = 0.0;
@@ -1841,8 +1839,6 @@ __global__ void __launch_bounds__(kWalkThreads) walk_gen_kernel(const WalkSegDev
     for (int m = 0; m < kSubsPerThread; ++m) {
         subs[m] = s;  // thread-local prefix before sub-block m
         const double sub0 = s;
-        float v[kWalkSub];
-#pragma unroll
         for (int q = 0; q < kWalkSub; q += 4) {
             uint64_t w[2];
             philox_words(static_cast<uint64_t>((k0 + m * kWalkSub + q) >> 2), seed, kClockTag, w);
@@ -1851,15 +1847,20 @@ __global__ void __launch_bounds__(kWalkThreads) walk_gen_kernel(const WalkSegDev
                 float zc, zs;
                 box_muller(w[p], zc, zs);
                 s = s + (0.0 + sigma * static_cast<double>(zc));
-                v[q + 2 * p] = static_cast<float>(s - sub0);
+                if (win) stage[t * kWalkStageRow + q + 2 * p] = static_cast<float>(s - sub0);
                 s = s + (0.0 + sigma * static_cast<double>(zs));
-                v[q + 2 * p + 1] = static_cast<float>(s - sub0);
+                if (win) stage[t * kWalkStageRow + q + 2 * p + 1] = static_cast<float>(s - sub0);
             }
         }
         if (win) {
-            float4 *o4 = reinterpret_cast<float4 *>(out + m * kWalkSub);
-#pragma unroll
-            for (int c = 0; c < kWalkSub / 4; ++c) o4[c] = make_float4(v[4 * c], v[4 * c + 1], v[4 * c + 2], v[4 * c + 3]);
+            // thread r's 32 values of sub-block m go to out[r * 256 + m * 32 + c]
+            __syncthreads();
+#pragma unroll 4
+            for (int e = t; e < kWalkThreads * kWalkSub; e += kWalkThreads) {
+                const int row = e >> kWalkSubShift, col = e & (kWalkSub - 1);
+                out[row * kWalkPerThread + m * kWalkSub + col] = stage[row * kWalkStageRow + col];
+            }
+            __syncthreads();
         }
     }
     // fixed-tree (Hillis-Steele) inclusive scan of the thread totals
