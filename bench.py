@@ -43,7 +43,7 @@ PEAK_FP64_TFLOPS = 78.6            # MI355X published FP64 vector peak
 C_LIGHT = 299792458.0
 # HBM bytes per render dispatch (reads + writes) of the precision form, from the rocprofv3
 # FETCH_SIZE and WRITE_SIZE passes scaled by a copy kernel of known size.
-TRAFFIC = {4: (166.93e9, "profiles/render_config4_hbm_traffic.md"),
+TRAFFIC = {4: (159.94e9, "profiles/render_config4_hbm_traffic.md"),
            2: (2.073e9, "profiles/render_config2_hbm_traffic.md")}
 WORKLOADS = {
     2: "BPSK LEO 437 MHz + 6 C/A MEO 1575.42 MHz; bands 48 kHz and 2.048 MHz; clock walk and kTB noise on; 60 s",
