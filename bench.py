@@ -471,6 +471,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines and the oracle error checks")
     ap.add_argument("--headline-only", action="store_true",
                     help="configuration 4's precision form only (for profiler runs)")
+    ap.add_argument("--plan-only", action="store_true",
+                    help="configuration 4's precision form tuned, then --steps plan passes in 1 Mi-sample "
+                         "chunks (for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -482,6 +485,11 @@ def main():
     S4 = Scene(4)
     log(f"[bench] config 4: {len(S4.jobs)} chunk jobs, {S4.out_samples} output samples, "
         f"{S4.contributions} contributions per step")
+    if args.plan_only:
+        time_form(S4, "precision", 1, 0)
+        print(json.dumps(plan_leg(S4, "precision", K, W, 1 << 20)), flush=True)
+        S4.close()
+        return
     prec = time_form(S4, "precision", K, W)
     forms4 = {"precision": prec}
     res = {"metric": METRIC, "value": prec["value"], "unit": "Msamples/s", "n_gpus": 1, "steps": K, "warmup": W,
