@@ -173,7 +173,7 @@ struct lpk_ctx {
     int tile_shift = kDefaultTileShift;
     int xcd_map = 1;
     // per-call buffers
-    DevBuf jobs, links, dt_stage, out_stage, codes_stage, blocksum, walk_tables;
+    DevBuf jobs, links, dt_stage, out_stage, codes_stage, blocksum, walk_tables, part_stage;
     // converged and learned forms: the network the learned form predicts
     // with, its prediction table and the solve counters
     lpk_lws *lws = nullptr;
@@ -711,14 +711,15 @@ void validate_job(const lpk_ctx *ctx, const lpk_band_chunk &B, uint32_t q) {
         throw bad(LPK_E_NONFINITE, "antenna parameters");
 }
 
-// The precision form takes at most kPrecisionMaxLinks transmitters per job.
-// The learned form's features assume a receiver fixed on the Earth; a moving
-// receiver renders in the plain, converged or precision form.
-void check_form_links(const lpk_ctx *ctx, int form, const lpk_band_chunk &B, uint32_t q) {
-    if (form == LPK_FORM_PRECISION && B.n_active > kPrecisionMaxLinks)
-        throw Fail{LPK_E_INVALID, "job " + std::to_string(q) + ": the precision form takes at most " +
-                                      std::to_string(kPrecisionMaxLinks) + " transmitters per band-chunk (" +
-                                      std::to_string(B.n_active) + " given); use the plain form"};
+// A streaming plan's precision form takes at most kPrecisionMaxLinks
+// transmitters per band (a render call splits larger band-chunks into
+// passes). The learned form's features assume a receiver fixed on the Earth;
+// a moving receiver renders in the plain, converged or precision form.
+void check_form_links(const lpk_ctx *ctx, int form, const lpk_band_chunk &B, uint32_t q, bool plan = false) {
+    if (plan && form == LPK_FORM_PRECISION && B.n_active > kPrecisionMaxLinks)
+        throw Fail{LPK_E_INVALID, "band " + std::to_string(q) + ": a streaming plan's precision form takes at most " +
+                                      std::to_string(kPrecisionMaxLinks) + " transmitters per band (" +
+                                      std::to_string(B.n_active) + " given); use lpk_render_chunk or the plain form"};
     if (form == LPK_FORM_LEARNED && B.rx < ctx->rx.size() && ctx->rx[B.rx].d.traj_kind != LPK_RX_STATIONARY)
         throw Fail{LPK_E_INVALID, "job " + std::to_string(q) +
                                       ": the learned form takes stationary receivers; use the plain, converged "
@@ -930,8 +931,25 @@ void render(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form,
     float *stage_dev = stage_total ? static_cast<float *>(ctx->out_stage.ensure(stage_total * 8, ctx->stream)) : nullptr;
     int16_t *codes_dev = code_total ? static_cast<int16_t *>(ctx->codes_stage.ensure(code_total * 4, ctx->stream)) : nullptr;
 
+    // ---- passes: the precision form renders a band-chunk of more than
+    // kPrecisionMaxLinks transmitters in passes of at most that many, the
+    // passes adding their f64 sums through HBM (JobDev::part)
+    std::vector<uint32_t> n_pass(n_jobs, 1);
+    uint32_t max_pass = 1;
+    uint64_t n_entries = 0, part_total = 0;
+    for (uint32_t q = 0; q < n_jobs; ++q) {
+        if (form == LPK_FORM_PRECISION && jobs[q].n_active > kPrecisionMaxLinks) {
+            n_pass[q] = (jobs[q].n_active + kPrecisionMaxLinks - 1) / kPrecisionMaxLinks;
+            part_total += jobs[q].clen;
+        }
+        max_pass = std::max(max_pass, n_pass[q]);
+        n_entries += n_pass[q];
+    }
+    double2 *part_dev =
+        part_total ? static_cast<double2 *>(ctx->part_stage.ensure(part_total * sizeof(double2), ctx->stream)) : nullptr;
+
     // ---- job and link tables
-    const size_t job_bytes = size_t(std::max<uint32_t>(n_jobs, 1)) * sizeof(JobDev);
+    const size_t job_bytes = size_t(std::max<uint64_t>(n_entries, 1)) * sizeof(JobDev);
     const size_t link_bytes = size_t(std::max<uint64_t>(total_links, 1)) * sizeof(LinkDev);
     const size_t flag_bytes = size_t(std::max<uint64_t>(total_links, 1)) * sizeof(uint32_t);
     const int slot = ctx->pin_next;
@@ -948,9 +966,10 @@ void render(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form,
     };
     std::vector<FlagJob> flag_jobs;
 
-    uint64_t tiles = 0, link_off = 0, dt_off = 0, st_off = 0, cd_off = 0, n_pred = 0;
+    uint64_t tiles = 0, link_off = 0, dt_off = 0, st_off = 0, cd_off = 0, n_pred = 0, part_off = 0;
     uint32_t tap_slot = tap ? tap->slot : 0;
     std::vector<uint32_t> perm;
+    std::vector<JobDev> whole(n_jobs);   // in `order`
     std::vector<float *> out_ptr(n_jobs, nullptr);
     std::vector<int16_t *> code_ptr(n_jobs, nullptr);
     for (uint32_t o = 0; o < n_jobs; ++o) {
@@ -1004,9 +1023,42 @@ void render(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form,
             cd_off += B.clen;
             code_ptr[q] = J.codes;
         }
-        jh[o] = J;
+        if (n_pass[q] > 1) {
+            J.part = part_dev + part_off;
+            part_off += B.clen;
+        }
+        whole[o] = J;
+    }
+    // pass p: the p-th slice of links of every job with more than p passes
+    // (every job in pass 0), each pass's tiles numbered from 0
+    std::vector<uint32_t> pass_jobs(max_pass, 0);
+    std::vector<uint64_t> pass_tiles(max_pass, 0);
+    {
+        uint64_t e = 0;
+        for (uint32_t pp = 0; pp < max_pass; ++pp) {
+            for (uint32_t o = 0; o < n_jobs; ++o) {
+                const uint32_t np = n_pass[order[o]];
+                if (pp >= np) continue;
+                JobDev J = whole[o];
+                if (np > 1) {
+                    const uint32_t lo = pp * kPrecisionMaxLinks;
+                    const uint32_t cnt = std::min(kPrecisionMaxLinks, J.n_active - lo);
+                    for (int g = 0; g < 3; ++g)
+                        J.group_end[g] = J.group_end[g] <= lo ? 0u : std::min(J.group_end[g] - lo, cnt);
+                    J.link0 += lo;
+                    J.n_active = cnt;
+                    J.pass = pp == 0 ? 1 : (pp + 1 == np ? 3 : 2);
+                }
+                J.tile0 = uint32_t(pass_tiles[pp]);
+                pass_tiles[pp] += J.n_tiles;
+                if (pass_tiles[pp] > 0xFFFFFFFFull) throw Fail{LPK_E_INVALID, "render call holds more than 2^32 tiles"};
+                jh[e++] = J;
+                ++pass_jobs[pp];
+            }
+        }
     }
     hip_check(hipMemcpyAsync(jd, jh, job_bytes, hipMemcpyHostToDevice, ctx->stream), "job table upload");
+    (void)tiles;
     hip_check(hipMemcpyAsync(ld, lh, link_bytes + flag_bytes, hipMemcpyHostToDevice, ctx->stream),
               "link table upload");
     pinned_mark(ctx, slot);
@@ -1020,12 +1072,22 @@ void render(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form,
             hip_check(launch_lws_predict(jd, int(n_jobs), ld, ctx->tx_dev, lws_net(ctx->lws), ts, n_pred, pred,
                                          ctx->stats, ctx->stream),
                       "warm-start prediction launch");
-        hip_check(launch_render(converged ? 2 : form, any_antenna, any_moving, ts, ctx->xcd_map != 0, jd, int(n_jobs), ld,
-                                ctx->tx_dev, uint32_t(tiles), nullptr, tap ? tap->t_emit : nullptr,
-                                tap ? tap->range : nullptr, tap ? tap->phase : nullptr, tap ? tap->delta_t : nullptr,
-                                tap_slot, pred, converged ? ctx->stats : nullptr, ctx->gate != 0,
-                                any_antenna ? ctx->stats + 4 : nullptr, ctx->stream),
-                  "render launch");
+        uint64_t e0 = 0;
+        for (uint32_t pp = 0; pp < max_pass; ++pp) {
+            // the tapped link is in pass tap_slot / kPrecisionMaxLinks of a job in passes
+            uint32_t slot_p = tap_slot;
+            if (max_pass > 1)
+                slot_p = tap_slot / kPrecisionMaxLinks == pp ? tap_slot % kPrecisionMaxLinks : 0xFFFFFFFFu;
+            if (pass_tiles[pp])
+                hip_check(launch_render(converged ? 2 : form, any_antenna, any_moving, ts, ctx->xcd_map != 0, jd + e0,
+                                        int(pass_jobs[pp]), ld, ctx->tx_dev, uint32_t(pass_tiles[pp]), nullptr,
+                                        tap ? tap->t_emit : nullptr, tap ? tap->range : nullptr,
+                                        tap ? tap->phase : nullptr, tap ? tap->delta_t : nullptr, slot_p, pred,
+                                        converged ? ctx->stats : nullptr, ctx->gate != 0,
+                                        any_antenna ? ctx->stats + 4 : nullptr, ctx->stream),
+                          "render launch");
+            e0 += pass_jobs[pp];
+        }
         timing_end(ctx);
     }
     if (want_flags) {
@@ -1211,7 +1273,7 @@ lpk_plan *plan_create(lpk_ctx *ctx, const lpk_band_chunk *bands, uint32_t n_band
     for (uint32_t b = 0; b < n_bands; ++b) {
         lpk_band_chunk &B = P->bands[b];
         validate_job(ctx, B, b);
-        check_form_links(ctx, form, B, b);
+        check_form_links(ctx, form, B, b, true);
         if (B.i0 != 0 || B.delta_t || B.out_host || B.adc_codes_host || B.unoccluded)
             throw Fail{LPK_E_INVALID, "band " + std::to_string(b) +
                                           ": a plan band starts at sample 0, uses the device clock and has "
@@ -1417,6 +1479,8 @@ void lpk_ctx_destroy(lpk_ctx *ctx) {
     ctx->out_stage.release();
     ctx->codes_stage.release();
     ctx->blocksum.release();
+    ctx->walk_tables.release();
+    ctx->part_stage.release();
     for (auto &s : ctx->pin) {
         if (s.p) (void)hipHostFree(s.p);
         if (s.done) (void)hipEventDestroy(s.done);

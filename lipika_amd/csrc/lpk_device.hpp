@@ -58,11 +58,13 @@ constexpr int kSuperShift = 3;
 __host__ __device__ constexpr int span_shift(int form, bool antenna, int tile_shift) {
     return tile_shift + (form == 1 && !antenna ? kSuperShift : 0);
 }
-// Transmitters per band-chunk the precision form takes. A 72-link band made
-// a device memory fault that no host-side check of the kernels' indexing has
-// explained, so the form is held to the largest count it has rendered
-// cleanly on the device (four tiles per anchor batch); the plain form has no
-// such limit.
+// Transmitters per pass of the precision form. A 72-link band rendered in one
+// pass (one tile per anchor batch, two record groups) made a device memory
+// fault that no host-side check of the kernels' indexing has explained, so a
+// pass holds at most the count rendered cleanly on the device (four tiles per
+// anchor batch, one record group): a render call splits a larger band-chunk
+// into passes of this many links, which add their sums in f64 through HBM
+// (JobDev::part). Streaming plans take at most this many per band.
 constexpr uint32_t kPrecisionMaxLinks = 32;
 // A dispatch packet counts its grid in work-items with 32 bits, so one
 // launch covers at most 2^32 / 512 - 1 tiles of the largest size; a render
@@ -241,7 +243,13 @@ struct JobDev {
     // with walk_base_w = walk_base + eps_k0 / 65 536 (the window's first block)
     const double *walk_base_w;
     double eps_k0d;
-    int32_t walk_rel, _wpad;
+    int32_t walk_rel;
+    // precision form in several passes over the job's links (kPrecisionMaxLinks
+    // each): 0 one pass; 1 the first (writes its f64 sums to part), 2 a middle
+    // one (adds part to its sums and writes them back), 3 the last (adds part,
+    // then noise, post-sum ops and the stores)
+    int32_t pass;
+    double2 *part;           // per sample of the chunk (re, im), for passes 1-3
 };
 
 // High 64 bits of a 64 x 64-bit product, and count of leading zeros, on the

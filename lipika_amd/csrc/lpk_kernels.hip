@@ -1288,7 +1288,34 @@ __device__ __forceinline__ void precision_span(const JobDev &J, uint32_t span, c
                         }
                     }
                 }
-                if (last_group) {
+                if (last_group && (J.pass == 1 || J.pass == 2)) {
+                    // a pass before the last: the f64 sums so far go to HBM
+#pragma unroll
+                    for (int q = 0; q < kSpanSamples; ++q) {
+                        if (valid[q]) {
+                            double2 *pp = J.part + (i[q] - J.i0);
+                            if (J.pass == 2) {
+                                const double2 prev = *pp;
+                                sum_re[q] += prev.x;
+                                sum_im[q] += prev.y;
+                            }
+                            *pp = make_double2(sum_re[q], sum_im[q]);
+                        }
+                        sum_re[q] = 0.0;
+                        sum_im[q] = 0.0;
+                    }
+                } else if (last_group) {
+                    if (J.pass == 3) {
+                        // the earlier passes' sums
+#pragma unroll
+                        for (int q = 0; q < kSpanSamples; ++q) {
+                            if (valid[q]) {
+                                const double2 prev = J.part[i[q] - J.i0];
+                                sum_re[q] += prev.x;
+                                sum_im[q] += prev.y;
+                            }
+                        }
+                    }
                     // AWGN, src/channel/mod.rs:328-333 (sigma = sqrt(P/2) as f32): the
                     // lane's samples are words 0 and 1 of Philox block i >> 1
                     uint64_t w[2] = {0, 0};
@@ -1658,6 +1685,25 @@ __global__ void __launch_bounds__(1 << TS, ANT ? 4 : (FORM == 1 ? (MOV ? 3 : 4) 
                 tap.phase[j] = ph;
             }
             __syncthreads();
+        }
+        if (J.pass == 1 || J.pass == 2) {
+            // a pass before the last: the f64 sums so far go to HBM (uniform return)
+            if (tap.on && valid) tap.delta_t[j] = dlt;
+            if (valid) {
+                double2 *pp = J.part + j;
+                if (J.pass == 2) {
+                    const double2 prev = *pp;
+                    sum_re += prev.x;
+                    sum_im += prev.y;
+                }
+                *pp = make_double2(sum_re, sum_im);
+            }
+            return;
+        }
+        if (J.pass == 3 && valid) {
+            const double2 prev = J.part[j];
+            sum_re += prev.x;
+            sum_im += prev.y;
         }
         acc_re = static_cast<float>(sum_re);
         acc_im = static_cast<float>(sum_im);
