@@ -69,8 +69,9 @@ typedef enum { LPK_POST_GAIN = 0, LPK_POST_NONLINEARITY = 1, LPK_POST_ADC = 2 } 
 /* Arithmetic form of a render. */
 typedef enum {
     LPK_FORM_PLAIN = 0,      /* the reference's f64 path, operation for operation */
-    LPK_FORM_PRECISION = 1,  /* f64 geometry and phase, f32 signal path, compensated sum; at most
-                              * 32 transmitters per band-chunk (LPK_E_INVALID beyond) */
+    LPK_FORM_PRECISION = 1,  /* f64 geometry and phase, f32 signal path, f64 transmitter sum;
+                              * render calls take band-chunks of more than 32 transmitters in
+                              * passes of 32, plans refuse them (LPK_E_INVALID) */
     LPK_FORM_CONVERGED = 2,  /* the plain form with the emission-time solve (src/channel/mod.rs:70-99)
                               * iterated on the unrounded time axis until an update is below
                               * 1e-13 s, from the reference's start t - 0.07 s */

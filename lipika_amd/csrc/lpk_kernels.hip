@@ -20,7 +20,7 @@
 // trajectory grid node takes the exact per-sample solve instead. The phase
 // is formed in f64 in the reference's operation order and reduced exactly;
 // sine and cosine (hardware, in turns) and the mix run in f32, and the f32
-// contributions are summed exactly in f64 (the compensated sum).
+// contributions are summed in f64 (stronger than a compensated f32 sum).
 //
 // FORM 2 (converged) is the plain form with the emission-time solve iterated
 // to a residual tolerance (kSolveTol) on the unrounded time axis instead of
