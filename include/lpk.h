@@ -70,8 +70,8 @@ typedef enum { LPK_POST_GAIN = 0, LPK_POST_NONLINEARITY = 1, LPK_POST_ADC = 2 } 
 typedef enum {
     LPK_FORM_PLAIN = 0,      /* the reference's f64 path, operation for operation */
     LPK_FORM_PRECISION = 1,  /* f64 geometry and phase, f32 signal path, f64 transmitter sum;
-                              * render calls take band-chunks of more than 32 transmitters in
-                              * passes of 32, plans refuse them (LPK_E_INVALID) */
+                              * a band-chunk of more than 32 transmitters renders in passes of
+                              * 32 that add their sums through HBM (render calls and plans) */
     LPK_FORM_CONVERGED = 2,  /* the plain form with the emission-time solve (src/channel/mod.rs:70-99)
                               * iterated on the unrounded time axis until an update is below
                               * 1e-13 s, from the reference's start t - 0.07 s */
@@ -259,11 +259,16 @@ int lpk_autotune(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int 
  *      band, one per stream, and lpk_plan_output gives the one the last
  *      step wrote. Steps run in order from 0; step 0 restarts
  *      the plan's receivers' clock walks. The tuning is the context's at
- *      creation. ---- */
+ *      creation. In the precision form a band of more than 32 transmitters
+ *      renders in passes (one launch each, in order on the step's stream)
+ *      through a staging buffer per stream of 16 B per sample of its chunk;
+ *      a band may hold up to 48 x 32 transmitters. ---- */
 typedef struct lpk_plan lpk_plan;
 int lpk_plan_create(lpk_ctx *ctx, const lpk_band_chunk *bands, uint32_t n_bands, uint64_t chunk_samples,
                     int form, lpk_plan **out);
 int lpk_plan_step(lpk_plan *plan, uint64_t chunk);     /* asynchronous, like lpk_render_chunk */
+/* n_chunks: the plan's steps; max_tiles: the workgroups its largest step
+ * launches, over all precision passes */
 int lpk_plan_info(const lpk_plan *plan, uint64_t *n_chunks, uint32_t *max_tiles);
 int lpk_plan_chunk_range(const lpk_plan *plan, uint32_t band, uint64_t chunk, uint64_t *i0, uint64_t *clen);
 int lpk_plan_output(const lpk_plan *plan, uint32_t band, void **chunk_dev);

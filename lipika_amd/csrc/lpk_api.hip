@@ -60,6 +60,10 @@ static __device__ __align__(256) float scratch[kNoAllocFloats];
 namespace {
 
 constexpr int kMaxDevices = 64;
+// A plan step's table header: word 1 the live walk blocks, word
+// kPlanPassHdr + p the live tiles of pass p (of at most kPlanMaxPasses).
+constexpr uint32_t kPlanPassHdr = 16;
+constexpr uint32_t kPlanMaxPasses = 256 / 4 - kPlanPassHdr;
 std::mutex g_scratch_mu;
 bool g_scratch_done[kMaxDevices] = {};
 hipError_t g_scratch_rc[kMaxDevices] = {};
@@ -210,12 +214,18 @@ struct lpk_plan {
     std::vector<uint32_t> order;          // bands in job-table order (heaviest first)
     std::vector<uint64_t> link0;          // per band: its first link
     uint64_t total_links = 0;
+    // precision bands of more than kPrecisionMaxLinks links render in passes
+    std::vector<uint32_t> n_pass;         // per band: its passes (1 for most)
+    std::vector<uint64_t> part_off;       // per band in passes: its samples' offset in part
+    std::vector<uint32_t> pass_jobs;      // per pass: the bands it renders (in `order`)
+    std::vector<uint32_t> pass_tiles;     // per pass: its largest tile count over the steps
+    DevBuf part[2];                       // per slot: the passes' f64 sums
     std::vector<uint32_t> walk_rx;        // receivers whose device clock the plan walks
     std::vector<int> walk_slot;           // per receiver: its index in walk_rx, or -1
     std::vector<DevBuf> local[2], toff[2];  // per slot and walked receiver: the step's window
     std::vector<int64_t> win_cap;         // per walked receiver: window capacity in blocks
     DevBuf blocksum[2];
-    uint32_t max_blocks = 0, max_tiles = 0;
+    uint32_t max_blocks = 0;
     std::vector<DevBuf> outbuf[2];        // per slot and band: the chunk buffer when out_dev is NULL
     size_t off_jobs = 0, off_links = 0, off_flags = 0, off_segs = 0, off_carry = 0, bytes = 0;
     DevBuf tables[2];
@@ -715,15 +725,31 @@ void validate_job(const lpk_ctx *ctx, const lpk_band_chunk &B, uint32_t q) {
 // transmitters per band (a render call splits larger band-chunks into
 // passes). The learned form's features assume a receiver fixed on the Earth;
 // a moving receiver renders in the plain, converged or precision form.
-void check_form_links(const lpk_ctx *ctx, int form, const lpk_band_chunk &B, uint32_t q, bool plan = false) {
-    if (plan && form == LPK_FORM_PRECISION && B.n_active > kPrecisionMaxLinks)
-        throw Fail{LPK_E_INVALID, "band " + std::to_string(q) + ": a streaming plan's precision form takes at most " +
-                                      std::to_string(kPrecisionMaxLinks) + " transmitters per band (" +
-                                      std::to_string(B.n_active) + " given); use lpk_render_chunk or the plain form"};
+void check_form_links(const lpk_ctx *ctx, int form, const lpk_band_chunk &B, uint32_t q) {
     if (form == LPK_FORM_LEARNED && B.rx < ctx->rx.size() && ctx->rx[B.rx].d.traj_kind != LPK_RX_STATIONARY)
         throw Fail{LPK_E_INVALID, "job " + std::to_string(q) +
                                       ": the learned form takes stationary receivers; use the plain, converged "
                                       "or precision form"};
+}
+
+// The precision form renders a band-chunk of more than kPrecisionMaxLinks
+// transmitters in passes of at most that many, adding their f64 sums through
+// JobDev::part: the number of passes of a job, and pass pp's job, which is
+// the job's pp-th slice of links (group ends clipped to it).
+uint32_t precision_passes(int form, uint32_t n_active) {
+    return form == LPK_FORM_PRECISION && n_active > kPrecisionMaxLinks
+               ? (n_active + kPrecisionMaxLinks - 1) / kPrecisionMaxLinks
+               : 1u;
+}
+JobDev pass_slice(JobDev J, uint32_t pp, uint32_t np) {
+    if (np <= 1) return J;
+    const uint32_t lo = pp * kPrecisionMaxLinks;
+    const uint32_t cnt = std::min(kPrecisionMaxLinks, J.n_active - lo);
+    for (int g = 0; g < 3; ++g) J.group_end[g] = J.group_end[g] <= lo ? 0u : std::min(J.group_end[g] - lo, cnt);
+    J.link0 += lo;
+    J.n_active = cnt;
+    J.pass = pp == 0 ? 1 : (pp + 1 == np ? 3 : 2);
+    return J;
 }
 
 // Whether the receiver of a (validated) job moves: its render takes the
@@ -938,10 +964,8 @@ void render(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form,
     uint32_t max_pass = 1;
     uint64_t n_entries = 0, part_total = 0;
     for (uint32_t q = 0; q < n_jobs; ++q) {
-        if (form == LPK_FORM_PRECISION && jobs[q].n_active > kPrecisionMaxLinks) {
-            n_pass[q] = (jobs[q].n_active + kPrecisionMaxLinks - 1) / kPrecisionMaxLinks;
-            part_total += jobs[q].clen;
-        }
+        n_pass[q] = precision_passes(form, jobs[q].n_active);
+        if (n_pass[q] > 1) part_total += jobs[q].clen;
         max_pass = std::max(max_pass, n_pass[q]);
         n_entries += n_pass[q];
     }
@@ -1039,16 +1063,7 @@ void render(lpk_ctx *ctx, const lpk_band_chunk *jobs, uint32_t n_jobs, int form,
             for (uint32_t o = 0; o < n_jobs; ++o) {
                 const uint32_t np = n_pass[order[o]];
                 if (pp >= np) continue;
-                JobDev J = whole[o];
-                if (np > 1) {
-                    const uint32_t lo = pp * kPrecisionMaxLinks;
-                    const uint32_t cnt = std::min(kPrecisionMaxLinks, J.n_active - lo);
-                    for (int g = 0; g < 3; ++g)
-                        J.group_end[g] = J.group_end[g] <= lo ? 0u : std::min(J.group_end[g] - lo, cnt);
-                    J.link0 += lo;
-                    J.n_active = cnt;
-                    J.pass = pp == 0 ? 1 : (pp + 1 == np ? 3 : 2);
-                }
+                JobDev J = pass_slice(whole[o], pp, np);
                 J.tile0 = uint32_t(pass_tiles[pp]);
                 pass_tiles[pp] += J.n_tiles;
                 if (pass_tiles[pp] > 0xFFFFFFFFull) throw Fail{LPK_E_INVALID, "render call holds more than 2^32 tiles"};
@@ -1151,8 +1166,9 @@ bool plan_window(const lpk_plan *P, uint32_t r, uint64_t k, int64_t &lo, int64_t
     return any;
 }
 
-// Fills the pinned table of step k: header (live tiles, live walk blocks),
-// jobs, links, zeroed flags, walk segments and carries.
+// Fills the pinned table of step k: header (live walk blocks, each pass's
+// live tiles), jobs (pass by pass), links, zeroed flags, walk segments and
+// carries.
 void plan_fill(lpk_plan *P, uint64_t k, int slot, char *dst) {
     lpk_ctx *ctx = P->ctx;
     char *dev = static_cast<char *>(P->tables[slot].p);
@@ -1203,18 +1219,16 @@ void plan_fill(lpk_plan *P, uint64_t k, int slot, char *dst) {
     }
     if (blocks > P->max_blocks)
         throw Fail{LPK_E_STATE, "clock walk of this step beyond the plan's capacity (clock rewound during the plan?)"};
-    // jobs and links
-    uint64_t tiles = 0;
+    // jobs and links: every band's job, then the passes' slices of them
     std::vector<uint32_t> perm;
+    std::vector<JobDev> whole(P->bands.size());
     for (uint32_t b : P->order) {
         const lpk_band_chunk &B = P->bands[b];
         uint64_t i0, clen;
         plan_chunk(P, b, k, i0, clen);
         JobDev J = fill_job(ctx, B, i0, clen, P->form, lh + P->link0[b], perm);
         J.flicker = B.delta_t ? nullptr : static_cast<const double *>(ctx->rx[B.rx].flick.dev.p);
-        J.tile0 = uint32_t(tiles);
         J.n_tiles = tiles_of(i0, clen, span_shift(P->form, P->antenna, P->tile_shift));
-        tiles += J.n_tiles;
         J.link0 = uint32_t(P->link0[b]);
         if (B.has_antenna) J.unocc = flags_dev;
         if (J.walk) {
@@ -1226,10 +1240,21 @@ void plan_fill(lpk_plan *P, uint64_t k, int slot, char *dst) {
             set_walk_rel(J, ctx->rx[B.rx].base, k1[w]);
         }
         J.out = B.out_dev ? static_cast<float *>(B.out_dev) + 2 * i0 : static_cast<float *>(P->outbuf[slot][b].p);
-        *jh++ = J;
+        if (P->n_pass[b] > 1) J.part = static_cast<double2 *>(P->part[slot].p) + P->part_off[b];
+        whole[b] = J;
     }
-    if (tiles > P->max_tiles) throw Fail{LPK_E_STATE, "step beyond the plan's tile capacity"};
-    hdr[0] = uint32_t(tiles);
+    for (uint32_t pp = 0; pp < P->pass_jobs.size(); ++pp) {
+        uint64_t tiles = 0;
+        for (uint32_t b : P->order) {
+            if (pp >= P->n_pass[b]) continue;
+            JobDev J = pass_slice(whole[b], pp, P->n_pass[b]);
+            J.tile0 = uint32_t(tiles);
+            tiles += J.n_tiles;
+            *jh++ = J;
+        }
+        if (tiles > P->pass_tiles[pp]) throw Fail{LPK_E_STATE, "step beyond the plan's tile capacity"};
+        hdr[kPlanPassHdr + pp] = uint32_t(tiles);
+    }
     hdr[1] = uint32_t(blocks);
 }
 
@@ -1248,6 +1273,7 @@ void plan_free(lpk_plan *P) {
         for (auto &d : P->local[i]) d.release();
         for (auto &d : P->toff[i]) d.release();
         for (auto &d : P->outbuf[i]) d.release();
+        P->part[i].release();
         P->blocksum[i].release();
         P->tables[i].release();
     }
@@ -1273,7 +1299,7 @@ lpk_plan *plan_create(lpk_ctx *ctx, const lpk_band_chunk *bands, uint32_t n_band
     for (uint32_t b = 0; b < n_bands; ++b) {
         lpk_band_chunk &B = P->bands[b];
         validate_job(ctx, B, b);
-        check_form_links(ctx, form, B, b, true);
+        check_form_links(ctx, form, B, b);
         if (B.i0 != 0 || B.delta_t || B.out_host || B.adc_codes_host || B.unoccluded)
             throw Fail{LPK_E_INVALID, "band " + std::to_string(b) +
                                           ": a plan band starts at sample 0, uses the device clock and has "
@@ -1305,6 +1331,17 @@ lpk_plan *plan_create(lpk_ctx *ctx, const lpk_band_chunk *bands, uint32_t n_band
         P->link0[b] = P->total_links;
         P->total_links += P->bands[b].n_active;
     }
+    P->n_pass.assign(n_bands, 1);
+    for (uint32_t b = 0; b < n_bands; ++b) {
+        P->n_pass[b] = precision_passes(form, P->bands[b].n_active);
+        if (P->n_pass[b] > P->pass_jobs.size()) P->pass_jobs.resize(P->n_pass[b], 0);
+    }
+    if (P->pass_jobs.size() > kPlanMaxPasses)
+        throw Fail{LPK_E_INVALID, "a plan band holds more than " + std::to_string(kPlanMaxPasses * kPrecisionMaxLinks) +
+                                      " transmitters in the precision form"};
+    for (uint32_t b = 0; b < n_bands; ++b)
+        for (uint32_t pp = 0; pp < P->n_pass[b]; ++pp) ++P->pass_jobs[pp];
+    P->pass_tiles.assign(P->pass_jobs.size(), 0);
     // flicker tables over every band's whole extent (steps only read them)
     for (const lpk_band_chunk &B : P->bands)
         if (!B.delta_t) ensure_flicker(ctx, ctx->rx[B.rx], B.i0 + B.clen, B.fs_hz);
@@ -1319,7 +1356,8 @@ lpk_plan *plan_create(lpk_ctx *ctx, const lpk_band_chunk *bands, uint32_t n_band
     std::vector<int64_t> nb(nw, 0), max_bhi(nw, -1);
     P->win_cap.assign(nw, 0);
     for (uint64_t k = 0; k < P->n_chunks; ++k) {
-        uint64_t blocks = 0, tiles = 0;
+        uint64_t blocks = 0;
+        std::vector<uint64_t> tiles(P->pass_jobs.size(), 0);
         for (size_t w = 0; w < nw; ++w) {
             int64_t lo, hi;
             if (!plan_window(P.get(), P->walk_rx[w], k, lo, hi)) continue;
@@ -1332,13 +1370,23 @@ lpk_plan *plan_create(lpk_ctx *ctx, const lpk_band_chunk *bands, uint32_t n_band
         for (uint32_t b = 0; b < n_bands; ++b) {
             uint64_t i0, clen;
             plan_chunk(P.get(), b, k, i0, clen);
-            tiles += tiles_of(i0, clen, span_shift(P->form, P->antenna, P->tile_shift));
+            const uint64_t nt = tiles_of(i0, clen, span_shift(P->form, P->antenna, P->tile_shift));
+            for (uint32_t pp = 0; pp < P->n_pass[b]; ++pp) tiles[pp] += nt;
             P->chunk_cap[b] = std::max(P->chunk_cap[b], clen);
         }
-        if (blocks > 0xFFFFFFull || tiles > 0xFFFFFFFFull) throw Fail{LPK_E_INVALID, "plan step too large"};
+        if (blocks > 0xFFFFFFull || tiles[0] > 0xFFFFFFFFull) throw Fail{LPK_E_INVALID, "plan step too large"};
         P->max_blocks = std::max<uint32_t>(P->max_blocks, uint32_t(blocks));
-        P->max_tiles = std::max<uint32_t>(P->max_tiles, uint32_t(tiles));
+        for (size_t pp = 0; pp < tiles.size(); ++pp)
+            P->pass_tiles[pp] = std::max<uint32_t>(P->pass_tiles[pp], uint32_t(tiles[pp]));
     }
+    // the passes' staging: every band in passes, its largest chunk
+    uint64_t part_total = 0;
+    P->part_off.assign(n_bands, 0);
+    for (uint32_t b = 0; b < n_bands; ++b)
+        if (P->n_pass[b] > 1) {
+            P->part_off[b] = part_total;
+            part_total += P->chunk_cap[b];
+        }
     // device buffers, one set per slot
     for (size_t w = 0; w < nw; ++w) ensure_base(ctx, ctx->rx[P->walk_rx[w]], max_bhi[w]);
     for (int i = 0; i < 2; ++i) {
@@ -1350,6 +1398,7 @@ lpk_plan *plan_create(lpk_ctx *ctx, const lpk_band_chunk *bands, uint32_t n_band
             P->toff[i][w].ensure(cap * size_t(kWalkBlock >> kWalkSubShift) * sizeof(double), ctx->stream);
         }
         P->blocksum[i].ensure(size_t(std::max<uint32_t>(P->max_blocks, 1)) * sizeof(double), ctx->stream);
+        if (part_total) P->part[i].ensure(size_t(part_total) * sizeof(double2), ctx->stream);
         P->outbuf[i].resize(n_bands);
         for (uint32_t b = 0; b < n_bands; ++b)
             if (!P->bands[b].out_dev)
@@ -1357,7 +1406,9 @@ lpk_plan *plan_create(lpk_ctx *ctx, const lpk_band_chunk *bands, uint32_t n_band
     }
     auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
     P->off_jobs = 256;
-    P->off_links = align(P->off_jobs + n_bands * sizeof(JobDev));
+    uint64_t n_entries = 0;
+    for (uint32_t b = 0; b < n_bands; ++b) n_entries += P->n_pass[b];
+    P->off_links = align(P->off_jobs + n_entries * sizeof(JobDev));
     P->off_flags = align(P->off_links + std::max<uint64_t>(P->total_links, 1) * sizeof(LinkDev));
     P->off_segs = align(P->off_flags + std::max<uint64_t>(P->total_links, 1) * sizeof(uint32_t));
     P->off_carry = align(P->off_segs + std::max<size_t>(nw, 1) * sizeof(WalkSegDev));
@@ -1414,13 +1465,18 @@ void plan_step(lpk_plan *P, uint64_t k) {
                   "clock walk carry");
         hip_check(hipEventRecord(P->carry_ev[slot], s), "event record");
     }
-    if (P->max_tiles)
-        hip_check(launch_render(P->form, P->antenna, P->moving, P->tile_shift, P->xcd_map != 0,
-                                reinterpret_cast<const JobDev *>(dev + P->off_jobs), int(P->bands.size()),
-                                reinterpret_cast<const LinkDev *>(dev + P->off_links), ctx->tx_dev, P->max_tiles,
-                                live, nullptr, nullptr, nullptr, nullptr, 0, nullptr, nullptr, ctx->gate != 0, nullptr,
-                                s),
-                  "render launch");
+    // one launch per pass, in order on the slot's stream (pass p + 1 reads
+    // the sums pass p wrote)
+    const JobDev *jobs_dev = reinterpret_cast<const JobDev *>(dev + P->off_jobs);
+    for (size_t pp = 0; pp < P->pass_jobs.size(); ++pp) {
+        if (P->pass_tiles[pp])
+            hip_check(launch_render(P->form, P->antenna, P->moving, P->tile_shift, P->xcd_map != 0, jobs_dev,
+                                    int(P->pass_jobs[pp]), reinterpret_cast<const LinkDev *>(dev + P->off_links),
+                                    ctx->tx_dev, P->pass_tiles[pp], live + kPlanPassHdr + pp, nullptr, nullptr,
+                                    nullptr, nullptr, 0, nullptr, nullptr, ctx->gate != 0, nullptr, s),
+                      "render launch");
+        jobs_dev += P->pass_jobs[pp];
+    }
     timing_end(ctx, s);
     hip_check(hipEventRecord(P->done[slot], s), "event record");
     P->used[slot] = true;
@@ -1726,7 +1782,11 @@ int lpk_plan_step(lpk_plan *plan, uint64_t chunk) {
 int lpk_plan_info(const lpk_plan *plan, uint64_t *n_chunks, uint32_t *max_tiles) {
     if (!plan) return LPK_E_INVALID;
     if (n_chunks) *n_chunks = plan->n_chunks;
-    if (max_tiles) *max_tiles = plan->max_tiles;
+    if (max_tiles) {
+        uint64_t t = 0;
+        for (uint32_t x : plan->pass_tiles) t += x;
+        *max_tiles = uint32_t(std::min<uint64_t>(t, 0xFFFFFFFFull));
+    }
     return LPK_OK;
 }
 

@@ -1119,22 +1119,48 @@ def test_precision_antenna_band_in_link_passes(backend):
     assert got[False][0].tobytes() == g.tobytes() and list(got[False][1]) == list(gflags)
 
 
-def test_plan_refuses_more_than_32_links_in_the_precision_form(backend):
-    """A streaming plan's precision form takes at most 32 transmitters per
-    band (render calls split larger bands into passes): a 48-link plan band
-    is refused before any device work, and the plain form's plan of it runs."""
-    sc = _link_pass_scene(16, 16, 8, 8)
-    seed = 17
+@pytest.mark.parametrize("counts", [(16, 16, 8, 8), (40, 30, 20, 10)])
+def test_plan_in_precision_link_passes(backend, counts):
+    """A streaming plan of a 48- and a 100-link band in the precision form
+    renders each step in passes of at most 32 links: every step's chunk
+    against the oracle within the form's bounds, the plan run twice (the
+    second restart reuses both slots' staging), with a 12-link band of the
+    same receiver in the plan that renders in one pass."""
+    sc = _link_pass_scene(*counts)
+    n_links = sum(counts)
+    seed = 23
     backend.upload(sc, seed)
+    orc = OracleScene(sc, seed)
+    o, _, _ = orc.render_band(0)
+    peak = float(np.max(np.abs(o)))
     n = sc.receivers[0].bands[0].n
     j, keep = backend.job(0, 0, 0, seed, 0, n)
-    with pytest.raises(_lib.LpkError) as e:
-        backend.plan([j], 8192, _lib.FORM_PRECISION)
-    assert _lib.STATUS[e.value.code] == "LPK_E_INVALID"
-    with backend.plan([j], 8192, _lib.FORM_PLAIN) as plan:
-        for kk in range(plan.n_chunks):
-            plan.step(kk)
+    j12, keep12 = backend.job(0, 0, 0, seed, 0, n)
+    j12.n_active = 12      # the first 12 active links: its own band in the plan
+    chunk = 6000
+    ref12 = np.empty(n, dtype=np.complex64)   # eager renders of the same chunks
+    for i0 in range(0, n, chunk):
+        jr, keep_r = backend.job(0, 0, 0, seed, i0, min(chunk, n - i0), out_host=ref12[i0:])
+        jr.n_active = 12
+        backend.render([jr], _lib.FORM_PRECISION)
         backend.sync()
+    lib = backend.lib
+    with backend.plan([j, j12], chunk, _lib.FORM_PRECISION) as plan:
+        for rnd in range(2):
+            got, got12 = np.zeros(n, dtype=np.complex64), np.zeros(n, dtype=np.complex64)
+            for kk in range(plan.n_chunks):
+                plan.step(kk)
+                backend.sync()
+                for band, dst in ((0, got), (1, got12)):
+                    i0, clen = plan.chunk_range(band, kk)
+                    if clen:
+                        backend.check(lib.lpk_copy_to_host(backend.h, dst[i0:].ctypes.data, plan.output(band),
+                                                           clen * 8))
+            d = np.abs(got.astype(np.complex128) - o.astype(np.complex128))
+            _say(f"plan of {n_links} links in passes, run {rnd}: max|d|/peak={d.max() / peak:.3e}")
+            assert np.all(np.isfinite(got)) and d.max() <= 2e-6 * peak
+            assert np.array_equal(got12, ref12)
+    assert plan.max_tiles > 0
 
 
 @pytest.mark.skip(reason="the first precision render of this 72-link band in one pass (one tile per anchor batch, "
