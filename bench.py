@@ -110,6 +110,9 @@ class Scene:
         self.be.close()
 
 
+FIXED_TUNING = None   # (tile_shift, xcd_map) from --tuning, else autotuned per form
+
+
 def time_form(S, name, steps, warmup):
     """Autotunes the form on chunk 0 of every band, then times `steps` whole
     renders (clock walks regenerated from index 0 each step). Returns wall ms
@@ -119,7 +122,11 @@ def time_form(S, name, steps, warmup):
     be, lib = S.be, S.lib
     form = form_code(name)
     first = [j for j in S.jobs if j.i0 == 0]
-    tuning, tune_ms = be.autotune(first, form)
+    if FIXED_TUNING:
+        be.set_tuning(*FIXED_TUNING)
+        tuning, tune_ms = FIXED_TUNING, 0.0
+    else:
+        tuning, tune_ms = be.autotune(first, form)
 
     def step():
         be.check(lib.lpk_clock_rewind(be.h))
@@ -469,12 +476,17 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines and the oracle error checks")
+    ap.add_argument("--tuning", type=str, default="",
+                    help="tile_shift,xcd_map for every form instead of the autotuner's pick")
     ap.add_argument("--headline-only", action="store_true",
                     help="configuration 4's precision form only (for profiler runs)")
     ap.add_argument("--plan-only", action="store_true",
                     help="configuration 4's precision form tuned, then --steps plan passes in 1 Mi-sample "
                          "chunks (for profiler runs)")
     args = ap.parse_args()
+    if args.tuning:
+        global FIXED_TUNING
+        FIXED_TUNING = tuple(int(v) for v in args.tuning.split(","))
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
               f"(the path shards by receiver with no collective; see the design notes).", file=sys.stderr)
@@ -506,7 +518,10 @@ def main():
         return
     forms4["plain"] = time_form(S4, "plain", few, 1)
     prec["speedup_vs_plain"] = round(forms4["plain"]["ms_per_step"] / prec["ms_per_step"], 3)
-    S4.be.autotune([j for j in S4.jobs if j.i0 == 0], 1)
+    if FIXED_TUNING:
+        S4.be.set_tuning(*FIXED_TUNING)
+    else:
+        S4.be.autotune([j for j in S4.jobs if j.i0 == 0], 1)
     plans = {c: plan_leg(S4, "precision", few, 1, c) for c in (1 << 20, 1 << 24)}
     for p in plans.values():
         p["speedup_vs_plain"] = round(forms4["plain"]["ms_per_step"] / p["ms_per_step"], 3)
