@@ -267,8 +267,8 @@ typedef struct lpk_plan lpk_plan;
 int lpk_plan_create(lpk_ctx *ctx, const lpk_band_chunk *bands, uint32_t n_bands, uint64_t chunk_samples,
                     int form, lpk_plan **out);
 int lpk_plan_step(lpk_plan *plan, uint64_t chunk);     /* asynchronous, like lpk_render_chunk */
-/* n_chunks: the plan's steps; max_tiles: the workgroups its largest step
- * launches, over all precision passes */
+/* n_chunks: the plan's steps; max_tiles: the render workgroups one step
+ * launches at most (with precision passes, the sum of each pass's largest) */
 int lpk_plan_info(const lpk_plan *plan, uint64_t *n_chunks, uint32_t *max_tiles);
 int lpk_plan_chunk_range(const lpk_plan *plan, uint32_t band, uint64_t chunk, uint64_t *i0, uint64_t *clen);
 int lpk_plan_output(const lpk_plan *plan, uint32_t band, void **chunk_dev);
