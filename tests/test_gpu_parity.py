@@ -1017,10 +1017,10 @@ def test_plain_form_with_72_links(backend):
     assert compare_iq(g, o, "72 links, plain, oracle clock offsets", strict=False) > 0.5
 
 
-def _link_pass_scene(n_ca, n_bpsk, n_cw, n_gated, duration_s=0.01):
+def _link_pass_scene(n_ca, n_bpsk, n_cw, n_gated, duration_s=0.01, rx_orbit=None):
     """One 2.048 MHz L1 band with the given numbers of C/A and BPSK
     transmitters on circular orbits and CW and burst-gated CW tones on the
-    ground, all in band."""
+    ground, all in band; the receiver in the lab, or on `rx_orbit`."""
     from lipika_amd.config import (Bpsk, CaCode, CircularKeplerian, Cw, GateConfig, ReceiverBandConfig, SceneConfig,
                                    SceneMeta, StationaryGeodetic, TransmitterConfig)
     txs = []
@@ -1038,8 +1038,10 @@ def _link_pass_scene(n_ca, n_bpsk, n_cw, n_gated, duration_s=0.01):
                                      StationaryGeodetic(53.0 + 0.3 * k, -3.0 + k, 300.0), -20.0,
                                      gate=GateConfig(0.004, [(0.0002 * k, 0.0015)], 0.0)))
     band = ReceiverBandConfig("l1", 1575.42e6, 2.048e6, noise_power_w=0.0)
-    sc = prepare(SceneConfig(SceneMeta(util.EPOCH, duration_s), txs,
-                             [util.lab_receiver([band], NodeClockSpec(allan_deviation_1s=1e-11))]))
+    rx = util.lab_receiver([band], NodeClockSpec(allan_deviation_1s=1e-11))
+    if rx_orbit is not None:
+        rx = util.lab_receiver([band], NodeClockSpec(allan_deviation_1s=1e-11), pos=rx_orbit)
+    sc = prepare(SceneConfig(SceneMeta(util.EPOCH, duration_s), txs, [rx]))
     assert len(sc.receivers[0].bands[0].active) == n_ca + n_bpsk + n_cw + n_gated
     return sc
 
@@ -1066,6 +1068,41 @@ def test_precision_form_in_link_passes(backend, counts):
     for slot in slots:
         compare_taps(backend, orc, sc, 0, 0, 0, seed, slot, 20_480, form=_lib.FORM_PRECISION,
                      label=f"{n} links in passes")
+
+
+def test_orbiting_receiver_in_link_passes(backend):
+    """A receiver on a 400 km orbit with 48 in-band transmitters in the
+    precision form's moving-receiver kernel, in two passes: a render call at
+    two chunkings and a streaming plan, against the oracle within the form's
+    bounds, and taps either side of the pass boundary."""
+    from lipika_amd.config import CircularKeplerian
+    sc = _link_pass_scene(16, 16, 8, 8, rx_orbit=CircularKeplerian(6_778_137.0, 70.0, 320.0, 50.0, util.EPOCH))
+    seed = 37
+    backend.upload(sc, seed)
+    orc = OracleScene(sc, seed)
+    o, _, _ = orc.render_band(0)
+    peak = float(np.max(np.abs(o)))
+    n = sc.receivers[0].bands[0].n
+    for chunk in (20_480, 7_000):
+        g, _ = gpu_band(backend, sc, 0, 0, 0, seed, form=_lib.FORM_PRECISION, chunk=chunk)
+        d = np.abs(g.astype(np.complex128) - o.astype(np.complex128))
+        _say(f"orbiting receiver, 48 links in passes, chunks of {chunk}: max|d|/peak={d.max() / peak:.3e}")
+        assert np.all(np.isfinite(g)) and d.max() <= 2e-6 * peak
+    j, keep = backend.job(0, 0, 0, seed, 0, n)
+    got = np.zeros(n, dtype=np.complex64)
+    with backend.plan([j], 6000, _lib.FORM_PRECISION) as plan:
+        for kk in range(plan.n_chunks):
+            plan.step(kk)
+            backend.sync()
+            i0, clen = plan.chunk_range(0, kk)
+            if clen:
+                backend.check(backend.lib.lpk_copy_to_host(backend.h, got[i0:].ctypes.data, plan.output(0), clen * 8))
+    d = np.abs(got.astype(np.complex128) - o.astype(np.complex128))
+    _say(f"orbiting receiver, plan of 48 links in passes: max|d|/peak={d.max() / peak:.3e}")
+    assert np.all(np.isfinite(got)) and d.max() <= 2e-6 * peak
+    for slot in (0, 31, 32, 47):
+        compare_taps(backend, orc, sc, 0, 0, 0, seed, slot, 20_480, form=_lib.FORM_PRECISION,
+                     label="orbiting receiver, 48 links in passes")
 
 
 def _antenna_pass_scene(duration_s=0.01):
