@@ -1017,7 +1017,7 @@ def test_plain_form_with_72_links(backend):
     assert compare_iq(g, o, "72 links, plain, oracle clock offsets", strict=False) > 0.5
 
 
-def _link_pass_scene(n_ca, n_bpsk, n_cw, n_gated, duration_s=0.01, rx_orbit=None):
+def _link_pass_scene(n_ca, n_bpsk, n_cw, n_gated, duration_s=0.01, rx_orbit=None, noise_power_w=0.0, frontend=None):
     """One 2.048 MHz L1 band with the given numbers of C/A and BPSK
     transmitters on circular orbits and CW and burst-gated CW tones on the
     ground, all in band; the receiver in the lab, or on `rx_orbit`."""
@@ -1037,7 +1037,8 @@ def _link_pass_scene(n_ca, n_bpsk, n_cw, n_gated, duration_s=0.01, rx_orbit=None
         txs.append(TransmitterConfig(f"gated{k}", 1575.42e6 - 5e3 * k, Cw(130.0 * k),
                                      StationaryGeodetic(53.0 + 0.3 * k, -3.0 + k, 300.0), -20.0,
                                      gate=GateConfig(0.004, [(0.0002 * k, 0.0015)], 0.0)))
-    band = ReceiverBandConfig("l1", 1575.42e6, 2.048e6, noise_power_w=0.0)
+    band = ReceiverBandConfig("l1", 1575.42e6, 2.048e6, noise_power_w=noise_power_w,
+                              **({"frontend": frontend} if frontend is not None else {}))
     rx = util.lab_receiver([band], NodeClockSpec(allan_deviation_1s=1e-11))
     if rx_orbit is not None:
         rx = util.lab_receiver([band], NodeClockSpec(allan_deviation_1s=1e-11), pos=rx_orbit)
@@ -1068,6 +1069,29 @@ def test_precision_form_in_link_passes(backend, counts):
     for slot in slots:
         compare_taps(backend, orc, sc, 0, 0, 0, seed, slot, 20_480, form=_lib.FORM_PRECISION,
                      label=f"{n} links in passes")
+
+
+def test_link_passes_with_noise_and_front_end(backend):
+    """48 links in passes with kTB noise, gain, IP3 and a 12-bit ADC, which
+    only the last pass applies (src/channel/mod.rs:310-334, src/rx/mod.rs:83-101):
+    IQ within one ADC step plus the form's bound of the oracle's, ADC codes
+    within one step of the oracle's and equal for at least 99 % of them."""
+    fe = FrontendConfig(gain_db=130.0, nonlinearity=NonlinearityConfig(30.0), adc=AdcConfig(12, 1.0))
+    sc = _link_pass_scene(16, 16, 8, 8, noise_power_w=None, frontend=fe)
+    seed = 43
+    backend.upload(sc, seed)
+    orc = OracleScene(sc, seed)
+    o, oc, _ = orc.render_band(0, codes=True)
+    lsb = 1.0 / 2047.0
+    assert oc.min() < -1000 and oc.max() > 1000     # the ADC is exercised
+    for chunk in (20_480, 7_000):
+        g, gc = gpu_band(backend, sc, 0, 0, 0, seed, form=_lib.FORM_PRECISION, chunk=chunk, codes=True)
+        d = np.abs(g.astype(np.complex128) - o.astype(np.complex128))
+        diff = np.abs(gc.astype(int) - oc.astype(int))
+        _say(f"48 links in passes with noise and front end, chunks of {chunk}: max|d|={d.max():.3e} "
+             f"(lsb {lsb:.3e}), codes equal {np.mean(diff == 0):.5f}, max code step {diff.max()}")
+        assert np.all(np.isfinite(g)) and d.max() <= 1.5 * lsb
+        assert diff.max() <= 1 and np.mean(diff == 0) >= 0.99
 
 
 def test_orbiting_receiver_in_link_passes(backend):
