@@ -1143,6 +1143,38 @@ def _antenna_pass_scene(duration_s=0.01):
     return SceneConfig(SceneMeta(util.EPOCH, duration_s), txs, [util.lab_receiver([band])])
 
 
+def test_plan_of_an_antenna_band_in_link_passes(backend):
+    """The 40-source antenna band as a streaming plan in the precision form:
+    each step's two passes through the antenna kernel, against the oracle
+    within the form's bounds, with the horizon gate on and off giving the same
+    bytes."""
+    sc = prepare(_antenna_pass_scene())
+    seed = 31
+    backend.upload(sc, seed)
+    o, _, _ = OracleScene(sc, seed).render_band(0)
+    peak = float(np.max(np.abs(o)))
+    n = sc.receivers[0].bands[0].n
+    got = {}
+    for gate in (True, False):
+        backend.set_horizon_gate(gate)
+        g = np.zeros(n, dtype=np.complex64)
+        j, keep = backend.job(0, 0, 0, seed, 0, n)
+        with backend.plan([j], 6000, _lib.FORM_PRECISION) as plan:
+            for kk in range(plan.n_chunks):
+                plan.step(kk)
+                backend.sync()
+                i0, clen = plan.chunk_range(0, kk)
+                if clen:
+                    backend.check(backend.lib.lpk_copy_to_host(backend.h, g[i0:].ctypes.data, plan.output(0),
+                                                               clen * 8))
+        got[gate] = g
+    backend.set_horizon_gate(True)
+    d = np.abs(got[True].astype(np.complex128) - o.astype(np.complex128))
+    _say(f"plan of the antenna band in passes: max|d|/peak={d.max() / peak:.3e}")
+    assert np.all(np.isfinite(got[True])) and d.max() <= 2e-6 * peak
+    assert got[True].tobytes() == got[False].tobytes()
+
+
 def test_precision_antenna_band_in_link_passes(backend):
     """An antenna band of 40 MEO sources, about a third above its cutoff, in
     the precision form's antenna kernel in two passes: IQ against the oracle
