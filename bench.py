@@ -77,7 +77,8 @@ class Scene:
         from lipika_amd import _lib, workloads
         from lipika_amd.pipeline import Backend, prepare
         sc = (workloads.micro() if n == "micro" else workloads.scene3_orbiting() if n == "3 orbiting"
-              else workloads.scene(3, 10.0) if n == "3 10 s" else workloads.scene(n))
+              else workloads.scene(3, 10.0) if n == "3 10 s" else workloads.scene3_wide() if n == "3 wide"
+              else workloads.scene(n))
         if not noise:
             for rx in sc.receivers:
                 for b in rx.bands:
@@ -440,6 +441,25 @@ def moving_leg(steps, warmup):
     return out
 
 
+def passes_leg(steps, warmup, moving):
+    """The precision form in two passes per band: configuration 3's receivers
+    and bands for 10 s under 64 C/A MEOs (workloads.scene3_wide), every chunk
+    in one render call, beside the same bands under the 32 of configuration 3
+    (one pass, the moving leg's fixed receivers): the passes' cost per
+    contribution."""
+    S = Scene("3 wide")
+    r = time_form(S, "precision", steps, warmup)
+    one = moving["fixed"]["forms"]["precision"]
+    out = {"workload": "configuration 3's 16 receivers and L1 bands at 2.048 MHz, 10 s, under 64 C/A MEOs "
+                       "(two passes of 32 per band)",
+           "contributions_per_step": S.contributions, "precision": r,
+           "one_pass_value": one["value"],
+           "per_contribution_vs_one_pass": round(r["value"] / one["value"], 4)}
+    S.close()
+    log(f"[bench] link passes: {json.dumps(out)}")
+    return out
+
+
 def roofline(contributions, kernel_ms, n):
     achieved = contributions * FLOPS_PER_CONTRIBUTION / (kernel_ms / 1e3) / 1e12
     traffic, source = TRAFFIC.get(n, (None, None))
@@ -571,6 +591,7 @@ def main():
     res["config"]["dropin"] = dropin_leg(2)
     res["config"]["micro"] = micro_leg(K, W, not args.no_cpu)
     res["config"]["moving"] = moving_leg(K, W)
+    res["config"]["link_passes"] = passes_leg(K, W, res["config"]["moving"])
     res["config"]["config5"] = learned_leg(max(1, K // 10), 1)
     print(json.dumps(res), flush=True)
 

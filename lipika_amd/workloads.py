@@ -157,6 +157,21 @@ def scene3_orbiting(duration_s: float = 10.0, n_rx: int = 16) -> SceneConfig:
     return sc
 
 
+def scene3_wide(duration_s: float = 10.0, n_rx: int = 16) -> SceneConfig:
+    """Configuration 3's receivers and bands under a sky of 64 C/A MEOs: its
+    32 and 32 more (PRNs 1-32 again, planes turned by 30 deg, names of their
+    own), so that every band holds 64 transmitters: the precision form's
+    two-pass workload."""
+    sc = scene3(duration_s, n_rx)
+    rng = np.random.default_rng(SEED + 64)
+    extra = _meo_constellation(rng)
+    for t in extra:
+        t.name = t.name.replace("gps_prn", "gps_b_prn")
+        t.trajectory = _kepler(MEO_A, 55.0, t.trajectory.raan_deg + 30.0, float(rng.uniform(0.0, 360.0)))
+    sc.transmitters = sc.transmitters + extra
+    return sc
+
+
 def scene(n: int, duration_s: Optional[float] = None, **kw) -> SceneConfig:
     f = {1: scene1, 2: scene2, 3: scene3, 4: scene4, 5: scene5}[n]
     return f(**kw) if duration_s is None else f(duration_s, **kw)
